@@ -1,0 +1,175 @@
+"""Writes the golden fixtures under tests/golden/ from runs of the reference solver.
+
+The reference binary is compiled from the reference sources by ``oracle/Makefile``
+(target ``ref``) into ``oracle/_ref/Olliptic.x``.  This script runs it on the
+configurations below, in a scratch directory, and keeps only data:
+
+* ``<name>.json``  - command line, per-level residual history (``Loo.gp``),
+  the final puncture ADM masses and per-box surface ADM masses (stdout), the
+  box table (level, box, n, dx, origin), cycle count;
+* ``<name>.npz``   - converged ``u`` of every box in plain lexicographic order
+  (``i`` fastest), float64, parsed from the reference's ``-p3d yes`` output
+  (12 significant digits), optionally subsampled by ``stride`` per axis.
+
+Usage: python tests/golden/make_golden.py [name ...]
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF_BIN = os.path.join(REPO, "oracle", "_ref", "Olliptic.x")
+
+PUNC = ["-eq", "Punctures", "-B", "Robin", "-BA", "0", "-Bn", "1"]
+CFG1 = PUNC + ["-NumBH", "1", "-bhmp1", "1.0", "-bhpx1", "0.2",
+               "-L", "256", "-N", "33", "-l", "4", "-O", "2", "-nu2", "10"]
+CFG2 = PUNC + ["-NumBH", "2", "-bhmp1", "0.487209", "-bhy1", "5.5",
+               "-bhpx1", "-0.0901099", "-bhpy1", "-0.000703975",
+               "-bhmp2", "0.487209", "-bhy2", "-5.5",
+               "-bhpx2", "0.0901099", "-bhpy2", "0.000703975",
+               "-L", "1000", "-N", "65", "-l", "9", "-O", "4", "-nu2", "20"]
+
+# name -> (argv, stride for the stored 3-D field)
+CASES = {
+    "cfg1_tol": (CFG1 + ["-s", "1e-10"], 2),
+    "cfg1_conv": (CFG1 + ["-s", "1e-16"], 1),
+    "cfg2_tol": (CFG2 + ["-s", "1e-11"], 4),
+    "cfg2_conv": (CFG2 + ["-s", "1e-16"], 4),
+    # small cases covering orders 4/6/8, off-lattice punctures, spins,
+    # box merging and the scalar-field equation
+    "small_o4_offgrid": (PUNC + [
+        "-NumBH", "1", "-bhmp1", "0.5", "-bhx1", "0.37", "-bhy1", "-0.81",
+        "-bhz1", "1.23", "-bhpx1", "0.1", "-bhpy1", "-0.05", "-bhpz1", "0.07",
+        "-bhsx1", "0.03", "-bhsy1", "0.02", "-bhsz1", "-0.04",
+        "-L", "40", "-N", "25", "-l", "3", "-O", "4", "-nu2", "6",
+        "-s", "1e-15"], 1),
+    "small_o6_merge": (PUNC + [
+        "-NumBH", "2", "-bhmp1", "0.4", "-bhx1", "1.1", "-bhy1", "0.3",
+        "-bhz1", "-0.2", "-bhpx1", "0.05", "-bhpy1", "0.12",
+        "-bhsz1", "0.05", "-bhmp2", "0.35", "-bhx2", "-1.3", "-bhy2", "-0.4",
+        "-bhz2", "0.5", "-bhpx2", "-0.05", "-bhpz2", "-0.1", "-bhsx2", "0.04",
+        "-L", "48", "-N", "25", "-l", "4", "-O", "6", "-nu2", "8",
+        "-s", "1e-15"], 1),
+    "small_o8": (PUNC + [
+        "-NumBH", "1", "-bhmp1", "0.8", "-bhx1", "0.5", "-bhpy1", "0.15",
+        "-bhsz1", "0.1", "-L", "50", "-N", "25", "-l", "3", "-O", "8",
+        "-nu2", "6", "-s", "1e-15"], 1),
+    "small_o2_4bh": (PUNC + [
+        "-NumBH", "4", "-bhmp1", "0.3", "-bhx1", "2.0", "-bhy1", "2.0",
+        "-bhmp2", "0.35", "-bhx2", "-2.0", "-bhy2", "1.5", "-bhz2", "0.5",
+        "-bhmp3", "0.4", "-bhx3", "1.0", "-bhy3", "-2.5", "-bhpx3", "0.1",
+        "-bhmp4", "0.45", "-bhx4", "-1.5", "-bhy4", "-1.0", "-bhz4", "-1.0",
+        "-bhpy4", "-0.1", "-bhsz4", "0.1",
+        "-L", "32", "-N", "25", "-l", "4", "-O", "2", "-nu2", "8",
+        "-s", "1e-15"], 1),
+    "small_sf_3bh": ([
+        "-eq", "Punctures_scalar_field", "-B", "Robin", "-BA", "0", "-Bn", "1",
+        "-psc", "2", "-phi0", "0.02", "-a2", "1", "-r0", "10", "-sigma", "4",
+        "-NumBH", "3", "-bhmp1", "0.4", "-bhx1", "3.0", "-bhy1", "1.0",
+        "-bhmp2", "0.3", "-bhx2", "-2.5", "-bhy2", "2.0", "-bhz2", "-1.0",
+        "-bhmp3", "0.3", "-bhx3", "0.5", "-bhy3", "-3.0", "-bhz3", "1.5",
+        "-bhpx1", "0.05", "-bhpy2", "-0.04", "-bhsz3", "0.03",
+        "-L", "64", "-N", "25", "-l", "3", "-O", "4", "-nu2", "8",
+        "-s", "1e-15"], 1),
+}
+
+BOX_RE = re.compile(r"level: (\d+) \| box: (\d+)")
+N_RE = re.compile(r"nx = (\d+),\s+ny = (\d+),\s+nz = (\d+)")
+ADM_RE = re.compile(r"ADM mass\[(\d+)\] = (\S+)")
+SURF_RE = re.compile(r"Level (\d+) box (\d+) r0= (\S+) AMDmass= (\S+)")
+
+
+def parse_stdout(text):
+    boxes = []
+    lines = text.splitlines()
+    for i, ln in enumerate(lines):
+        m = BOX_RE.search(ln)
+        if m and i + 2 < len(lines):
+            mn = N_RE.search(lines[i + 2])
+            if mn:
+                boxes.append({"level": int(m.group(1)), "box": int(m.group(2)),
+                              "n": [int(mn.group(k)) for k in (1, 2, 3)]})
+    adm = {}
+    surf = {}
+    for ln in lines:
+        m = ADM_RE.search(ln)
+        if m:
+            adm[int(m.group(1))] = float(m.group(2))
+        m = SURF_RE.search(ln)
+        if m:
+            surf[(int(m.group(1)), int(m.group(2)))] = float(m.group(4))
+    return boxes, [adm[k] for k in sorted(adm)], surf
+
+
+def parse_history(path):
+    hist = []
+    with open(path) as f:
+        for ln in f:
+            c, v = ln.split()
+            hist.append([int(c), float(v)])
+    return hist
+
+
+def parse_xyz(path):
+    a = np.loadtxt(path)
+    xs, ys, zs = (np.unique(np.round(a[:, d], 9)) for d in range(3))
+    nx, ny, nz = len(xs), len(ys), len(zs)
+    x0, y0, z0 = a[:, 0].min(), a[:, 1].min(), a[:, 2].min()
+    dx = (a[:, 0].max() - x0) / (nx - 1)
+    dy = (a[:, 1].max() - y0) / (ny - 1)
+    dz = (a[:, 2].max() - z0) / (nz - 1)
+    i = np.rint((a[:, 0] - x0) / dx).astype(int)
+    j = np.rint((a[:, 1] - y0) / dy).astype(int)
+    k = np.rint((a[:, 2] - z0) / dz).astype(int)
+    u = np.full((nz, ny, nx), np.nan)
+    u[k, j, i] = a[:, 3]
+    assert not np.isnan(u).any(), path
+    return u, [x0, y0, z0], [dx, dy, dz]
+
+
+def run_case(name):
+    argv, stride = CASES[name]
+    with tempfile.TemporaryDirectory() as tmp:
+        cmd = [REF_BIN] + argv + ["-o", "run", "-p3d", "yes", "-p1d", "no",
+                                  "-pint", "no"]
+        t0 = time.time()
+        out = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True)
+        wall = time.time() - t0
+        boxes, adm, surf = parse_stdout(out.stdout)
+        hist = parse_history(os.path.join(tmp, "run", "Loo.gp"))
+        arrays = {}
+        for b in boxes:
+            lv, bx = b["level"], b["box"]
+            u, org, d = parse_xyz(os.path.join(tmp, "run", "u-l%db%d-0.xyz" % (lv, bx)))
+            assert list(u.shape[::-1]) == b["n"], (name, b, u.shape)
+            b["origin"] = org
+            b["dx"] = d
+            b["surface_adm"] = surf.get((lv, bx))
+            arrays["u_l%d_b%d" % (lv, bx)] = u[::stride, ::stride, ::stride].copy()
+        levels = max(b["level"] for b in boxes) + 1
+        top = [v for (c, v), nxt in zip(hist, hist[1:] + [[None, None]])]
+        meta = {
+            "name": name, "argv": argv, "stride": stride,
+            "ref_wall_s": round(wall, 3), "levels": levels, "boxes": boxes,
+            "history": hist, "adm_mass": adm,
+            "cycles": hist[-1][0] + 1 if hist else 0,
+            "note": "reference run; u stored lexicographic (k,j,i), 12 digits",
+        }
+        del top
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("%s: %d cycles, %.1f s" % (name, meta["cycles"], wall), flush=True)
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(CASES)
+    for n in names:
+        run_case(n)
