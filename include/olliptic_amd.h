@@ -1,0 +1,161 @@
+/*
+ * olliptic_amd: MI355X-native FAS multigrid solve of the puncture
+ * Hamiltonian constraint  Lap u + b (c + u)^-7 [+ d (c+u) + e (c+u)^5] = 0
+ * on nested fixed refinement boxes with 2nd-8th order finite differences and
+ * Robin outer boundaries.
+ *
+ * C ABI: plain pointers and sizes only.  One handle (oll_hier) owns one box
+ * hierarchy resident in HBM and one HIP stream; calls on one handle are not
+ * reentrant, distinct handles may be used concurrently from different host
+ * threads.  Every function returns 0 on success and a negative code on
+ * failure; oll_last_error() returns the calling thread's message.
+ *
+ * Each entry point names the reference interface it replaces
+ * (files of WuShichao/OllipticID).
+ */
+#ifndef OLLIPTIC_AMD_H
+#define OLLIPTIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OLL_MAX_BH 16
+
+/* error codes */
+#define OLL_OK 0
+#define OLL_EINVAL (-1)     /* bad argument / configuration */
+#define OLL_EGRID (-2)      /* not enough grid points for multigrid */
+#define OLL_EDEVICE (-3)    /* HIP runtime / kernel failure */
+#define OLL_ENAN (-4)       /* NaN produced by the smoother */
+#define OLL_ENOMEM (-5)     /* device allocation failed */
+
+/* equations (interface.cc:1184-1323 names "Punctures", "Punctures_scalar_field") */
+#define OLL_EQ_PUNCTURES 0
+#define OLL_EQ_PUNCTURES_SCALAR 1
+
+/* arithmetic of the correction cycles */
+#define OLL_PREC_FP64 0
+#define OLL_PREC_MIXED 1
+
+/* field ids (reference field names, operator.cc:103-124,509-567) */
+#define OLL_F_U 0
+#define OLL_F_B 1
+#define OLL_F_C 2
+#define OLL_F_D 3
+#define OLL_F_E 4
+#define OLL_F_RHO 5
+#define OLL_F_R 6
+#define OLL_F_V 7
+#define OLL_F_W 8
+#define OLL_F_WW 9
+#define OLL_NFIELDS 10
+
+/* One puncture: bare mass, position, linear momentum, spin
+ * (reference -bhmp<i> -bhx<i> ... -bhsz<i>, interface.cc:748-864). */
+typedef struct oll_puncture {
+  double m, x, y, z, px, py, pz, sx, sy, sz;
+} oll_puncture;
+
+/* Problem description: the reference's hot-path CLI parameters
+ * (interface.cc:35-355 defaults, elliptic.cc:31-221 Make). */
+typedef struct oll_problem {
+  int eq;               /* OLL_EQ_* (-eq) */
+  int num_bh;           /* -NumBH */
+  oll_puncture bh[OLL_MAX_BH];
+  double L;             /* -L */
+  int N;                /* -N, snapped to the reference's valid sizes */
+  int levels;           /* -l refinement levels */
+  int order;            /* -O 2,4,6,8 */
+  int nu1, nu2;         /* -nu1 -nu2 */
+  double tol;           /* -s */
+  double robin_a;       /* -BA Robin value at infinity (reference: 0) */
+  int robin_n;          /* -Bn fall-off power (reference: 1) */
+  int ps_case;          /* -psc */
+  double ps_r0, ps_sigma, ps_phi0, ps_a2;  /* -r0 -sigma -phi0 -a2 */
+  int max_cycles;       /* 0: reference stopping rule only */
+  int precision;        /* OLL_PREC_* */
+} oll_problem;
+
+typedef struct oll_hier oll_hier;
+
+typedef struct oll_box_desc {
+  int level;            /* 0 = coarsest multigrid level */
+  int order;            /* stencil order of this box (2 on coarsened levels) */
+  int ghost;            /* ghost width R = max(1, order/2) */
+  int n[3];             /* points per axis incl. ghosts */
+  double origin[3];     /* coordinate of index 0 per axis */
+  double d[3];          /* spacing */
+  double centre[3];
+} oll_box_desc;
+
+typedef struct oll_result {
+  int cycles;           /* V-cycles run */
+  double residual;      /* last top-level post-smoothing norm */
+  int hist_len;         /* entries written to history (cycle, norm) pairs */
+} oll_result;
+
+/* Defaults of the reference CLI (interface.cc:35-355). */
+void oll_problem_default(oll_problem *p);
+
+/* Builds the box hierarchy (elliptic.cc:300-921 Set_Grids), allocates it in
+ * HBM and evaluates the coefficients b, c [,d, e] on the device
+ * (operator.cc:629-868 Set_Coefficients, boundary.cc:8-84 Set_Boundary). */
+int oll_hier_create(const oll_problem *p, oll_hier **out);
+void oll_hier_destroy(oll_hier *h);
+const char *oll_last_error(void);
+
+/* Hierarchy shape (multigrid.h:49-551 Get_num_dom / Get_dom_index). */
+int oll_num_levels(const oll_hier *h);
+int oll_mgrid_levels(const oll_hier *h);
+int oll_num_boxes(const oll_hier *h);
+int oll_level_boxes(const oll_hier *h, int level, int *first, int *count);
+int oll_box_info(const oll_hier *h, int ibox, oll_box_desc *out);
+/* Bytes of HBM the hierarchy holds. */
+size_t oll_device_bytes(const oll_hier *h);
+/* Box table of the hierarchy a problem would build, host only (no device
+ * needed): elliptic.cc:300-921 Set_Grids.  Writes min(cap, *nbox) boxes. */
+int oll_problem_boxes(const oll_problem *p, oll_box_desc *out, int cap, int *nbox,
+                      int *mgrid);
+/* Footprint oll_hier_create would allocate, computed on the host only. */
+int oll_problem_footprint(const oll_problem *p, size_t *bytes);
+
+/* Field access, lexicographic (i fastest) float64 (ffunction.h Get_val/Set). */
+int oll_get_field(oll_hier *h, int ibox, int field, double *host);
+int oll_set_field(oll_hier *h, int ibox, int field, const double *host);
+
+/* Single stages, all boxes of one level (for tests and tools). */
+/* elliptic.cc:1126-1242 Smooth (multi-colour order); linf = max |N(u)-rho|. */
+int oll_smooth(oll_hier *h, int level, int nu, double *linf);
+/* elliptic.cc:944-991 App_Ell_Operator: r = N(u) on the interior. */
+int oll_residual(oll_hier *h, int level, double *linf);
+/* boundary.cc:447-800 App_Bound (Robin update of the ghost shell). */
+int oll_robin(oll_hier *h, int level);
+/* boundary.cc:1118-1535 App_Boundary_Operator: r = Robin operator on shell. */
+int oll_boundary_operator(oll_hier *h, int level);
+/* elliptic.cc:1469-1968 FASN(k): one FAS step of level k >= 1 (recursive). */
+int oll_fas(oll_hier *h, int level, double *linf);
+/* elliptic.cc:1251-1342 Vcycle loop; hist (2*cap doubles) may be NULL. */
+int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap);
+/* elliptic.cc:2447-2611 Compute_ADM_Mass: per puncture and per box. */
+int oll_adm_mass(oll_hier *h, double *puncture, double *surface);
+
+/* Device time of the last oll_solve (HIP events on the handle's stream). */
+int oll_last_solve_ms(const oll_hier *h, double *ms);
+/* Restores the initial state u = 0, rho = Robin value on the shell
+ * (boundary.cc:8-84 Set_Boundary) so the same hierarchy can be solved again. */
+int oll_reset(oll_hier *h);
+/* Smoother launch timing: when on, every smoother launch is bracketed by HIP
+ * events on the handle's stream; oll_kernel_timing returns the number of
+ * launches, their summed device time and their algorithmic bytes (40 B per
+ * updated point: u, b, c, rho read and u written; 56 B with d, e). */
+int oll_set_kernel_timing(oll_hier *h, int on);
+int oll_kernel_timing(oll_hier *h, int *launches, double *total_ms, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

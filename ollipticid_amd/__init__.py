@@ -1,0 +1,11 @@
+"""MI355X-native FAS multigrid solver of the puncture Hamiltonian constraint.
+
+The solver is the HIP library libolliptic_amd.so (C ABI in
+include/olliptic_amd.h); this package is its Python binding and serving
+front end.  Build with `python -m ollipticid_amd.build`.
+"""
+from .native import (  # noqa: F401
+    EQ_PUNCTURES, EQ_PUNCTURES_SCALAR, F_B, F_C, F_D, F_E, F_R, F_RHO, F_U, F_V, F_W,
+    F_WW, PREC_FP64, PREC_MIXED, Hierarchy, OllError, Problem, default_problem, load,
+    problem_boxes, problem_footprint, problem_from_argv,
+)

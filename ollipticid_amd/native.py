@@ -1,0 +1,320 @@
+"""ctypes binding of libolliptic_amd.so (the C ABI of include/olliptic_amd.h).
+
+Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback in the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libolliptic_amd.so")
+
+MAX_BH = 16
+EQ_PUNCTURES, EQ_PUNCTURES_SCALAR = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+F_U, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW = range(10)
+NFIELDS = 10
+ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EDEVICE", -4: "ENAN", -5: "ENOMEM"}
+
+
+class Puncture(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("m", "x", "y", "z", "px", "py", "pz", "sx", "sy", "sz")]
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("eq", C.c_int), ("num_bh", C.c_int), ("bh", Puncture * MAX_BH),
+        ("L", C.c_double), ("N", C.c_int), ("levels", C.c_int), ("order", C.c_int),
+        ("nu1", C.c_int), ("nu2", C.c_int), ("tol", C.c_double),
+        ("robin_a", C.c_double), ("robin_n", C.c_int),
+        ("ps_case", C.c_int), ("ps_r0", C.c_double), ("ps_sigma", C.c_double),
+        ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
+        ("max_cycles", C.c_int), ("precision", C.c_int),
+    ]
+
+
+class BoxDesc(C.Structure):
+    _fields_ = [("level", C.c_int), ("order", C.c_int), ("ghost", C.c_int),
+                ("n", C.c_int * 3), ("origin", C.c_double * 3), ("d", C.c_double * 3),
+                ("centre", C.c_double * 3)]
+
+    def as_dict(self):
+        return {"level": self.level, "order": self.order, "R": self.ghost,
+                "n": list(self.n), "origin": list(self.origin), "d": list(self.d),
+                "centre": list(self.centre)}
+
+
+class Result(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("residual", C.c_double), ("hist_len", C.c_int)]
+
+
+# the exported C ABI: name -> (restype, argtypes)
+_vp = C.c_void_p
+_P = C.POINTER(Problem)
+SIGNATURES = {
+    "oll_problem_default": (None, [_P]),
+    "oll_hier_create": (C.c_int, [_P, C.POINTER(_vp)]),
+    "oll_hier_destroy": (None, [_vp]),
+    "oll_last_error": (C.c_char_p, []),
+    "oll_num_levels": (C.c_int, [_vp]),
+    "oll_mgrid_levels": (C.c_int, [_vp]),
+    "oll_num_boxes": (C.c_int, [_vp]),
+    "oll_level_boxes": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "oll_box_info": (C.c_int, [_vp, C.c_int, C.POINTER(BoxDesc)]),
+    "oll_device_bytes": (C.c_size_t, [_vp]),
+    "oll_problem_boxes": (C.c_int, [_P, C.POINTER(BoxDesc), C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
+    "oll_problem_footprint": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    "oll_get_field": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "oll_set_field": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "oll_smooth": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "oll_residual": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
+    "oll_robin": (C.c_int, [_vp, C.c_int]),
+    "oll_boundary_operator": (C.c_int, [_vp, C.c_int]),
+    "oll_fas": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
+    "oll_solve": (C.c_int, [_vp, C.POINTER(Result), C.POINTER(C.c_double), C.c_int]),
+    "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "oll_last_solve_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "oll_reset": (C.c_int, [_vp]),
+    "oll_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "oll_kernel_timing": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library; raises when it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libolliptic_amd.so is not built: run `python -m ollipticid_amd.build` "
+                "(the solver has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class OllError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        raise OllError(rc, load().oll_last_error().decode())
+    return rc
+
+
+def default_problem():
+    p = Problem()
+    load().oll_problem_default(C.byref(p))
+    return p
+
+
+def problem_from_argv(argv):
+    """Maps a reference command line onto Problem with the reference's rules
+    (interface.cc:366-874): flags match exactly, a repeated flag keeps its
+    last value, unknown flags are ignored, nothing is parsed when the line has
+    fewer than two arguments.  -BA/-Bn are honoured (defaults 0 and 1, the
+    values the reference hard-codes for Punctures)."""
+    p = default_problem()
+    a = list(argv)
+    if len(a) < 2:
+        return p
+    eq = None
+    nbh = p.num_bh
+    bh = {}
+    for i, tok in enumerate(a):
+        if i + 1 >= len(a):
+            break
+        v = a[i + 1]
+        if tok in ("-eq", "--equation"):
+            eq = v
+        elif tok in ("-NumBH", "--num-bh"):
+            nbh = int(float(v))
+        elif tok in ("-L", "--length", "-Lx", "--length-x"):
+            p.L = float(v)
+        elif tok in ("-N", "-n"):
+            p.N = int(float(v))
+        elif tok in ("-l", "--levels"):
+            p.levels = int(float(v))
+        elif tok in ("-O", "--order"):
+            p.order = int(float(v))
+        elif tok in ("-nu1", "--pre-smooth"):
+            p.nu1 = int(float(v))
+        elif tok in ("-nu2", "--post-smooth"):
+            p.nu2 = int(float(v))
+        elif tok in ("-s", "--tolerance"):
+            p.tol = float(v)
+        elif tok == "-BA":
+            p.robin_a = float(v)
+        elif tok == "-Bn":
+            p.robin_n = int(float(v))
+        elif tok in ("-psc", "-ps_case"):
+            p.ps_case = int(float(v))
+        elif tok in ("-r0", "--radius_sf_r0"):
+            p.ps_r0 = float(v)
+        elif tok in ("-sigma", "--sigma_sf"):
+            p.ps_sigma = float(v)
+        elif tok in ("-phi0", "--phi0_sf"):
+            p.ps_phi0 = float(v)
+        elif tok in ("-a2", "--a2_sf"):
+            p.ps_a2 = float(v)
+        elif tok.startswith("-bh"):
+            for key in ("mp", "px", "py", "pz", "sx", "sy", "sz", "x", "y", "z"):
+                rest = tok[3 + len(key):]
+                if tok.startswith("-bh" + key) and rest.isdigit():
+                    bh[(key, int(rest) - 1)] = float(v)
+                    break
+    if eq == "Punctures_scalar_field":
+        p.eq = EQ_PUNCTURES_SCALAR
+    elif eq == "Punctures":
+        p.eq = EQ_PUNCTURES
+    elif eq is not None:
+        raise ValueError("equation %r is outside this solver (Punctures, "
+                         "Punctures_scalar_field)" % eq)
+    if not 1 <= nbh <= MAX_BH:
+        raise ValueError("NumBH must be in 1..%d" % MAX_BH)
+    p.num_bh = nbh
+    names = {"mp": "m", "x": "x", "y": "y", "z": "z", "px": "px", "py": "py", "pz": "pz",
+             "sx": "sx", "sy": "sy", "sz": "sz"}
+    for j in range(MAX_BH):
+        for key, field in names.items():
+            setattr(p.bh[j], field, bh.get((key, j), 0.0))
+    return p
+
+
+def problem_boxes(p):
+    """Box table of the hierarchy (host only, no device needed)."""
+    L = load()
+    n, mg = C.c_int(), C.c_int()
+    _check(L.oll_problem_boxes(C.byref(p), None, 0, C.byref(n), C.byref(mg)))
+    arr = (BoxDesc * n.value)()
+    _check(L.oll_problem_boxes(C.byref(p), arr, n.value, C.byref(n), C.byref(mg)))
+    return [b.as_dict() for b in arr], mg.value
+
+
+def problem_footprint(p):
+    out = C.c_size_t()
+    _check(load().oll_problem_footprint(C.byref(p), C.byref(out)))
+    return out.value
+
+
+class Hierarchy:
+    """One box hierarchy resident in HBM (oll_hier)."""
+
+    def __init__(self, problem):
+        self.L = load()
+        self.problem = problem
+        h = _vp()
+        _check(self.L.oll_hier_create(C.byref(problem), C.byref(h)))
+        self.h = h
+        self.nbox = self.L.oll_num_boxes(self.h)
+        self.levels = self.L.oll_num_levels(self.h)
+        self.mgrid = self.L.oll_mgrid_levels(self.h)
+        self._info = [self.box_info(i) for i in range(self.nbox)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oll_hier_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def box_info(self, ib):
+        d = BoxDesc()
+        _check(self.L.oll_box_info(self.h, ib, C.byref(d)))
+        return d.as_dict()
+
+    def boxes(self):
+        return list(self._info)
+
+    def level_boxes(self, level):
+        f, n = C.c_int(), C.c_int()
+        _check(self.L.oll_level_boxes(self.h, level, C.byref(f), C.byref(n)))
+        return list(range(f.value, f.value + n.value))
+
+    def get_field(self, ib, fid):
+        n = self._info[ib]["n"]
+        out = np.empty((n[2], n[1], n[0]), dtype=np.float64)
+        _check(self.L.oll_get_field(self.h, ib, fid, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def set_field(self, ib, fid, arr):
+        n = self._info[ib]["n"]
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(n[2], n[1], n[0])
+        _check(self.L.oll_set_field(self.h, ib, fid, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def smooth(self, level, nu):
+        out = C.c_double()
+        _check(self.L.oll_smooth(self.h, level, nu, C.byref(out)))
+        return out.value
+
+    def residual(self, level):
+        out = C.c_double()
+        _check(self.L.oll_residual(self.h, level, C.byref(out)))
+        return out.value
+
+    def robin(self, level):
+        _check(self.L.oll_robin(self.h, level))
+
+    def boundary_operator(self, level):
+        _check(self.L.oll_boundary_operator(self.h, level))
+
+    def fas(self, level):
+        out = C.c_double()
+        _check(self.L.oll_fas(self.h, level, C.byref(out)))
+        return out.value
+
+    def solve(self, hist_cap=100000):
+        res = Result()
+        hist = (C.c_double * (2 * hist_cap))()
+        _check(self.L.oll_solve(self.h, C.byref(res), hist, hist_cap))
+        h = np.array(hist[: 2 * min(res.hist_len, hist_cap)]).reshape(-1, 2)
+        return res.cycles, h, res.residual
+
+    def last_solve_ms(self):
+        out = C.c_double()
+        _check(self.L.oll_last_solve_ms(self.h, C.byref(out)))
+        return out.value
+
+    def adm_mass(self):
+        nbh = self.problem.num_bh
+        p = (C.c_double * nbh)()
+        s = (C.c_double * self.nbox)()
+        _check(self.L.oll_adm_mass(self.h, p, s))
+        return list(p), list(s)
+
+    def reset(self):
+        _check(self.L.oll_reset(self.h))
+
+    def set_kernel_timing(self, on):
+        _check(self.L.oll_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        n, ms, by = C.c_int(), C.c_double(), C.c_double()
+        _check(self.L.oll_kernel_timing(self.h, C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
+
+    def device_bytes(self):
+        return self.L.oll_device_bytes(self.h)

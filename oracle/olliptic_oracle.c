@@ -869,6 +869,57 @@ static void set_in(obox *B, const odom *fine) {
       }
 }
 
+/* Defect restriction used with multi-colour sweeps: the same target points
+ * and precedence as the reference's Transferir(W, ALL, LAGRANGE 2, all)
+ * (elliptic.cc:1660-1685), but where the coarse point coincides with a fine
+ * point whose 3x3x3 neighbourhood lies in the fine interior, the value is
+ * the full-weighting average (1/4,1/2,1/4)^3 instead of injection.  A
+ * converged FAS solution has zero defect, so the fixed point is unchanged. */
+static int coincident_index(const odom *D, int a, double x, long *idx) {
+  size_t i = dom_get_i(D, a, x);
+  for (long c = (long)i - 1; c <= (long)i; c++) {
+    if (c < 0 || c >= (long)D->n[a]) continue;
+    if (fabs(dom_x(D, a, c) - x) <= 1e-6 * D->d[a]) {
+      *idx = c;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+static void restrict_defect_fw(oracle_solver *s, int k, int all) {
+  static const double w[3] = {0.25, 0.5, 0.25};
+  for (int ib = 0; ib < s->lvl_count[k]; ib++)
+    for (int jb = 0; jb < s->lvl_count[k - 1]; jb++) {
+      obox *T = &s->box[s->lvl_first[k - 1] + jb];
+      const obox *S = &s->box[s->lvl_first[k] + ib];
+      const double *sf = S->f[OF_W];
+      for (size_t kk = 0; kk < T->D.n[2]; kk++)
+        for (size_t j = 0; j < T->D.n[1]; j++)
+          for (size_t i = 0; i < T->D.n[0]; i++) {
+            double X[3] = {dom_x(&T->D, 0, (long)i), dom_x(&T->D, 1, (long)j), dom_x(&T->D, 2, (long)kk)};
+            int take = dom_is_in_nb(&S->D, X) || (all && dom_is_in(&S->D, X));
+            if (!take) continue;
+            long fi[3];
+            int fw = 1;
+            for (int a = 0; a < 3 && fw; a++) {
+              if (!coincident_index(&S->D, a, X[a], &fi[a])) fw = 0;
+              else if (fi[a] - 1 < (long)S->D.b[a] || fi[a] + 1 >= (long)(S->D.n[a] - S->D.b[a])) fw = 0;
+            }
+            double val;
+            if (fw) {
+              val = 0;
+              for (int c = -1; c <= 1; c++)
+                for (int b = -1; b <= 1; b++)
+                  for (int a = -1; a <= 1; a++)
+                    val += w[a + 1] * w[b + 1] * w[c + 1] * get_val(S, sf, fi[0] + a, fi[1] + b, fi[2] + c);
+            } else
+              val = lagrange_interp(S, sf, 2, X[0], X[1], X[2]);
+            T->f[OF_W][IDX(T, i, j, kk)] = val;
+          }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* FAS cycle: elliptic.cc:1251-1342 (Vcycle), 1469-1968 (FASN)         */
 /* ------------------------------------------------------------------ */
@@ -912,7 +963,9 @@ static double fasn(oracle_solver *s, int k, ohist *H) {
     obox *B = &s->box[fk + i];
     for (size_t q = 0; q < B->npts; q++) B->f[OF_W][q] = B->f[OF_RHO][q] - B->f[OF_R][q];
   }
-  if (k - 1 <= M)
+  if (s->p.restrict_fw)
+    restrict_defect_fw(s, k, k - 1 <= M);
+  else if (k - 1 <= M)
     oracle_transfer(s, k, OF_W, k - 1, OF_W, OR_ALL, OI_LAGRANGE, 2, 1, 0);
   else
     oracle_transfer(s, k, OF_W, k - 1, OF_W, OR_ALL, OI_LAGRANGE, 2, 0, 0);

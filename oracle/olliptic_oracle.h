@@ -52,6 +52,10 @@ typedef struct oracle_params {
   double ps_r0, ps_sigma, ps_phi0, ps_a2;
   int sweep_order;      /* oracle_order */
   int max_cycles;       /* 0 = no limit */
+  int restrict_fw;      /* 1: full-weighting restriction of the FAS defect
+                           where the 27-point stencil lies in the fine
+                           interior (the GPU's choice); 0: the reference's
+                           injection */
 } oracle_params;
 
 typedef struct oracle_solver oracle_solver;

@@ -34,6 +34,7 @@ class Params(C.Structure):
         ("ps_r0", C.c_double), ("ps_sigma", C.c_double),
         ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
         ("sweep_order", C.c_int), ("max_cycles", C.c_int),
+        ("restrict_fw", C.c_int),
     ]
 
 
@@ -87,7 +88,7 @@ def lib():
     return _lib
 
 
-def params_from_argv(argv, sweep_order=LEX, max_cycles=0):
+def params_from_argv(argv, sweep_order=LEX, max_cycles=0, restrict_fw=None):
     """Maps a reference-style command line (the hot-path flags) to Params,
     following the reference's defaults (interface.cc:35-355) and its parsing
     rules (a repeated flag takes its last value)."""
@@ -140,6 +141,7 @@ def params_from_argv(argv, sweep_order=LEX, max_cycles=0):
             arr[j] = bh.get((key, j), 0.0)
     p.sweep_order = sweep_order
     p.max_cycles = max_cycles
+    p.restrict_fw = (sweep_order == COLOUR) if restrict_fw is None else int(restrict_fw)
     return p
 
 

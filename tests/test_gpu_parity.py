@@ -1,0 +1,179 @@
+"""GPU parity: every HIP stage against the oracle on the same inputs, through
+the C ABI.
+
+Stage tests compare with the oracle in multi-colour order (the GPU's order;
+within one colour the result is order independent), so the only differences
+are floating-point contraction and pow() versus repeated multiplication:
+tolerance 1e-12 relative.  Whole solves are compared with the reference's own
+converged solution (tests/golden, reference iterated to stagnation) on every
+point the discrete problem determines: tolerance 1e-9 absolute.
+"""
+import numpy as np
+import pytest
+
+import ollipticid_amd as oa
+from oracle import oracle as O
+from tests.util import determined_mask, golden_u, load_golden
+
+pytestmark = pytest.mark.gpu
+
+PUNC = ["-eq", "Punctures", "-B", "Robin", "-BA", "0", "-Bn", "1"]
+# two punctures whose finer boxes merge, 4th order, spins and momenta
+SMALL_O4 = PUNC + ["-NumBH", "2", "-bhmp1", "0.4", "-bhx1", "1.1", "-bhy1", "0.3",
+                   "-bhz1", "-0.2", "-bhpx1", "0.05", "-bhpy1", "0.12", "-bhsz1", "0.05",
+                   "-bhmp2", "0.35", "-bhx2", "-1.3", "-bhy2", "-0.4", "-bhz2", "0.5",
+                   "-bhpx2", "-0.05", "-bhpz2", "-0.1", "-bhsx2", "0.04",
+                   "-L", "48", "-N", "25", "-l", "3", "-O", "4", "-nu2", "4", "-s", "1e-12"]
+SCALAR_O6 = ["-eq", "Punctures_scalar_field", "-B", "Robin", "-psc", "2", "-phi0", "0.02",
+             "-a2", "1", "-r0", "10", "-sigma", "4", "-NumBH", "1", "-bhmp1", "0.5",
+             "-bhx1", "0.7", "-bhpy1", "0.1", "-L", "64", "-N", "25", "-l", "2", "-O", "6",
+             "-nu2", "4", "-s", "1e-12"]
+O8 = PUNC + ["-NumBH", "1", "-bhmp1", "0.8", "-bhx1", "0.5", "-bhpy1", "0.15", "-bhsz1", "0.1",
+             "-L", "50", "-N", "25", "-l", "2", "-O", "8", "-nu2", "4", "-s", "1e-12"]
+CASES = {"o4_merge": SMALL_O4, "sf_o6": SCALAR_O6, "o8": O8}
+
+
+def pair(argv):
+    print("case argv:", " ".join(argv), flush=True)
+    h = oa.Hierarchy(oa.problem_from_argv(argv))
+    s = O.Solver.from_argv(argv, sweep_order=O.COLOUR)
+    assert [b["n"] for b in h.boxes()] == [b["n"] for b in s.boxes()]
+    return h, s
+
+
+def seed_state(h, s, seed):
+    """Same smooth-plus-noise u on both (c + u stays positive)."""
+    rng = np.random.default_rng(seed)
+    for ib, b in enumerate(s.boxes()):
+        n = b["n"]
+        z, y, x = np.meshgrid(*[np.linspace(0, 1, n[a]) for a in (2, 1, 0)], indexing="ij")
+        u = 0.01 * np.sin(3 * x + 1) * np.cos(2 * y) * (1 + z) + 1e-3 * rng.standard_normal(x.shape)
+        s.field(ib, O.F_U)[:] = u
+        h.set_field(ib, oa.F_U, u)
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_coefficients(case):
+    h, s = pair(CASES[case])
+    fields = [(oa.F_B, O.F_B), (oa.F_C, O.F_C), (oa.F_RHO, O.F_RHO), (oa.F_U, O.F_U)]
+    if "scalar" in " ".join(CASES[case]):
+        fields += [(oa.F_D, O.F_D), (oa.F_E, O.F_E)]
+    for ib in range(s.nbox):
+        for fg, fo in fields:
+            g, o = h.get_field(ib, fg), s.field(ib, fo)
+            # c is ~1e15 at a puncture on a grid point; compare relative per point
+            den = np.maximum(np.abs(o), 1e-300)
+            assert (np.abs(g - o) / den)[np.abs(o) > 0].max(initial=0) < 1e-12, (case, ib, fg)
+            assert np.all((o == 0) <= (np.abs(g) < 1e-300)), (case, ib, fg)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_residual_every_level(case):
+    h, s = pair(CASES[case])
+    seed_state(h, s, 1)
+    for lv in range(s.levels):
+        ng = h.residual(lv)
+        no = max(s.residual(ib) for ib in s.level_boxes(lv))
+        assert abs(ng - no) <= 1e-12 * no, (case, lv)
+        for ib in s.level_boxes(lv):
+            R = s.box_info(ib)["R"]
+            g = h.get_field(ib, oa.F_R)[R:-R, R:-R, R:-R]
+            o = s.field(ib, O.F_R)[R:-R, R:-R, R:-R]
+            assert rel_err(g, o) < 1e-12, (case, lv, ib)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_smooth_sweeps_every_level(case):
+    h, s = pair(CASES[case])
+    seed_state(h, s, 2)
+    for lv in range(s.levels):
+        ng = h.smooth(lv, 2)
+        no = max(s.smooth(ib, 2) for ib in s.level_boxes(lv))
+        assert abs(ng - no) <= 1e-11 * no, (case, lv, ng, no)
+        for ib in s.level_boxes(lv):
+            assert rel_err(h.get_field(ib, oa.F_U), s.field(ib, O.F_U)) < 1e-12, (case, lv, ib)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_robin_and_boundary_operator(case):
+    """Robin operator on a seeded state (non-zero everywhere on the shell),
+    then the Robin update, which leaves operator == rho on the shell."""
+    h, s = pair(CASES[case])
+    seed_state(h, s, 3)
+    for lv in range(s.mgrid + 1):
+        ib = s.level_boxes(lv)[0]
+        R = s.box_info(ib)["R"]
+        shell = np.ones(s.field(ib, O.F_U).shape, dtype=bool)
+        shell[R:-R, R:-R, R:-R] = False
+        h.boundary_operator(lv)
+        s.boundary_operator(ib)
+        g, o = h.get_field(ib, oa.F_R)[shell], s.field(ib, O.F_R)[shell]
+        assert rel_err(g, o) < 1e-12, (case, lv)
+        h.robin(lv)
+        s.robin(ib)
+        assert rel_err(h.get_field(ib, oa.F_U), s.field(ib, O.F_U)) < 1e-12, (case, lv)
+        # after the update the Robin equation holds on the shell (rho = 0)
+        h.boundary_operator(lv)
+        assert np.abs(h.get_field(ib, oa.F_R)[shell]).max() < 1e-12 * np.abs(o).max()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fas_cycles_match_oracle(case):
+    """Three whole FAS V-cycles from u = 0: every field value of every box."""
+    h, s = pair(CASES[case])
+    top = s.levels - 1
+    for cyc in range(3):
+        ng = h.fas(top)
+        no = s.fasn(top)
+        assert abs(ng - no) <= 1e-9 * no, (case, cyc, ng, no)
+    for ib, b in enumerate(s.boxes()):
+        g, o = h.get_field(ib, oa.F_U), s.field(ib, O.F_U)
+        assert np.abs(g - o).max() <= 1e-11 * max(1.0, np.abs(o).max()), (case, ib)
+
+
+def test_solve_cfg1_matches_reference():
+    """cfg1 to stagnation: every determined point within 1e-9 of the
+    reference's converged u (tests/golden/cfg1_conv)."""
+    meta, arrays = load_golden("cfg1_conv")
+    h = oa.Hierarchy(oa.problem_from_argv(meta["argv"]))
+    cycles, hist, res = h.solve()
+    print("cfg1 conv: cycles", cycles, "res", res, "ms", h.last_solve_ms(), flush=True)
+    assert res < 1e-14
+    worst = 0.0
+    for ib, (b, ref) in enumerate(zip(h.boxes(), golden_u(meta, arrays, h.boxes()))):
+        m = determined_mask(b, h.mgrid)
+        worst = max(worst, np.abs(h.get_field(ib, oa.F_U) - ref)[m].max())
+    print("cfg1 max |u - u_ref|:", worst, flush=True)
+    assert worst <= 1e-9
+    adm, _ = h.adm_mass()
+    np.testing.assert_allclose(adm, meta["adm_mass"], rtol=5.01e-9)
+
+
+def test_solve_small_converging_case_matches_reference():
+    """Four punctures, merged boxes, 2nd order: the reference converges to
+    3.6e-15; every determined point within 1e-9 of its u."""
+    meta, arrays = load_golden("small_o2_4bh")
+    h = oa.Hierarchy(oa.problem_from_argv(meta["argv"]))
+    cycles, hist, res = h.solve()
+    print("small_o2_4bh: cycles", cycles, "res", res, flush=True)
+    worst = 0.0
+    for ib, (b, ref) in enumerate(zip(h.boxes(), golden_u(meta, arrays, h.boxes()))):
+        m = determined_mask(b, h.mgrid)
+        worst = max(worst, np.abs(h.get_field(ib, oa.F_U) - ref)[m].max())
+    assert worst <= 1e-9
+
+
+def test_adm_mass_matches_oracle():
+    h, s = pair(SMALL_O4)
+    top = s.levels - 1
+    for _ in range(2):
+        h.fas(top)
+        s.fasn(top)
+    pg, sg = h.adm_mass()
+    po, so = s.adm_mass()
+    np.testing.assert_allclose(pg, po, rtol=1e-12)
+    np.testing.assert_allclose(sg, so, rtol=1e-10)

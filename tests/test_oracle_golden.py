@@ -1,0 +1,46 @@
+"""The oracle (plain-C restatement, oracle/) against the reference's own
+outputs (tests/golden, written by tests/golden/make_golden.py from the
+reference binary compiled from its sources).  In the reference's
+lexicographic order the oracle must reproduce the residual history
+(Loo.gp, 18 digits) exactly, the cycle count, the converged u to the
+fixtures' 12 printed digits and the ADM masses to their 9 printed digits."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.util import golden_u, load_golden
+
+FAST = ["cfg1_tol", "small_o4_offgrid", "small_o6_merge", "small_sf_3bh", "small_o2_4bh"]
+
+
+@pytest.mark.parametrize("name", FAST)
+def test_oracle_matches_reference(name):
+    meta, arrays = load_golden(name)
+    s = O.Solver.from_argv(meta["argv"])
+    boxes = s.boxes()
+    assert [(b["level"], b["n"]) for b in boxes] == [(b["level"], b["n"]) for b in meta["boxes"]]
+    cycles, hist, res = s.solve()
+    assert cycles == meta["cycles"]
+    ref_hist = np.array(meta["history"])
+    assert hist.shape == ref_hist.shape
+    np.testing.assert_array_equal(hist[:, 0], ref_hist[:, 0])
+    # Loo.gp prints 18 significant digits
+    np.testing.assert_allclose(hist[:, 1], ref_hist[:, 1], rtol=1e-15, atol=0)
+    st = meta["stride"]
+    for ib, ref in enumerate(golden_u(meta, arrays, boxes)):
+        u = s.field(ib, O.F_U)[::st, ::st, ::st]
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(u - ref).max() <= 1e-11 * scale + 1e-14
+    adm, _ = s.adm_mass()
+    # stdout prints 9 significant digits
+    np.testing.assert_allclose(adm, meta["adm_mass"], rtol=5.01e-9)
+
+
+def test_oracle_converged_cfg1_field():
+    """cfg1 iterated to stagnation (-s 1e-16): 30 cycles, u to print precision."""
+    meta, arrays = load_golden("cfg1_conv")
+    s = O.Solver.from_argv(meta["argv"])
+    cycles, hist, res = s.solve()
+    assert cycles == meta["cycles"] == 30
+    for ib, ref in enumerate(golden_u(meta, arrays, s.boxes())):
+        assert np.abs(s.field(ib, O.F_U) - ref).max() < 1e-14
