@@ -132,6 +132,7 @@ def main():
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "cycles": cycles,
+        "coarse_sweeps_per_solve": h.last_solve_coarse_sweeps(),
         "residual": res,
         "adm_mass": adm,
         "du_inf_vs_fp64": 0.0,

@@ -145,6 +145,8 @@ int oll_adm_mass(oll_hier *h, double *puncture, double *surface);
 
 /* Device time of the last oll_solve (HIP events on the handle's stream). */
 int oll_last_solve_ms(const oll_hier *h, double *ms);
+/* Sweeps spent in the coarsest-level loop during the last oll_solve. */
+int oll_last_solve_coarse_sweeps(const oll_hier *h, long long *sweeps);
 /* Restores the initial state u = 0, rho = Robin value on the shell
  * (boundary.cc:8-84 Set_Boundary) so the same hierarchy can be solved again. */
 int oll_reset(oll_hier *h);

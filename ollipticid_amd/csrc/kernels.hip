@@ -104,11 +104,6 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return r;
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, double v) {
-  // non-negative doubles order like their bit patterns
-  atomicMax(slot, (unsigned long long)__double_as_longlong(v));
-}
-
 // locate the box of a flat interior row index within a level
 __device__ __forceinline__ int find_box_row(const LevelView &lv, int row) {
   int b = 0;
@@ -122,29 +117,57 @@ __device__ __forceinline__ int find_box_pt(const LevelView &lv, long long pt) {
   return b;
 }
 
+// Norm slots: every logical norm is kNormSub consecutive u64 words; blocks
+// spread their atomics over them and the host takes the max.  A block only
+// issues its atomic when it raises the value it reads.
+constexpr int kNormSub = 8;
+
+__device__ __forceinline__ void norm_update(unsigned long long *slot, double v) {
+  unsigned long long *s = slot + (blockIdx.x & (kNormSub - 1));
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  if (bits > __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(s, bits);
+}
+
 // ------------------------------------------------------------------
 // smoother: one colour of one Newton-Gauss-Seidel sweep on every box of a
-// level.  Block = 4 interior rows x 64 threads; thread t of a row owns the
-// colour points i = i_first + t (R+1).
+// level.  Thread g owns colour point t = g mod per_row of interior row
+// g / per_row: i = i_first(row) + t (R+1).
 // ------------------------------------------------------------------
+// exact g / d for g < 2^24 via a float reciprocal and one correction step
+__device__ __forceinline__ unsigned fast_div(unsigned g, unsigned d, float inv_d) {
+  unsigned q = unsigned(float(g) * inv_d);
+  if (q * d > g) --q;
+  else if ((q + 1) * d <= g) ++q;
+  return q;
+}
+
+// wave-level max, then one (read-checked) atomic per wave
+__device__ __forceinline__ void wave_norm_update(unsigned long long *slot, double v) {
+  v = warp_max(v);
+  if ((threadIdx.x & 63) == 0 && v > 0.0) norm_update(slot, v);
+}
+
 template <int ORD, int EQ>
 __global__ __launch_bounds__(256) void k_smooth(LevelView lv, const DevBox *__restrict__ boxes,
-                                                 Fields F, int colour,
+                                                 Fields F, int colour, unsigned per_row,
+                                                 float inv_per_row,
                                                  unsigned long long *__restrict__ norm) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
-  __shared__ double sh[4];
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int row = blockIdx.x * 4 + ty;
+  const unsigned g = blockIdx.x * 256u + threadIdx.x;
+  const unsigned row = fast_div(g, per_row, inv_per_row);
+  const int t = int(g - row * per_row);
   double rmax = 0.0;
-  if (row < lv.row0[lv.nbox]) {
-    const int bi = find_box_row(lv, row);
+  if (row < unsigned(lv.row0[lv.nbox])) {
+    const int bi = find_box_row(lv, int(row));
     const DevBox &B = boxes[lv.box[bi]];
-    const int r = row - lv.row0[bi];
-    const int nyi = B.n[1] - 2 * R;
-    const int j = R + r % nyi, k = R + r / nyi;
+    const unsigned r = row - unsigned(lv.row0[bi]);
+    const unsigned nyi = unsigned(B.n[1] - 2 * R);
+    const unsigned kq = fast_div(r, nyi, 1.0f / float(nyi));
+    const int k = R + int(kq);
+    const int j = R + int(r - kq * nyi);
     const int first = R + ((colour - j - k - R) % NC + 2 * NC) % NC;
-    const int i = first + (blockIdx.y * 64 + tx) * NC;
+    const int i = first + t * NC;
     if (i < B.n[0] - R) {
       const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
       const long long p = B.off + i + sy * j + sz * k;
@@ -155,14 +178,13 @@ __global__ __launch_bounds__(256) void k_smooth(LevelView lv, const DevBox *__re
       nonlinear<EQ>(lu, B.dulap, F.f[F_B][p], cu, EQ ? F.f[F_D][p] : 0.0,
                     EQ ? F.f[F_E][p] : 0.0, N, dN);
       const double res = N - F.f[F_RHO][p];
-      F.f[F_R][p] = res;
-      u[p] = u[p] - res / dN;
+      const double un = u[p] - res / dN;
+      u[p] = un;
       rmax = fabs(res);
-      if (!(u[p] == u[p])) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
+      if (!(un == un)) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
     }
   }
-  const double m = block_max(rmax, sh);
-  if (threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(norm, m);
+  wave_norm_update(norm, rmax);
 }
 
 // residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u)
@@ -198,7 +220,7 @@ __global__ __launch_bounds__(256) void k_residual(LevelView lv, const DevBox *__
     }
   }
   const double m = block_max(rmax, sh);
-  if (norm && threadIdx.x == 0 && m > 0.0) atomic_max_nonneg(norm, m);
+  if (norm && threadIdx.x == 0 && m > 0.0) norm_update(norm, m);
 }
 
 // ------------------------------------------------------------------
@@ -237,13 +259,31 @@ __device__ __forceinline__ double coord(const DevBox &B, int a, int i) {
   return __dadd_rn(B.lo[a], __dmul_rn((double)i, B.d[a]));
 }
 
+// Box-local views of the fields a Robin stage touches (point (0,0,0) of the
+// box at index 0).  u may live in global memory or in LDS.
+struct BoxData {
+  double *u;
+  const double *rho;
+  double *r;
+  double *w;
+};
+
+__device__ __forceinline__ BoxData box_data(const DevBox &B, Fields F) {
+  BoxData D;
+  D.u = F.f[F_U] + B.off;
+  D.rho = F.f[F_RHO] + B.off;
+  D.r = F.f[F_R] + B.off;
+  D.w = F.f[F_W] + B.off;
+  return D;
+}
+
 // Robin update / operator at one shell point; `mask` has bit a set for
 // each axis a whose index lies in the ghost shell.
-__device__ __forceinline__ void robin_point(const DevBox &B, Fields F, int i, int j, int k,
+__device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, int j, int k,
                                             int mask, int mode, double ipn) {
   const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  const long long p = B.off + i + sy * j + sz * k;
-  double *__restrict__ u = F.f[F_U];
+  const long long p = i + sy * j + sz * k;
+  double *u = D.u;
   const int ijk[3] = {i, j, k};
   const long long st[3] = {1, sy, sz};
   double X[3];
@@ -259,40 +299,30 @@ __device__ __forceinline__ void robin_point(const DevBox &B, Fields F, int i, in
   }
   const double op = r2 * ipn * sF + u[p];
   if (mode == ROBIN_UPDATE) {
-    const double Fv = op - F.f[F_RHO][p];
+    const double Fv = op - D.rho[p];
     const double dF = r2 * ipn * sdF + 1.0;
     u[p] = u[p] - Fv / dF;
   } else if (mode == ROBIN_OPERATOR_R) {
-    F.f[F_R][p] = op;
+    D.r[p] = op;
   } else {
-    F.f[F_W][p] = F.f[F_RHO][p] - op;
+    D.w[p] = D.rho[p] - op;
   }
 }
 
-// faces: one thread per (face, column); layers inner -> outer
-__global__ __launch_bounds__(256) void k_robin_faces(const DevBox *__restrict__ boxes, int box,
-                                                      Fields F, int mode, double ipn) {
-  const DevBox &B = boxes[box];
+// face column t (0 .. 2(nj nk + ni nk + ni nj) - 1) of a box: all R layers,
+// inner -> outer
+__device__ __forceinline__ void robin_face_column(const DevBox &B, BoxData D, int t, int mode,
+                                                  double ipn) {
   const int R = B.R;
   const int ni = B.n[0] - 2 * R, nj = B.n[1] - 2 * R, nk = B.n[2] - 2 * R;
   const int fx = nj * nk, fy = ni * nk, fz = ni * nj;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
   int face, a0, a1;
   if (t < 2 * fx) {
-    face = t / fx;
-    t %= fx;
-    a0 = t % nj;
-    a1 = t / nj;  // j, k
+    face = t / fx; t %= fx; a0 = t % nj; a1 = t / nj;  // j, k
   } else if ((t -= 2 * fx) < 2 * fy) {
-    face = 2 + t / fy;
-    t %= fy;
-    a0 = t % ni;
-    a1 = t / ni;  // i, k
+    face = 2 + t / fy; t %= fy; a0 = t % ni; a1 = t / ni;  // i, k
   } else if ((t -= 2 * fy) < 2 * fz) {
-    face = 4 + t / fz;
-    t %= fz;
-    a0 = t % ni;
-    a1 = t / ni;  // i, j
+    face = 4 + t / fz; t %= fz; a0 = t % ni; a1 = t / ni;  // i, j
   } else {
     return;
   }
@@ -307,8 +337,15 @@ __global__ __launch_bounds__(256) void k_robin_faces(const DevBox *__restrict__ 
     } else {
       i = R + a0; j = R + a1; k = g;
     }
-    robin_point(B, F, i, j, k, 1 << axis, mode, ipn);
+    robin_point(B, D, i, j, k, 1 << axis, mode, ipn);
   }
+}
+
+// faces: one thread per (face, column); layers inner -> outer
+__global__ __launch_bounds__(256) void k_robin_faces(const DevBox *__restrict__ boxes, int box,
+                                                      Fields F, int mode, double ipn) {
+  const DevBox &B = boxes[box];
+  robin_face_column(B, box_data(B, F), blockIdx.x * blockDim.x + threadIdx.x, mode, ipn);
 }
 
 // ghost index of position q (0..R-1) on side `hi` of an axis, in the
@@ -319,7 +356,7 @@ __device__ __forceinline__ int shell_index(int q, int hi, int n, int R) {
 }
 
 // edges then corners, one block per box (faces must be done)
-__device__ void robin_edges_corners(const DevBox &B, Fields F, int mode, double ipn) {
+__device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode, double ipn) {
   const int R = B.R;
   const int nint[3] = {B.n[0] - 2 * R, B.n[1] - 2 * R, B.n[2] - 2 * R};
   // edges XY(4) run along z, XZ(4) along y, YZ(4) along x; within an edge
@@ -348,7 +385,7 @@ __device__ void robin_edges_corners(const DevBox &B, Fields F, int mode, double 
         } else {  // YiZi, YiZf, YfZi, YfZf: z outer, y inner; along x
           j = shell_index(qb, hA, B.n[1], R); k = shell_index(qa, hB, B.n[2], R); i = R + s; mask = 6;
         }
-        robin_point(B, F, i, j, k, mask, mode, ipn);
+        robin_point(B, D, i, j, k, mask, mode, ipn);
       }
   }
   __syncthreads();
@@ -360,7 +397,7 @@ __device__ void robin_edges_corners(const DevBox &B, Fields F, int mode, double 
     for (int qk = 0; qk < R; ++qk)
       for (int qj = 0; qj < R; ++qj)
         for (int qi = 0; qi < R; ++qi)
-          robin_point(B, F, shell_index(qi, hx, B.n[0], R), shell_index(qj, hy, B.n[1], R),
+          robin_point(B, D, shell_index(qi, hx, B.n[0], R), shell_index(qj, hy, B.n[1], R),
                       shell_index(qk, hz, B.n[2], R), 7, mode, ipn);
   }
 }
@@ -368,7 +405,7 @@ __device__ void robin_edges_corners(const DevBox &B, Fields F, int mode, double 
 __global__ __launch_bounds__(1024) void k_robin_edges_corners(const DevBox *__restrict__ boxes,
                                                                int box, Fields F, int mode,
                                                                double ipn) {
-  robin_edges_corners(boxes[box], F, mode, ipn);
+  robin_edges_corners(boxes[box], box_data(boxes[box], F), mode, ipn);
 }
 
 // ------------------------------------------------------------------
@@ -590,96 +627,185 @@ __global__ __launch_bounds__(256) void k_prolong(XferView xv, const DevBox *__re
   }
 }
 
-// ghost shell of refinement boxes from the coarser level (Lagrange order+1)
+// shell point q (0 .. n0 n1 n2 - (n0-2R)(n1-2R)(n2-2R) - 1) of a box:
+// the R-thick z slabs first, then y slabs, then x slabs
+__device__ __forceinline__ void shell_point(const DevBox &B, unsigned q, int &i, int &j, int &k) {
+  const unsigned R = unsigned(B.R);
+  const unsigned n0 = unsigned(B.n[0]), n1 = unsigned(B.n[1]), n2 = unsigned(B.n[2]);
+  const unsigned zs = n0 * n1 * R;  // one z slab
+  if (q < 2 * zs) {
+    const unsigned s = q >= zs, r = q - s * zs;
+    const unsigned plane = n0 * n1;
+    const unsigned l = r / plane, rr = r - l * plane;
+    j = int(rr / n0);
+    i = int(rr - unsigned(j) * n0);
+    k = s ? int(n2 - R + l) : int(l);
+    return;
+  }
+  q -= 2 * zs;
+  const unsigned ys = n0 * R * (n2 - 2 * R);
+  if (q < 2 * ys) {
+    const unsigned s = q >= ys, r = q - s * ys;
+    const unsigned row = r / n0;
+    i = int(r - row * n0);
+    const unsigned kk = row / R, l = row - kk * R;
+    k = int(R + kk);
+    j = s ? int(n1 - R + l) : int(l);
+    return;
+  }
+  q -= 2 * ys;
+  const unsigned xs = R * (n1 - 2 * R) * (n2 - 2 * R);
+  const unsigned s = q >= xs, r = q - s * xs;
+  const unsigned col = r / R, l = r - col * R;
+  const unsigned kk = col / (n1 - 2 * R);
+  j = int(R + (col - kk * (n1 - 2 * R)));
+  k = int(R + kk);
+  i = s ? int(n0 - R + l) : int(l);
+}
+
+// Lagrange tensor interpolation that skips the nodes of coincident axes
+// (their weights are exactly one unit vector)
+__device__ double tensor_interp_sparse(const DevBox &S, const double *__restrict__ f,
+                                       const AxisRef ax[3], const int t[3], unsigned fl0,
+                                       unsigned fl1, unsigned fl2) {
+  int s0[3], kk[3];
+  const double *w[3];
+  const unsigned fl[3] = {fl0, fl1, fl2};
+  for (int a = 0; a < 3; ++a) {
+    if (fl[a] & 4) {
+      s0[a] = ax[a].coinc[t[a]];
+      kk[a] = 1;
+      w[a] = nullptr;
+    } else {
+      s0[a] = ax[a].start[t[a]];
+      kk[a] = ax[a].k;
+      w[a] = ax[a].w + (long long)t[a] * ax[a].k;
+    }
+  }
+  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
+  double acc = 0.0;
+  for (int I = 0; I < kk[0]; ++I) {
+    double uj = 0.0;
+    for (int J = 0; J < kk[1]; ++J) {
+      double uk = 0.0;
+      const long long base = S.off + (s0[0] + I) + sy * (s0[1] + J) + sz * (long long)s0[2];
+      for (int K = 0; K < kk[2]; ++K) uk += f[base + sz * K] * (w[2] ? w[2][K] : 1.0);
+      uj += uk * (w[1] ? w[1][J] : 1.0);
+    }
+    acc += uj * (w[0] ? w[0][I] : 1.0);
+  }
+  return acc;
+}
+
+// ghost shell of refinement boxes from the coarser level (Lagrange order+1);
+// one thread per shell point
 __global__ __launch_bounds__(256) void k_ghost_fill(XferView xv, const DevBox *__restrict__ boxes,
                                                      Fields F) {
   const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= xv.tgt.pt0[xv.tgt.nbox]) return;
-  const int bi = find_box_pt(xv.tgt, pt);
+  const int nb = xv.tgt.nbox;
+  if (pt >= xv.tgt.shell0[nb]) return;
+  int bi = 0;
+  while (bi + 1 < nb && pt >= xv.tgt.shell0[bi + 1]) ++bi;
   const DevBox &T = boxes[xv.tgt.box[bi]];
-  const long long q = pt - xv.tgt.pt0[bi];
-  const int i = int(q % T.n[0]), j = int((q / T.n[0]) % T.n[1]), k = int(q / ((long long)T.n[0] * T.n[1]));
-  if (!in_region(T, i, j, k, REG_BOUNDARY)) return;
+  int i, j, k;
+  shell_point(T, unsigned(pt - xv.tgt.shell0[bi]), i, j, k);
   const int t[3] = {i, j, k};
   for (int x = xv.x0[bi + 1] - 1; x >= xv.x0[bi]; --x) {
     const DevXfer &X = xv.xfer[x];
     AxisRef ax[3] = {axis_ref(xv, X.ax[0]), axis_ref(xv, X.ax[1]), axis_ref(xv, X.ax[2])};
-    const unsigned f = ax[0].flags[i] & ax[1].flags[j] & ax[2].flags[k];
+    const unsigned f0 = ax[0].flags[i], f1 = ax[1].flags[j], f2 = ax[2].flags[k];
+    const unsigned f = f0 & f1 & f2;
     if (!(f & 3)) continue;
     const DevBox &S = boxes[X.source];
-    double v;
-    if (f & 4)
-      v = F.f[F_U][S.off + ax[0].coinc[i] + (long long)S.n[0] * ax[1].coinc[j] +
-                   (long long)S.n[0] * S.n[1] * ax[2].coinc[k]];
-    else
-      v = tensor_interp(S, F.f[F_U], ax, t);
-    F.f[F_U][T.off + q] = v;
+    const double v = tensor_interp_sparse(S, F.f[F_U], ax, t, f0, f1, f2);
+    F.f[F_U][T.off + i + (long long)T.n[0] * j + (long long)T.n[0] * T.n[1] * k] = v;
     return;
   }
 }
 
 // ------------------------------------------------------------------
 // coarsest level: {Robin, one multi-colour sweep} until the sweep norm is
-// <= eps or stops changing; one block, everything in one launch.
+// <= eps or stops changing (elliptic.cc:1782-1816); one block, one launch.
+// u lives in LDS for the whole loop.  Interior updates are over-relaxed by
+// omega (omega = 1 is the reference's Gauss-Seidel-Newton); the loop's fixed
+// point, the coarse solution, does not depend on omega.
 // ------------------------------------------------------------------
+constexpr int kCoarseOwn = 8;  // interior points per thread (1024 threads)
+
 template <int EQ>
 __global__ __launch_bounds__(1024) void k_coarse_solve(const DevBox *__restrict__ boxes, int box,
                                                         Fields F, double ipn, double eps,
-                                                        int max_iter,
+                                                        double omega, int max_iter,
                                                         unsigned long long *__restrict__ norm,
                                                         int *__restrict__ iters) {
-  const DevBox &B = boxes[box];
+  extern __shared__ double lds[];
   __shared__ double sh[16];
   __shared__ double s_norm;
+  const DevBox &B = boxes[box];
   const int R = B.R;  // 1: the coarsest level always uses order 2
-  const int ni = B.n[0] - 2 * R, nj = B.n[1] - 2 * R, nk = B.n[2] - 2 * R;
-  const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  double *__restrict__ u = F.f[F_U];
-  double prev = 1.0, cur = 1.0;
-  int it = 0;
-  const int fx = nj * nk, fy = ni * nk, fz = ni * nj;
-  const int nface = 2 * (fx + fy + fz);
-  do {
-    prev = cur;
-    // Robin faces (R = 1: a single layer)
-    for (int t0 = threadIdx.x; t0 < nface; t0 += blockDim.x) {
-      int t = t0, face, a0, a1;
-      if (t < 2 * fx) {
-        face = t / fx; t %= fx; a0 = t % nj; a1 = t / nj;
-      } else if ((t -= 2 * fx) < 2 * fy) {
-        face = 2 + t / fy; t %= fy; a0 = t % ni; a1 = t / ni;
-      } else {
-        t -= 2 * fy; face = 4 + t / fz; t %= fz; a0 = t % ni; a1 = t / ni;
-      }
-      const int axis = face >> 1, hi = face & 1;
-      for (int l = 0; l < R; ++l) {
-        const int g = hi ? B.n[axis] - R + l : R - 1 - l;
-        int i, j, k;
-        if (axis == 0) { i = g; j = R + a0; k = R + a1; }
-        else if (axis == 1) { i = R + a0; j = g; k = R + a1; }
-        else { i = R + a0; j = R + a1; k = g; }
-        robin_point(B, F, i, j, k, 1 << axis, ROBIN_UPDATE, ipn);
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int np = n0 * n1 * n2;
+  double *su = lds;        // u
+  double *srho = lds + np; // rho
+  const int ni = n0 - 2 * R, nj = n1 - 2 * R, nk = n2 - 2 * R;
+  const long long sy = n0, sz = (long long)n0 * n1;
+  double *__restrict__ ug = F.f[F_U] + B.off;
+  for (int q = threadIdx.x; q < np; q += blockDim.x) {
+    su[q] = ug[q];
+    srho[q] = F.f[F_RHO][B.off + q];
+  }
+  // owned interior points: t = threadIdx.x + m * blockDim.x
+  const int nint = ni * nj * nk;
+  int own_p[kCoarseOwn], own_c[kCoarseOwn];
+  double own_b[kCoarseOwn], own_cc[kCoarseOwn], own_d[kCoarseOwn], own_e[kCoarseOwn];
+#pragma unroll
+  for (int m = 0; m < kCoarseOwn; ++m) {
+    const int t = threadIdx.x + m * blockDim.x;
+    own_p[m] = -1;
+    own_c[m] = -1;
+    own_b[m] = own_cc[m] = own_d[m] = own_e[m] = 0.0;
+    if (t < nint) {
+      const int i = R + t % ni, j = R + (t / ni) % nj, k = R + t / (ni * nj);
+      const int p = i + n0 * (j + n1 * k);
+      own_p[m] = p;
+      own_c[m] = (i + j + k) % (R + 1);
+      own_b[m] = F.f[F_B][B.off + p];
+      own_cc[m] = F.f[F_C][B.off + p];
+      if (EQ) {
+        own_d[m] = F.f[F_D][B.off + p];
+        own_e[m] = F.f[F_E][B.off + p];
       }
     }
+  }
+  __syncthreads();
+  BoxData D;
+  D.u = su;
+  D.rho = srho;
+  D.r = nullptr;
+  D.w = nullptr;
+  const int nface = 2 * (nj * nk + ni * nk + ni * nj);
+  double prev = 1.0, cur = 1.0;
+  int it = 0;
+  do {
+    prev = cur;
+    for (int t = threadIdx.x; t < nface; t += blockDim.x) robin_face_column(B, D, t, ROBIN_UPDATE, ipn);
     __syncthreads();
-    robin_edges_corners(B, F, ROBIN_UPDATE, ipn);
+    robin_edges_corners(B, D, ROBIN_UPDATE, ipn);
     __syncthreads();
     double m = 0.0;
-    const int nint = ni * nj * nk;
     for (int col = 0; col <= R; ++col) {
-      for (int t = threadIdx.x; t < nint; t += blockDim.x) {
-        const int i = R + t % ni, j = R + (t / ni) % nj, k = R + t / (ni * nj);
-        if ((i + j + k) % (R + 1) != col) continue;
-        const long long p = B.off + i + sy * j + sz * k;
-        const double lu = lap<2>(u, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
+#pragma unroll
+      for (int q = 0; q < kCoarseOwn; ++q) {
+        if (own_c[q] != col) continue;
+        const long long p = own_p[q];
+        const double lu = lap<2>(su, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
         double N, dN;
-        nonlinear<EQ>(lu, B.dulap, F.f[F_B][p], F.f[F_C][p] + u[p], EQ ? F.f[F_D][p] : 0.0,
-                      EQ ? F.f[F_E][p] : 0.0, N, dN);
-        const double res = N - F.f[F_RHO][p];
-        F.f[F_R][p] = res;
-        u[p] = u[p] - res / dN;
+        nonlinear<EQ>(lu, B.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
+        const double res = N - srho[p];
+        const double un = su[p] - omega * (res / dN);
+        su[p] = un;
         m = fmax(m, fabs(res));
-        if (!(u[p] == u[p])) m = __longlong_as_double(0x7ff0000000000000LL);
+        if (!(un == un)) m = __longlong_as_double(0x7ff0000000000000LL);
       }
       __syncthreads();
     }
@@ -690,6 +816,7 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(const DevBox *__restrict_
     ++it;
     __syncthreads();
   } while (cur > eps && prev != cur && it < max_iter && !isinf(cur));
+  for (int q = threadIdx.x; q < np; q += blockDim.x) ug[q] = su[q];
   if (threadIdx.x == 0) {
     *norm = (unsigned long long)__double_as_longlong(cur);
     *iters = it;
@@ -992,11 +1119,12 @@ template <int ORD, int EQ>
 void smooth_impl(hipStream_t s, const LevelView &lv, const DevBox *boxes, Fields F, int colour,
                  unsigned long long *norm) {
   constexpr int NC = ORD / 2 + 1;
-  const int rows = lv.row0[lv.nbox];
-  const int per_row = (lv.max_row_len + NC - 1) / NC;
-  dim3 grid(ceil_div(rows, 4), ceil_div(per_row, 64));
+  const long long rows = lv.row0[lv.nbox];
+  const unsigned per_row = unsigned((lv.max_row_len + NC - 1) / NC);
+  const dim3 grid(ceil_div(rows * per_row, 256));
   if (empty_grid(grid)) return;
-  hipLaunchKernelGGL((k_smooth<ORD, EQ>), grid, dim3(256), 0, s, lv, boxes, F, colour, norm);
+  hipLaunchKernelGGL((k_smooth<ORD, EQ>), grid, dim3(256), 0, s, lv, boxes, F, colour, per_row,
+                     1.0f / float(per_row), norm);
 }
 
 template <int ORD, int EQ>
@@ -1054,20 +1182,34 @@ void launch_prolong(hipStream_t s, const XferView &xv, const DevBox *boxes, Fiel
 }
 
 void launch_ghost_fill(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F) {
-  const long long np = xv.tgt.pt0[xv.tgt.nbox];
+  const long long np = xv.tgt.shell0[xv.tgt.nbox];
+  if (np == 0) return;
   hipLaunchKernelGGL(k_ghost_fill, dim3(ceil_div(np, 256)), dim3(256), 0, s, xv, boxes, F);
 }
 
-void launch_coarse_solve(hipStream_t s, const DevBox *boxes, int box, const DevBox &hb, Fields F,
-                         int eq, double ipn, double eps, int max_iter, unsigned long long *norm,
-                         int *iters) {
-  (void)hb;
+int coarse_solve_lds_limit() { return 160 * 1024; }
+
+bool launch_coarse_solve(hipStream_t s, const DevBox *boxes, int box, const DevBox &hb, Fields F,
+                         int eq, double ipn, double eps, double omega, int max_iter,
+                         unsigned long long *norm, int *iters) {
+  const size_t lds = 2 * sizeof(double) * size_t(hb.n[0]) * hb.n[1] * hb.n[2];
+  const long long nint = (long long)(hb.n[0] - 2) * (hb.n[1] - 2) * (hb.n[2] - 2);
+  if (lds + 2048 > size_t(coarse_solve_lds_limit()) || nint > 1024LL * kCoarseOwn) return false;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, coarse_solve_lds_limit() - 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, coarse_solve_lds_limit() - 1024);
+    attr_set = true;
+  }
   if (eq)
-    hipLaunchKernelGGL(k_coarse_solve<1>, dim3(1), dim3(1024), 0, s, boxes, box, F, ipn, eps,
-                       max_iter, norm, iters);
+    hipLaunchKernelGGL(k_coarse_solve<1>, dim3(1), dim3(1024), lds, s, boxes, box, F, ipn, eps,
+                       omega, max_iter, norm, iters);
   else
-    hipLaunchKernelGGL(k_coarse_solve<0>, dim3(1), dim3(1024), 0, s, boxes, box, F, ipn, eps,
-                       max_iter, norm, iters);
+    hipLaunchKernelGGL(k_coarse_solve<0>, dim3(1), dim3(1024), lds, s, boxes, box, F, ipn, eps,
+                       omega, max_iter, norm, iters);
+  return true;
 }
 
 void launch_zero_u64(hipStream_t s, unsigned long long *p, int n) {

@@ -20,6 +20,7 @@ struct LevelView {
   int box[kMaxLevelBoxes];
   int row0[kMaxLevelBoxes + 1];     // interior rows prefix
   long long pt0[kMaxLevelBoxes + 1];  // all points prefix
+  long long shell0[kMaxLevelBoxes + 1];  // ghost-shell points prefix
   int max_row_len;                  // longest interior row (points)
 };
 
@@ -52,9 +53,11 @@ void launch_restrict_defect(hipStream_t s, const XferView &xv, const DevBox *box
                             int order, int eq, int mg, double robin_n);
 void launch_prolong(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F, int region);
 void launch_ghost_fill(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F);
-void launch_coarse_solve(hipStream_t s, const DevBox *boxes, int box, const DevBox &hb,
-                         Fields F, int eq, double robin_n, double eps, int max_iter,
+// false when the box does not fit the LDS-resident kernel
+bool launch_coarse_solve(hipStream_t s, const DevBox *boxes, int box, const DevBox &hb,
+                         Fields F, int eq, double ipn, double eps, double omega, int max_iter,
                          unsigned long long *norm, int *iters);
+constexpr int kNormSubSlots = 8;  // u64 words per logical norm slot
 void launch_zero_u64(hipStream_t s, unsigned long long *p, int n);
 void launch_copy(hipStream_t s, double *dst, const double *src, long long n);
 void launch_adm_sphere(hipStream_t s, const DevBox *boxes, int nbox, Fields F,

@@ -76,6 +76,9 @@ struct oll_hier {
   double last_ms = 0;
   size_t device_bytes = 0;
   int coarse_max_iter = 100000;
+  double coarse_omega = 1.0;
+  int coarse_too_big = 0;
+  long long coarse_sweeps = 0;  // coarsest-level sweeps of the last oll_solve
   // optional per-launch timing of the smoother (events around each launch)
   int timing = 0;
   std::vector<hipEvent_t> tev;
@@ -88,12 +91,13 @@ namespace {
 
 int slot_pre(int k) { return 2 * k; }
 int slot_post(int k) { return 2 * k + 1; }
+unsigned long long *slot_ptr(oll_hier &h, int slot) { return h.norms + size_t(slot) * kNormSubSlots; }
 
 LevelView make_level_view(const oll_hier &h, int level) {
   LevelView v{};
   v.nbox = h.geo.level_count[size_t(level)];
   int rows = 0;
-  long long pts = 0;
+  long long pts = 0, shell = 0;
   int maxrow = 0;
   for (int b = 0; b < v.nbox; ++b) {
     const int ib = h.geo.level_first[size_t(level)] + b;
@@ -101,12 +105,16 @@ LevelView make_level_view(const oll_hier &h, int level) {
     v.box[b] = ib;
     v.row0[b] = rows;
     v.pt0[b] = pts;
+    v.shell0[b] = shell;
     rows += (B.n[1] - 2 * B.R) * (B.n[2] - 2 * B.R);
     pts += (long long)B.n[0] * B.n[1] * B.n[2];
+    shell += (long long)B.n[0] * B.n[1] * B.n[2] -
+             (long long)(B.n[0] - 2 * B.R) * (B.n[1] - 2 * B.R) * (B.n[2] - 2 * B.R);
     maxrow = std::max(maxrow, B.n[0] - 2 * B.R);
   }
   v.row0[v.nbox] = rows;
   v.pt0[v.nbox] = pts;
+  v.shell0[v.nbox] = shell;
   v.max_row_len = maxrow;
   return v;
 }
@@ -209,7 +217,7 @@ void fas(oll_hier &h, int k) {
   const bool mg = k <= M;
   // pre-smoothing
   if (mg) robin_level(h, k, ROBIN_UPDATE);
-  smooth_level(h, k, h.prob.nu1, h.norms + slot_pre(k));
+  smooth_level(h, k, h.prob.nu1, slot_ptr(h, slot_pre(k)));
   // fine defect W = S - R (interior; Robin shell on multigrid levels)
   launch_residual(h.stream, h.lv[size_t(k)], h.dbox, h.F, level_order(h, k), h.prob.eq, 1, nullptr);
   if (mg) robin_level(h, k, ROBIN_DEFECT_W);
@@ -219,8 +227,10 @@ void fas(oll_hier &h, int k) {
                          mg ? 1 : 0, h.ipn);
   if (k == 1) {
     const int b0 = h.geo.level_first[0];
-    launch_coarse_solve(h.stream, h.dbox, b0, h.hbox[size_t(b0)], h.F, h.prob.eq, h.ipn,
-                        h.eps_coarse, h.coarse_max_iter, h.norms + slot_pre(0), h.iters);
+    if (!launch_coarse_solve(h.stream, h.dbox, b0, h.hbox[size_t(b0)], h.F, h.prob.eq, h.ipn,
+                             h.eps_coarse, h.coarse_omega, h.coarse_max_iter,
+                             slot_ptr(h, slot_pre(0)), h.iters))
+      h.coarse_too_big = 1;
   } else {
     fas(h, k - 1);
   }
@@ -228,8 +238,10 @@ void fas(oll_hier &h, int k) {
   launch_prolong(h.stream, h.pv[size_t(k)], h.dbox, h.F, mg ? REG_ALL : REG_INSIDE);
   if (!mg) launch_ghost_fill(h.stream, h.gf[size_t(k)], h.dbox, h.F);
   if (mg) robin_level(h, k, ROBIN_UPDATE);
-  smooth_level(h, k, h.prob.nu2, h.norms + slot_post(k));
+  smooth_level(h, k, h.prob.nu2, slot_ptr(h, slot_post(k)));
 }
+
+void zero_norms(oll_hier &h) { launch_zero_u64(h.stream, h.norms, h.nslots * kNormSubSlots); }
 
 double slot_value(unsigned long long v) {
   double d;
@@ -238,12 +250,17 @@ double slot_value(unsigned long long v) {
 }
 
 int read_slots(oll_hier &h, std::vector<double> &out) {
-  std::vector<unsigned long long> raw(size_t(h.nslots));
+  std::vector<unsigned long long> raw(size_t(h.nslots) * kNormSubSlots);
   HIP_CHECK(hipMemcpyAsync(raw.data(), h.norms, raw.size() * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, h.stream));
   HIP_CHECK(hipStreamSynchronize(h.stream));
-  out.resize(raw.size());
-  for (size_t i = 0; i < raw.size(); ++i) out[i] = slot_value(raw[i]);
+  if (h.coarse_too_big) return fail(OLL_EINVAL, "coarsest box exceeds the LDS of one workgroup");
+  out.assign(size_t(h.nslots), 0.0);
+  for (int sl = 0; sl < h.nslots; ++sl) {
+    unsigned long long m = 0;
+    for (int q = 0; q < kNormSubSlots; ++q) m = std::max(m, raw[size_t(sl) * kNormSubSlots + q]);
+    out[size_t(sl)] = slot_value(m);
+  }
   return OLL_OK;
 }
 
@@ -292,7 +309,14 @@ int build(oll_hier &h) {
   h.device_bytes = slab_bytes + h.hbox.size() * sizeof(DevBox);
 
   // level views and transfer tables
-  for (int l = 0; l < h.geo.levels; ++l) h.lv.push_back(make_level_view(h, l));
+  for (int l = 0; l < h.geo.levels; ++l) {
+    h.lv.push_back(make_level_view(h, l));
+    // the smoother maps threads to points with 24-bit exact float division
+    const LevelView &v = h.lv.back();
+    const int nc = level_order(h, l) / 2 + 1;
+    if ((long long)v.row0[v.nbox] * ((v.max_row_len + nc - 1) / nc) >= (1LL << 24))
+      return fail(OLL_EINVAL, "a level exceeds 2^24 points per colour");
+  }
   for (int l = 0; l < h.geo.levels; ++l) {
     const int nc = level_order(h, l) / 2 + 1;
     std::vector<double> cnt(size_t(nc), 0.0);
@@ -337,9 +361,15 @@ int build(oll_hier &h) {
       x.dpool = h.dpool;
     }
   h.nslots = 2 * h.geo.levels + 4;
-  HIP_CHECK(hipMalloc(&h.norms, size_t(h.nslots) * sizeof(unsigned long long)));
+  HIP_CHECK(hipMalloc(&h.norms, size_t(h.nslots) * kNormSubSlots * sizeof(unsigned long long)));
   HIP_CHECK(hipMalloc(&h.iters, 4 * sizeof(int)));
-  launch_zero_u64(h.stream, h.norms, h.nslots);
+  zero_norms(h);
+  {
+    // over-relaxation of the coarsest-level loop, tuned for the 13^3-21^3
+    // coarsest boxes the hierarchy produces
+    const int n = h.hbox[size_t(h.geo.level_first[0])].n[0];
+    h.coarse_omega = 2.0 - 1.6 / double(n - 1);
+  }
 
   // coefficients and initial state (operator.cc:752-868, boundary.cc:8-84)
   h.pa.n = p.num_bh;
@@ -562,8 +592,8 @@ int oll_smooth(oll_hier *h, int level, int nu, double *linf) {
   int rc = check_level(h, level);
   if (rc) return rc;
   const int scratch = h->nslots - 1;
-  launch_zero_u64(h->stream, h->norms + scratch, 1);
-  smooth_level(*h, level, nu, h->norms + scratch);
+  zero_norms(*h);
+  smooth_level(*h, level, nu, slot_ptr(*h, scratch));
   HIP_CHECK(hipGetLastError());
   std::vector<double> s;
   if ((rc = read_slots(*h, s))) return rc;
@@ -576,9 +606,9 @@ int oll_residual(oll_hier *h, int level, double *linf) {
   int rc = check_level(h, level);
   if (rc) return rc;
   const int scratch = h->nslots - 1;
-  launch_zero_u64(h->stream, h->norms + scratch, 1);
+  zero_norms(*h);
   launch_residual(h->stream, h->lv[size_t(level)], h->dbox, h->F, level_order(*h, level), h->prob.eq,
-                  0, h->norms + scratch);
+                  0, slot_ptr(*h, scratch));
   HIP_CHECK(hipGetLastError());
   std::vector<double> s;
   if ((rc = read_slots(*h, s))) return rc;
@@ -608,7 +638,7 @@ int oll_fas(oll_hier *h, int level, double *linf) {
   int rc = check_level(h, level);
   if (rc) return rc;
   if (level < 1) return fail(OLL_EINVAL, "FAS step needs level >= 1");
-  launch_zero_u64(h->stream, h->norms, h->nslots);
+  zero_norms(*h);
   fas(*h, level);
   HIP_CHECK(hipGetLastError());
   std::vector<double> s;
@@ -625,14 +655,20 @@ int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
   double r = 1e12, r_prev;
   int cycle = 0, nh = 0;
   std::vector<double> s;
+  h->coarse_sweeps = 0;
   HIP_CHECK(hipEventRecord(h->ev0, h->stream));
   do {
     r_prev = r;
-    launch_zero_u64(h->stream, h->norms, h->nslots);
+    zero_norms(*h);
     fas(*h, top);
     HIP_CHECK(hipGetLastError());
     int rc = read_slots(*h, s);
     if (rc) return rc;
+    {
+      int it = 0;
+      HIP_CHECK(hipMemcpy(&it, h->iters, sizeof(int), hipMemcpyDeviceToHost));
+      h->coarse_sweeps += it;
+    }
     for (int k = top; k >= 0; --k) {
       if (hist && nh < hist_cap) {
         hist[2 * nh] = cycle;
@@ -695,6 +731,12 @@ int oll_kernel_timing(oll_hier *h, int *launches, double *total_ms, double *byte
 int oll_last_solve_ms(const oll_hier *h, double *ms) {
   if (!h || !ms) return fail(OLL_EINVAL, "null argument");
   *ms = h->last_ms;
+  return OLL_OK;
+}
+
+int oll_last_solve_coarse_sweeps(const oll_hier *h, long long *sweeps) {
+  if (!h || !sweeps) return fail(OLL_EINVAL, "null argument");
+  *sweeps = h->coarse_sweeps;
   return OLL_OK;
 }
 

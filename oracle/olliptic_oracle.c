@@ -223,6 +223,7 @@ struct oracle_solver {
   double ps_r0, ps_isigma, ps_phi0, ps_d1, ps_e1, ps_e2, ps_de1;
   double eps_solv, eps_coarse;
   int failed;
+  long coarse_iters;  /* sweeps spent in the coarsest-level loop */
 };
 
 #define IDX(B, i, j, k) ((size_t)(i) + (B)->D.n[0] * ((size_t)(j) + (B)->D.n[1] * (size_t)(k)))
@@ -580,8 +581,12 @@ static void op_at(const oracle_solver *s, const obox *B, size_t i, size_t j,
   }
 }
 
+static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega);
+
 /* elliptic.cc:1126-1242 Smooth; lexicographic or multi-colour order */
-double oracle_smooth(oracle_solver *s, int ibox, int nu) {
+double oracle_smooth(oracle_solver *s, int ibox, int nu) { return smooth_omega(s, ibox, nu, 1.0); }
+
+static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega) {
   obox *B = &s->box[ibox];
   double *u = B->f[OF_U], *r = B->f[OF_R], *rho = B->f[OF_RHO];
   const size_t R = B->D.b[0];
@@ -599,7 +604,10 @@ double oracle_smooth(oracle_solver *s, int ibox, int nu) {
             double N, dN;
             op_at(s, B, i, j, k, &N, &dN);
             r[p] = N - rho[p];
-            u[p] -= r[p] / dN;
+            if (omega == 1.0)
+              u[p] -= r[p] / dN;
+            else
+              u[p] -= omega * (r[p] / dN);
             double f = fabs(r[p]);
             if (Linf < f) Linf = f;
             if (!(u[p] == u[p])) {
@@ -938,6 +946,15 @@ static void hist_add(ohist *H, double v) {
   if (H) H->n++;
 }
 
+/* Relaxation of the coarsest-level loop: 1 (the reference), a given value,
+ * or (negative setting) 2 - 1.6/(n-1) for an n^3 coarsest box, the GPU's
+ * choice. */
+static double coarse_omega(const oracle_solver *s, int ibox) {
+  if (s->p.coarse_omega > 0) return s->p.coarse_omega;
+  if (s->p.coarse_omega < 0) return 2.0 - 1.6 / (double)(s->box[ibox].D.n[0] - 1);
+  return 1.0;
+}
+
 static double fasn(oracle_solver *s, int k, ohist *H) {
   const int M = s->mgrid;
   double norm_k = 0, norm_km1 = 0;
@@ -989,7 +1006,8 @@ static double fasn(oracle_solver *s, int k, ohist *H) {
       do {
         norm_prev = norm_km1;
         oracle_robin(s, fc + j);
-        norm_km1 = oracle_smooth(s, fc + j, 1);
+        norm_km1 = smooth_omega(s, fc + j, 1, coarse_omega(s, fc + j));
+        s->coarse_iters++;
         if (s->failed) return NAN;
       } while (norm_km1 > s->eps_coarse && norm_prev != norm_km1);
     }
@@ -1211,6 +1229,7 @@ void oracle_destroy(oracle_solver *s) {
 }
 
 int oracle_num_levels(const oracle_solver *s) { return s->levels; }
+long oracle_coarse_iterations(const oracle_solver *s) { return s->coarse_iters; }
 int oracle_mgrid_levels(const oracle_solver *s) { return s->mgrid; }
 int oracle_num_boxes(const oracle_solver *s) { return s->nbox; }
 int oracle_level_first_box(const oracle_solver *s, int level) { return s->lvl_first[level]; }

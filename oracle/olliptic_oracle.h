@@ -56,6 +56,9 @@ typedef struct oracle_params {
                            where the 27-point stencil lies in the fine
                            interior (the GPU's choice); 0: the reference's
                            injection */
+  double coarse_omega;  /* over-relaxation of the coarsest-level loop:
+                           0 plain Gauss-Seidel (the reference), >0 that
+                           factor, <0 2 - 1.6/(n-1) (the GPU's choice) */
 } oracle_params;
 
 typedef struct oracle_solver oracle_solver;
@@ -67,6 +70,8 @@ void oracle_destroy(oracle_solver *s);
 const char *oracle_last_error(void);
 
 int oracle_num_levels(const oracle_solver *s);   /* incl. multigrid levels */
+/* Sweeps spent in the coarsest-level loop since creation. */
+long oracle_coarse_iterations(const oracle_solver *s);
 int oracle_mgrid_levels(const oracle_solver *s);
 int oracle_num_boxes(const oracle_solver *s);
 int oracle_level_first_box(const oracle_solver *s, int level);

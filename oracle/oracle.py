@@ -34,7 +34,7 @@ class Params(C.Structure):
         ("ps_r0", C.c_double), ("ps_sigma", C.c_double),
         ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
         ("sweep_order", C.c_int), ("max_cycles", C.c_int),
-        ("restrict_fw", C.c_int),
+        ("restrict_fw", C.c_int), ("coarse_omega", C.c_double),
     ]
 
 
@@ -82,6 +82,8 @@ def lib():
         L.oracle_interp_point.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.c_double, C.c_double, C.c_double]
         L.oracle_interp_point.restype = C.c_double
+        L.oracle_coarse_iterations.argtypes = [vp]
+        L.oracle_coarse_iterations.restype = C.c_long
         L.oracle_valid_n.argtypes = [C.c_size_t]
         L.oracle_valid_n.restype = C.c_size_t
         _lib = L
@@ -142,6 +144,8 @@ def params_from_argv(argv, sweep_order=LEX, max_cycles=0, restrict_fw=None):
     p.sweep_order = sweep_order
     p.max_cycles = max_cycles
     p.restrict_fw = (sweep_order == COLOUR) if restrict_fw is None else int(restrict_fw)
+    # the GPU's coarsest-level over-relaxation goes with its colour order
+    p.coarse_omega = -1.0 if sweep_order == COLOUR else 0.0
     return p
 
 
@@ -228,6 +232,9 @@ class Solver:
         s = (C.c_double * self.nbox)()
         self.L.oracle_adm_mass(self.h, p, s)
         return list(p), list(s)
+
+    def coarse_iterations(self):
+        return self.L.oracle_coarse_iterations(self.h)
 
     def interp(self, ib, fid, method, ord_, x, y, z):
         return self.L.oracle_interp_point(self.h, ib, fid, method, ord_, x, y, z)
