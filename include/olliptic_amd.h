@@ -138,6 +138,10 @@ int oll_robin(oll_hier *h, int level);
 int oll_boundary_operator(oll_hier *h, int level);
 /* elliptic.cc:1469-1968 FASN(k): one FAS step of level k >= 1 (recursive). */
 int oll_fas(oll_hier *h, int level, double *linf);
+/* Per-level norms of the last oll_fas / last V-cycle of oll_solve: pre[k]
+ * the pre-smoothing norm of level k (the coarsest-loop norm for k = 0),
+ * post[k] the post-smoothing norm (output.cc:1355-1594 Print_Pre/Post_Cycle). */
+int oll_last_norms(const oll_hier *h, double *pre, double *post, int nlevels);
 /* elliptic.cc:1251-1342 Vcycle loop; hist (2*cap doubles) may be NULL. */
 int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap);
 /* elliptic.cc:2447-2611 Compute_ADM_Mass: per puncture and per box. */

@@ -79,6 +79,7 @@ struct oll_hier {
   double coarse_omega = 1.0;
   int coarse_too_big = 0;
   long long coarse_sweeps = 0;  // coarsest-level sweeps of the last oll_solve
+  std::vector<double> last_slots;  // norms of the last FAS step / V-cycle
   // optional per-launch timing of the smoother (events around each launch)
   int timing = 0;
   std::vector<hipEvent_t> tev;
@@ -643,6 +644,7 @@ int oll_fas(oll_hier *h, int level, double *linf) {
   HIP_CHECK(hipGetLastError());
   std::vector<double> s;
   if ((rc = read_slots(*h, s))) return rc;
+  h->last_slots = s;
   if (std::isinf(s[size_t(slot_post(level))])) return fail(OLL_ENAN, "NAN check error in the smoother");
   if (linf) *linf = s[size_t(slot_post(level))];
   return OLL_OK;
@@ -677,6 +679,7 @@ int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
       ++nh;
     }
     r = s[size_t(slot_post(top))];
+    h->last_slots = s;
     ++cycle;
     for (int k = 0; k <= top; ++k)
       if (std::isinf(s[size_t(slot_pre(k))]) || std::isinf(s[size_t(slot_post(k))]))
@@ -731,6 +734,16 @@ int oll_kernel_timing(oll_hier *h, int *launches, double *total_ms, double *byte
 int oll_last_solve_ms(const oll_hier *h, double *ms) {
   if (!h || !ms) return fail(OLL_EINVAL, "null argument");
   *ms = h->last_ms;
+  return OLL_OK;
+}
+
+int oll_last_norms(const oll_hier *h, double *pre, double *post, int nlevels) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  for (int k = 0; k < nlevels && k < h->geo.levels; ++k) {
+    const bool have = !h->last_slots.empty();
+    if (pre) pre[k] = have ? h->last_slots[size_t(slot_pre(k))] : 0.0;
+    if (post) post[k] = have ? h->last_slots[size_t(slot_post(k))] : 0.0;
+  }
   return OLL_OK;
 }
 

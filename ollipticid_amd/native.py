@@ -78,6 +78,7 @@ SIGNATURES = {
     "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oll_last_solve_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "oll_reset": (C.c_int, [_vp]),
+    "oll_last_norms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "oll_last_solve_coarse_sweeps": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
     "oll_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "oll_kernel_timing": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
@@ -305,6 +306,12 @@ class Hierarchy:
         s = (C.c_double * self.nbox)()
         _check(self.L.oll_adm_mass(self.h, p, s))
         return list(p), list(s)
+
+    def last_norms(self):
+        pre = (C.c_double * self.levels)()
+        post = (C.c_double * self.levels)()
+        _check(self.L.oll_last_norms(self.h, pre, post, self.levels))
+        return list(pre), list(post)
 
     def last_solve_coarse_sweeps(self):
         out = C.c_longlong()

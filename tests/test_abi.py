@@ -1,0 +1,69 @@
+"""CPU checks of the drop-in boundary: the HIP library loads, exports every
+function include/olliptic_amd.h declares, the Python binding declares the
+same set, and the host-only geometry entry points reproduce the reference's
+box tables (tests/golden)."""
+import ctypes
+import glob
+import json
+import os
+import re
+
+import pytest
+
+import ollipticid_amd as oa
+from ollipticid_amd import native
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "olliptic_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\**\s*(oll_[a-z_0-9]+)\s*\(",
+                                 text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "oll_hier_create" in names and "oll_solve" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(native.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared_functions()) == set(native.SIGNATURES)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(REPO, "tests", "golden", "*.json"))))
+def test_box_tables_match_reference(path):
+    meta = json.load(open(path))
+    boxes, mgrid = oa.problem_boxes(oa.problem_from_argv(meta["argv"]))
+    assert [(b["level"], b["n"]) for b in boxes] == [(b["level"], b["n"]) for b in meta["boxes"]]
+    for b, r in zip(boxes, meta["boxes"]):
+        # the fixtures print 12 significant digits
+        for a in range(3):
+            assert abs(b["origin"][a] - r["origin"][a]) <= 1e-11 * max(1.0, abs(r["origin"][a]))
+            assert abs(b["d"][a] - r["dx"][a]) <= 1e-11 * r["dx"][a]
+
+
+def test_geometry_errors_are_codes_not_exits():
+    p = oa.problem_from_argv(["-eq", "Punctures", "-B", "Robin", "-N", "17", "-L", "10"])
+    with pytest.raises(oa.OllError) as e:
+        oa.problem_boxes(p)
+    assert e.value.code == -2 and "enough grid-points" in str(e.value)
+
+
+def test_cli_parsing_rules():
+    # repeated flag keeps the last value; unknown flags ignored; -bh* positional
+    p = oa.problem_from_argv(["-eq", "Punctures", "-N", "33", "-N", "65", "-foo", "1",
+                              "-NumBH", "2", "-bhmp2", "0.7", "-bhy1", "3.5"])
+    assert p.N == 65 and p.num_bh == 2
+    assert p.bh[1].m == 0.7 and p.bh[0].y == 3.5 and p.bh[0].m == 0.0
+    # fewer than two arguments: nothing parsed (interface.cc:373)
+    q = oa.problem_from_argv(["-N"])
+    assert q.N == 30
