@@ -32,6 +32,7 @@ extern "C" {
 #define OLL_EDEVICE (-3)    /* HIP runtime / kernel failure */
 #define OLL_ENAN (-4)       /* NaN produced by the smoother */
 #define OLL_ENOMEM (-5)     /* device allocation failed */
+#define OLL_EBUSY (-6)      /* engine: admitting would exceed the HBM budget */
 
 /* equations (interface.cc:1184-1323 names "Punctures", "Punctures_scalar_field") */
 #define OLL_EQ_PUNCTURES 0
@@ -160,6 +161,30 @@ int oll_reset(oll_hier *h);
  * updated point: u, b, c, rho read and u written; 56 B with d, e). */
 int oll_set_kernel_timing(oll_hier *h, int on);
 int oll_kernel_timing(oll_hier *h, int *launches, double *total_ms, double *bytes);
+/* V-cycles of oll_solve are captured once as a HIP graph and replayed
+ * (default on); off issues every launch from the host. */
+int oll_set_graphs(oll_hier *h, int on);
+
+/* ------------------------------------------------------------------
+ * Engine: many resident requests solved together (no reference
+ * counterpart: the reference solves one problem per process).  Requests of
+ * any N, level count, order and equation are admitted while their HBM
+ * footprint fits the budget; oll_engine_step runs one V-cycle of every
+ * active request, one launch per stage covering all of them, and moves
+ * requests that met the stopping rule to the done queue.
+ * ------------------------------------------------------------------ */
+typedef struct oll_engine oll_engine;
+/* hbm_budget 0: no limit. */
+int oll_engine_create(size_t hbm_budget, int use_graphs, oll_engine **out);
+void oll_engine_destroy(oll_engine *e);
+/* OLL_EBUSY when the request does not fit the remaining budget. */
+int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag);
+int oll_engine_step(oll_engine *e, int *active_after);
+/* Pops one finished request: returns 1 and fills tag, result and the
+ * puncture ADM masses (num_bh values), 0 when none is finished, <0 on error
+ * (the request's own failure, e.g. OLL_ENAN).  Frees its HBM. */
+int oll_engine_pop_done(oll_engine *e, long long *tag, oll_result *res, double *adm);
+int oll_engine_stats(const oll_engine *e, size_t *bytes_in_use, int *active, int *done_pending);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@ def build(verbose=True):
     geo_o = os.path.join(objdir, "geometry.o")
     _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "geometry.cpp"), "-o", geo_o])
     objs = [geo_o]
-    for src in ("kernels.hip", "solver.hip"):
+    for src in ("kernels.hip", "engine.hip", "capi.hip"):
         o = os.path.join(objdir, src.replace(".hip", ".o"))
         _run([HIPCC] + DEVICE_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

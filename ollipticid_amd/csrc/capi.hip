@@ -1,0 +1,510 @@
+// C ABI of include/olliptic_amd.h over the engine (engine.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/olliptic_amd.h"
+#include "engine.hpp"
+
+using namespace oll;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define CAPI_CHECK(expr)                                                             \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(OLL_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+#define CAPI_RC(expr)                   \
+  do {                                  \
+    std::string e_;                     \
+    const int rc_ = (expr);             \
+    if (rc_ != OLL_OK) return fail(rc_, e_); \
+  } while (0)
+
+}  // namespace
+
+struct oll_hier {
+  Hier H;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Schedule vcycle;   // top-level V-cycle (graph-replayed)
+  Schedule scratch;  // single stages
+  Timing timing;
+  double last_ms = 0;
+  bool graphs = true;
+};
+
+namespace {
+
+int hier_check(const oll_hier *h, int level) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  if (level < 0 || level >= h->H.geo.levels) return fail(OLL_EINVAL, "level out of range");
+  return OLL_OK;
+}
+
+int run_stage(oll_hier *h, int level, int stage, int nu, double *linf) {
+  std::string err;
+  int rc = build_stage(h->H, level, stage, nu, h->scratch, err);
+  if (!rc) rc = run_schedule(h->scratch, h->stream, nullptr, false, err);
+  if (!rc && linf) {
+    rc = read_norms(h->H, h->stream, err);
+    if (!rc) {
+      if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(OLL_EDEVICE, "sync");
+      decode_norms(h->H);
+      *linf = h->H.last_slots[size_t(h->H.slot_scratch())];
+      if (std::isinf(*linf)) return fail(OLL_ENAN, "NAN check error in the smoother");
+    }
+  } else if (!rc) {
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(OLL_EDEVICE, "sync");
+  }
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *oll_last_error(void) { return g_err.c_str(); }
+
+void oll_problem_default(oll_problem *p) {
+  std::memset(p, 0, sizeof *p);
+  p->eq = OLL_EQ_PUNCTURES;
+  p->num_bh = 1;
+  p->L = 2.0;
+  p->N = 30;
+  p->levels = 1;
+  p->order = 4;
+  p->nu1 = 3;
+  p->nu2 = 3;
+  p->tol = 1e-8;
+  p->robin_a = 0.0;
+  p->robin_n = 1;
+  p->ps_case = 1;
+  p->ps_r0 = 120;
+  p->ps_sigma = 8;
+  p->ps_phi0 = 0.025;
+  p->ps_a2 = 1;
+  p->precision = OLL_PREC_FP64;
+}
+
+int oll_hier_create(const oll_problem *p, oll_hier **out) {
+  g_err.clear();
+  if (!p || !out) return fail(OLL_EINVAL, "null argument");
+  *out = nullptr;
+  kernels_init();
+  oll_hier *h = new (std::nothrow) oll_hier();
+  if (!h) return fail(OLL_ENOMEM, "host allocation failed");
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(OLL_EDEVICE, "hipStreamCreate failed");
+  }
+  (void)hipEventCreate(&h->ev0);
+  (void)hipEventCreate(&h->ev1);
+  std::string err;
+  const int rc = hier_build(h->H, *p, h->stream, err);
+  if (rc) {
+    oll_hier_destroy(h);
+    return fail(rc, err);
+  }
+  *out = h;
+  return OLL_OK;
+}
+
+void oll_hier_destroy(oll_hier *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  release_schedule(h->vcycle);
+  release_schedule(h->scratch);
+  hier_release(h->H);
+  for (hipEvent_t e : h->timing.ev) (void)hipEventDestroy(e);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int oll_num_levels(const oll_hier *h) { return h ? h->H.geo.levels : 0; }
+int oll_mgrid_levels(const oll_hier *h) { return h ? h->H.geo.mgrid : 0; }
+int oll_num_boxes(const oll_hier *h) { return h ? int(h->H.geo.boxes.size()) : 0; }
+
+int oll_level_boxes(const oll_hier *h, int level, int *first, int *count) {
+  int rc = hier_check(h, level);
+  if (rc) return rc;
+  if (first) *first = h->H.geo.level_first[size_t(level)];
+  if (count) *count = h->H.geo.level_count[size_t(level)];
+  return OLL_OK;
+}
+
+static void fill_desc(const Box &b, oll_box_desc *out) {
+  out->level = b.level;
+  out->order = b.order;
+  out->ghost = b.R;
+  for (int a = 0; a < 3; ++a) {
+    out->n[a] = int(b.n[a]);
+    out->origin[a] = b.lo[a];
+    out->d[a] = b.d[a];
+    out->centre[a] = b.c[a];
+  }
+}
+
+int oll_box_info(const oll_hier *h, int ibox, oll_box_desc *out) {
+  if (!h || !out) return fail(OLL_EINVAL, "null argument");
+  if (ibox < 0 || ibox >= int(h->H.geo.boxes.size())) return fail(OLL_EINVAL, "box out of range");
+  fill_desc(h->H.geo.boxes[size_t(ibox)], out);
+  return OLL_OK;
+}
+
+size_t oll_device_bytes(const oll_hier *h) { return h ? h->H.bytes : 0; }
+
+int oll_problem_boxes(const oll_problem *p, oll_box_desc *out, int cap, int *nbox, int *mgrid) {
+  if (!p || !nbox) return fail(OLL_EINVAL, "null argument");
+  PunctureSet ps;
+  for (int i = 0; i < p->num_bh && i < OLL_MAX_BH; ++i) {
+    ps.x.push_back(p->bh[i].x);
+    ps.y.push_back(p->bh[i].y);
+    ps.z.push_back(p->bh[i].z);
+  }
+  Hierarchy g;
+  const std::string err = build_hierarchy(size_t(p->N), p->L, p->levels, p->order, ps, g);
+  if (!err.empty()) return fail(err.find("enough") != std::string::npos ? OLL_EGRID : OLL_EINVAL, err);
+  *nbox = int(g.boxes.size());
+  if (mgrid) *mgrid = g.mgrid;
+  for (int i = 0; i < *nbox && i < cap && out; ++i) fill_desc(g.boxes[size_t(i)], &out[i]);
+  return OLL_OK;
+}
+
+int oll_problem_footprint(const oll_problem *p, size_t *bytes) {
+  if (!p || !bytes) return fail(OLL_EINVAL, "null argument");
+  std::string err;
+  const int rc = problem_footprint(*p, *bytes, err);
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
+int oll_get_field(oll_hier *h, int ibox, int field, double *host) {
+  if (!h || !host) return fail(OLL_EINVAL, "null argument");
+  if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= NFIELDS)
+    return fail(OLL_EINVAL, "box or field out of range");
+  const DevBox &b = h->H.hbox[size_t(ibox)];
+  const size_t n = size_t(b.n[0]) * size_t(b.n[1]) * size_t(b.n[2]);
+  CAPI_CHECK(hipMemcpyAsync(host, b.f[field], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CAPI_CHECK(hipStreamSynchronize(h->stream));
+  return OLL_OK;
+}
+
+int oll_set_field(oll_hier *h, int ibox, int field, const double *host) {
+  if (!h || !host) return fail(OLL_EINVAL, "null argument");
+  if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= NFIELDS)
+    return fail(OLL_EINVAL, "box or field out of range");
+  const DevBox &b = h->H.hbox[size_t(ibox)];
+  const size_t n = size_t(b.n[0]) * size_t(b.n[1]) * size_t(b.n[2]);
+  CAPI_CHECK(hipMemcpyAsync(b.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  CAPI_CHECK(hipStreamSynchronize(h->stream));
+  return OLL_OK;
+}
+
+int oll_smooth(oll_hier *h, int level, int nu, double *linf) {
+  int rc = hier_check(h, level);
+  if (rc) return rc;
+  double v = 0;
+  rc = run_stage(h, level, STAGE_SMOOTH, nu, &v);
+  if (linf) *linf = v;
+  return rc;
+}
+
+int oll_residual(oll_hier *h, int level, double *linf) {
+  int rc = hier_check(h, level);
+  if (rc) return rc;
+  double v = 0;
+  rc = run_stage(h, level, STAGE_RESIDUAL, 0, &v);
+  if (linf) *linf = v;
+  return rc;
+}
+
+int oll_robin(oll_hier *h, int level) {
+  int rc = hier_check(h, level);
+  return rc ? rc : run_stage(h, level, STAGE_ROBIN, 0, nullptr);
+}
+
+int oll_boundary_operator(oll_hier *h, int level) {
+  int rc = hier_check(h, level);
+  return rc ? rc : run_stage(h, level, STAGE_BOUNDARY_OP, 0, nullptr);
+}
+
+int oll_fas(oll_hier *h, int level, double *linf) {
+  int rc = hier_check(h, level);
+  if (rc) return rc;
+  if (level < 1) return fail(OLL_EINVAL, "FAS step needs level >= 1");
+  std::string err;
+  std::vector<Hier *> M{&h->H};
+  if ((rc = build_vcycle_from(M, {level}, h->scratch, err))) return fail(rc, err);
+  if ((rc = run_schedule(h->scratch, h->stream, nullptr, false, err))) return fail(rc, err);
+  if ((rc = read_norms(h->H, h->stream, err))) return fail(rc, err);
+  CAPI_CHECK(hipStreamSynchronize(h->stream));
+  decode_norms(h->H);
+  const double v = h->H.last_slots[size_t(Hier::slot_post(level))];
+  if (std::isinf(v)) return fail(OLL_ENAN, "NAN check error in the smoother");
+  if (linf) *linf = v;
+  return OLL_OK;
+}
+
+int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  if (h->H.top() < 1) return fail(OLL_EINVAL, "hierarchy has a single level");
+  std::string err;
+  int rc;
+  if (h->vcycle.launches.empty()) {
+    std::vector<Hier *> M{&h->H};
+    if ((rc = build_vcycle(M, h->vcycle, err))) return fail(rc, err);
+  }
+  Hier &H = h->H;
+  H.cycle = 0;
+  H.res = H.res_prev = 1e12;
+  H.done = false;
+  H.status = 0;
+  H.hist.clear();
+  H.coarse_sweeps = 0;
+  CAPI_CHECK(hipEventRecord(h->ev0, h->stream));
+  while (!H.done) {
+    if ((rc = run_schedule(h->vcycle, h->stream, &h->timing, h->graphs, err))) return fail(rc, err);
+    if ((rc = finish_cycle(H, h->stream, err))) return fail(rc, err);
+  }
+  CAPI_CHECK(hipEventRecord(h->ev1, h->stream));
+  CAPI_CHECK(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  CAPI_CHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  const int nh = int(H.hist.size() / 2);
+  for (int i = 0; hist && i < nh && i < hist_cap; ++i) {
+    hist[2 * i] = H.hist[size_t(2 * i)];
+    hist[2 * i + 1] = H.hist[size_t(2 * i + 1)];
+  }
+  if (res) {
+    res->cycles = H.cycle;
+    res->residual = H.res;
+    res->hist_len = nh;
+  }
+  return OLL_OK;
+}
+
+int oll_last_norms(const oll_hier *h, double *pre, double *post, int nlevels) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  const std::vector<double> &s = h->H.last_slots;
+  for (int k = 0; k < nlevels && k < h->H.geo.levels; ++k) {
+    if (pre) pre[k] = s.empty() ? 0.0 : s[size_t(Hier::slot_pre(k))];
+    if (post) post[k] = s.empty() ? 0.0 : s[size_t(Hier::slot_post(k))];
+  }
+  return OLL_OK;
+}
+
+int oll_last_solve_ms(const oll_hier *h, double *ms) {
+  if (!h || !ms) return fail(OLL_EINVAL, "null argument");
+  *ms = h->last_ms;
+  return OLL_OK;
+}
+
+int oll_last_solve_coarse_sweeps(const oll_hier *h, long long *sweeps) {
+  if (!h || !sweeps) return fail(OLL_EINVAL, "null argument");
+  *sweeps = h->H.coarse_sweeps;
+  return OLL_OK;
+}
+
+int oll_reset(oll_hier *h) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  std::string err;
+  const int rc = hier_reset(h->H, h->stream, err);
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
+int oll_set_kernel_timing(oll_hier *h, int on) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  CAPI_CHECK(hipStreamSynchronize(h->stream));
+  h->timing.on = on != 0;
+  h->timing.used = 0;
+  h->timing.bytes = 0;
+  return OLL_OK;
+}
+
+int oll_kernel_timing(oll_hier *h, int *launches, double *total_ms, double *bytes) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  CAPI_CHECK(hipStreamSynchronize(h->stream));
+  double tot = 0;
+  for (size_t i = 0; i + 1 < h->timing.used; i += 2) {
+    float ms = 0;
+    CAPI_CHECK(hipEventElapsedTime(&ms, h->timing.ev[i], h->timing.ev[i + 1]));
+    tot += ms;
+  }
+  if (launches) *launches = int(h->timing.used / 2);
+  if (total_ms) *total_ms = tot;
+  if (bytes) *bytes = h->timing.bytes;
+  return OLL_OK;
+}
+
+int oll_set_graphs(oll_hier *h, int on) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  h->graphs = on != 0;
+  return OLL_OK;
+}
+
+int oll_adm_mass(oll_hier *h, double *puncture, double *surface) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  std::string err;
+  const int rc = adm_mass(h->H, h->stream, puncture, surface, err);
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
+// ---------------------------------------------------------------------
+// engine: many resident requests, V-cycles in lockstep
+// ---------------------------------------------------------------------
+}  // extern "C"
+
+struct oll_engine {
+  hipStream_t stream = nullptr;
+  size_t budget = 0, in_use = 0;
+  bool graphs = true;
+  struct Member {
+    std::unique_ptr<Hier> H;
+    long long tag;
+  };
+  std::vector<Member> active;
+  std::deque<Member> done;
+  // one schedule per (nu1, nu2) class of the active members
+  std::map<std::pair<int, int>, Schedule> sched;
+  bool dirty = true;
+  long long cycles_run = 0;
+};
+
+extern "C" {
+
+int oll_engine_create(size_t hbm_budget, int use_graphs, oll_engine **out) {
+  g_err.clear();
+  if (!out) return fail(OLL_EINVAL, "null argument");
+  kernels_init();
+  oll_engine *e = new (std::nothrow) oll_engine();
+  if (!e) return fail(OLL_ENOMEM, "host allocation failed");
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(OLL_EDEVICE, "hipStreamCreate failed");
+  }
+  e->budget = hbm_budget;
+  e->graphs = use_graphs != 0;
+  *out = e;
+  return OLL_OK;
+}
+
+void oll_engine_destroy(oll_engine *e) {
+  if (!e) return;
+  (void)hipStreamSynchronize(e->stream);
+  for (auto &kv : e->sched) release_schedule(kv.second);
+  for (auto &m : e->active) hier_release(*m.H);
+  for (auto &m : e->done) hier_release(*m.H);
+  (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag) {
+  if (!e || !p) return fail(OLL_EINVAL, "null argument");
+  std::string err;
+  size_t need = 0;
+  int rc = problem_footprint(*p, need, err);
+  if (rc) return fail(rc, err);
+  if (e->budget && e->in_use + need > e->budget) return fail(OLL_EBUSY, "HBM budget exhausted");
+  oll_engine::Member m;
+  m.H.reset(new (std::nothrow) Hier());
+  m.tag = tag;
+  if (!m.H) return fail(OLL_ENOMEM, "host allocation failed");
+  if ((rc = hier_build(*m.H, *p, e->stream, err))) {
+    hier_release(*m.H);
+    return fail(rc, err);
+  }
+  e->in_use += m.H->bytes;
+  e->active.push_back(std::move(m));
+  e->dirty = true;
+  return OLL_OK;
+}
+
+int oll_engine_step(oll_engine *e, int *active_after) {
+  if (!e) return fail(OLL_EINVAL, "null handle");
+  std::string err;
+  int rc;
+  if (e->dirty) {
+    for (auto &kv : e->sched) release_schedule(kv.second);
+    e->sched.clear();
+    std::map<std::pair<int, int>, std::vector<Hier *>> cls;
+    for (auto &m : e->active) cls[{m.H->prob.nu1, m.H->prob.nu2}].push_back(m.H.get());
+    for (auto &kv : cls)
+      if ((rc = build_vcycle(kv.second, e->sched[kv.first], err))) return fail(rc, err);
+    e->dirty = false;
+  }
+  for (auto &kv : e->sched)
+    if ((rc = run_schedule(kv.second, e->stream, nullptr, e->graphs, err))) return fail(rc, err);
+  for (auto &m : e->active)
+    if ((rc = read_norms(*m.H, e->stream, err))) return fail(rc, err);
+  CAPI_CHECK(hipStreamSynchronize(e->stream));
+  ++e->cycles_run;
+  std::vector<oll_engine::Member> still;
+  for (auto &m : e->active) {
+    std::string merr;
+    apply_cycle(*m.H, merr);
+    if (m.H->done) {
+      e->done.push_back(std::move(m));
+      e->dirty = true;
+    } else {
+      still.push_back(std::move(m));
+    }
+  }
+  e->active = std::move(still);
+  if (active_after) *active_after = int(e->active.size());
+  return OLL_OK;
+}
+
+int oll_engine_pop_done(oll_engine *e, long long *tag, oll_result *res, double *adm) {
+  if (!e) return fail(OLL_EINVAL, "null handle");
+  if (e->done.empty()) return 0;
+  oll_engine::Member m = std::move(e->done.front());
+  e->done.pop_front();
+  Hier &H = *m.H;
+  if (tag) *tag = m.tag;
+  if (res) {
+    res->cycles = H.cycle;
+    res->residual = H.res;
+    res->hist_len = int(H.hist.size() / 2);
+  }
+  int rc = H.status;
+  if (!rc && adm) {
+    std::string err;
+    rc = adm_mass(H, e->stream, adm, nullptr, err);
+    if (rc) g_err = err;
+  }
+  e->in_use -= std::min(e->in_use, H.bytes);
+  hier_release(H);
+  if (H.status) return fail(H.status, "NAN check error in the smoother");
+  return rc ? rc : 1;
+}
+
+int oll_engine_stats(const oll_engine *e, size_t *bytes_in_use, int *active, int *done_pending) {
+  if (!e) return fail(OLL_EINVAL, "null handle");
+  if (bytes_in_use) *bytes_in_use = e->in_use;
+  if (active) *active = int(e->active.size());
+  if (done_pending) *done_pending = int(e->done.size());
+  return OLL_OK;
+}
+
+}  // extern "C"

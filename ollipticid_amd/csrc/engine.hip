@@ -1,0 +1,843 @@
+// Host engine; see engine.hpp.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+namespace oll {
+
+namespace {
+
+#define ENG_CHECK(expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);                             \
+      return OLL_EDEVICE;                                                                  \
+    }                                                                                      \
+  } while (0)
+
+double dulap_of(int order, const double idx2[3]) {
+  const double s = idx2[0] + idx2[1] + idx2[2];
+  switch (order) {
+    case 2: return -2.0 * s;
+    case 4: return -2.5 * s;
+    case 6: return -490.0 * s / 180.0;
+    default: return -205.0 * s / 72.0;
+  }
+}
+
+long long npts(const DevBox &b) { return (long long)b.n[0] * b.n[1] * b.n[2]; }
+long long shell_pts(const DevBox &b) {
+  return npts(b) - (long long)(b.n[0] - 2 * b.R) * (b.n[1] - 2 * b.R) * (b.n[2] - 2 * b.R);
+}
+int interior_rows(const DevBox &b) { return (b.n[1] - 2 * b.R) * (b.n[2] - 2 * b.R); }
+int face_columns(const DevBox &b) {
+  const int ni = b.n[0] - 2 * b.R, nj = b.n[1] - 2 * b.R, nk = b.n[2] - 2 * b.R;
+  return 2 * (nj * nk + ni * nk + ni * nj);
+}
+
+int validate(const oll_problem &p, std::string &err) {
+  if (p.num_bh < 1 || p.num_bh > OLL_MAX_BH) return err = "NumBH out of range", OLL_EINVAL;
+  if (!(p.order == 2 || p.order == 4 || p.order == 6 || p.order == 8))
+    return err = "order must be [2,4,6,8].", OLL_EINVAL;
+  if (p.levels < 1) return err = "levels must be >= 1", OLL_EINVAL;
+  if (p.N < 3) return err = "N too small", OLL_EINVAL;
+  if (p.nu1 < 0 || p.nu2 < 0) return err = "negative sweep count", OLL_EINVAL;
+  if (p.robin_n < 1) return err = "Robin fall-off power must be >= 1", OLL_EINVAL;
+  if (p.eq != OLL_EQ_PUNCTURES && p.eq != OLL_EQ_PUNCTURES_SCALAR)
+    return err = "unknown equation", OLL_EINVAL;
+  if (p.eq == OLL_EQ_PUNCTURES)
+    for (int i = 0; i < p.num_bh; ++i)
+      if (p.bh[i].m <= 0) return err = "Wrong punture mass parameters", OLL_EINVAL;
+  if (p.precision != OLL_PREC_FP64) return err = "only OLL_PREC_FP64 is available", OLL_EINVAL;
+  return OLL_OK;
+}
+
+int geometry(const oll_problem &p, Hierarchy &g, std::string &err) {
+  PunctureSet ps;
+  for (int i = 0; i < p.num_bh; ++i) {
+    ps.x.push_back(p.bh[i].x);
+    ps.y.push_back(p.bh[i].y);
+    ps.z.push_back(p.bh[i].z);
+  }
+  err = build_hierarchy(size_t(p.N), p.L, p.levels, p.order, ps, g);
+  if (!err.empty()) return err.find("enough") != std::string::npos ? OLL_EGRID : OLL_EINVAL;
+  return OLL_OK;
+}
+
+// transfer tables of one hierarchy, built on the host
+struct Pools {
+  std::vector<int> ip;
+  std::vector<unsigned char> bp;
+  std::vector<double> dp;
+  struct X {
+    int src;
+    int start[3], coinc[3], flags[3], w[3], k[3], n[3];
+  };
+  std::vector<X> xs;
+
+  Hier::XRef add(const Hierarchy &g, int tbox, int slevel, int method, int ord) {
+    Hier::XRef r;
+    r.off = int(xs.size());
+    const int fs = g.level_first[size_t(slevel)], ns = g.level_count[size_t(slevel)];
+    for (int s = 0; s < ns; ++s) {
+      X x{};
+      x.src = fs + s;
+      for (int a = 0; a < 3; ++a) {
+        const AxisTable t = build_axis_table(g.boxes[size_t(tbox)], g.boxes[size_t(fs + s)], a, method, ord);
+        x.start[a] = int(ip.size());
+        ip.insert(ip.end(), t.start.begin(), t.start.end());
+        x.coinc[a] = int(ip.size());
+        ip.insert(ip.end(), t.coinc.begin(), t.coinc.end());
+        x.flags[a] = int(bp.size());
+        bp.insert(bp.end(), t.flags.begin(), t.flags.end());
+        x.w[a] = int(dp.size());
+        dp.insert(dp.end(), t.w.begin(), t.w.end());
+        x.k[a] = t.k;
+        x.n[a] = int(t.start.size());
+      }
+      xs.push_back(x);
+    }
+    r.cnt = int(xs.size()) - r.off;
+    return r;
+  }
+};
+
+template <typename T>
+int upload(T **dst, const std::vector<T> &v, size_t &bytes, std::string &err) {
+  const size_t n = std::max<size_t>(1, v.size()) * sizeof(T);
+  ENG_CHECK(hipMalloc(reinterpret_cast<void **>(dst), n));
+  if (!v.empty()) ENG_CHECK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  bytes += n;
+  return OLL_OK;
+}
+
+}  // namespace
+
+int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err) {
+  int rc = validate(p, err);
+  if (rc) return rc;
+  Hierarchy g;
+  if ((rc = geometry(p, g, err))) return rc;
+  size_t pts = 0, tables = 0;
+  for (const Box &b : g.boxes) pts += b.npts();
+  for (int l = 1; l < g.levels; ++l)
+    for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
+      const Box &B = g.boxes[size_t(g.level_first[size_t(l)] + b)];
+      const size_t nsrc = size_t(g.level_count[size_t(l - 1)]);
+      tables += nsrc * (B.n[0] + B.n[1] + B.n[2]) * (9 + 8 * size_t(p.order + 2)) * 2;
+    }
+  bytes = pts * sizeof(double) * NFIELDS + tables + (64 << 10);
+  return OLL_OK;
+}
+
+int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
+  int rc = validate(p, err);
+  if (rc) return rc;
+  H.prob = p;
+  if ((rc = geometry(p, H.geo, err))) return rc;
+  const Hierarchy &g = H.geo;
+  // slab: field-major, boxes back to back
+  long long total = 0;
+  std::vector<long long> off;
+  for (const Box &b : g.boxes) {
+    off.push_back(total);
+    total += (long long)b.npts();
+  }
+  const size_t slab_bytes = size_t(total) * sizeof(double) * NFIELDS;
+  hipError_t e = hipMalloc(&H.slab, slab_bytes);
+  if (e != hipSuccess) {
+    err = std::string("hipMalloc slab: ") + hipGetErrorString(e);
+    return OLL_ENOMEM;
+  }
+  H.bytes += slab_bytes;
+  ENG_CHECK(hipMemsetAsync(H.slab, 0, slab_bytes, s));
+  const double ipn = 1.0 / double(p.robin_n);
+  for (size_t ib = 0; ib < g.boxes.size(); ++ib) {
+    const Box &b = g.boxes[ib];
+    DevBox d{};
+    for (int a = 0; a < 3; ++a) {
+      d.n[a] = int(b.n[a]);
+      d.lo[a] = b.lo[a];
+      d.d[a] = b.d[a];
+      d.idx[a] = 1.0 / b.d[a];
+      d.idx2[a] = d.idx[a] * d.idx[a];
+    }
+    d.R = b.R;
+    d.order = b.order;
+    d.level = b.level;
+    d.dulap = dulap_of(b.order, d.idx2);
+    d.ipn = ipn;
+    for (int f = 0; f < NFIELDS; ++f) d.f[f] = H.slab + size_t(f) * size_t(total) + size_t(off[ib]);
+    H.hbox.push_back(d);
+  }
+  for (int l = 0; l < g.levels; ++l) {
+    const DevBox &B = H.hbox[size_t(g.level_first[size_t(l)])];
+    const int nc = B.order / 2 + 1;
+    std::vector<double> cnt(size_t(nc), 0.0);
+    int rows = 0, maxrow = 0;
+    for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
+      const DevBox &X = H.hbox[size_t(g.level_first[size_t(l)] + b)];
+      rows += interior_rows(X);
+      maxrow = std::max(maxrow, X.n[0] - 2 * X.R);
+      for (int k = X.R; k < X.n[2] - X.R; ++k)
+        for (int j = X.R; j < X.n[1] - X.R; ++j)
+          for (int i = X.R; i < X.n[0] - X.R; ++i) cnt[size_t((i + j + k) % nc)] += 1.0;
+    }
+    // the smoother maps threads to points with 24-bit exact float division
+    if ((long long)rows * ((maxrow + nc - 1) / nc) >= (1LL << 24))
+      return err = "a level exceeds 2^24 points per colour", OLL_EINVAL;
+    H.colour_pts.push_back(cnt);
+  }
+  if ((rc = upload(&H.dbox, H.hbox, H.bytes, err))) return rc;
+
+  // transfer tables
+  Pools P;
+  const int L = g.levels;
+  H.ru.assign(size_t(L), {});
+  H.rw.assign(size_t(L), {});
+  H.pv.assign(size_t(L), {});
+  H.gf.assign(size_t(L), {});
+  for (int k = 1; k < L; ++k) {
+    for (int t = 0; t < g.level_count[size_t(k - 1)]; ++t) {
+      const int tb = g.level_first[size_t(k - 1)] + t;
+      H.ru[size_t(k)].push_back(P.add(g, tb, k, 1, 3));
+      H.rw[size_t(k)].push_back(P.add(g, tb, k, 1, 2));
+    }
+    for (int t = 0; t < g.level_count[size_t(k)]; ++t) {
+      const int tb = g.level_first[size_t(k)] + t;
+      H.pv[size_t(k)].push_back(P.add(g, tb, k - 1, 0, 1));
+      H.gf[size_t(k)].push_back(P.add(g, tb, k - 1, 1, p.order + 1));
+    }
+  }
+  if ((rc = upload(&H.ipool, P.ip, H.bytes, err))) return rc;
+  if ((rc = upload(&H.bpool, P.bp, H.bytes, err))) return rc;
+  if ((rc = upload(&H.dpool, P.dp, H.bytes, err))) return rc;
+  std::vector<DevXfer> xf;
+  for (const Pools::X &x : P.xs) {
+    DevXfer d{};
+    d.src = H.dbox + x.src;
+    for (int a = 0; a < 3; ++a) {
+      d.ax[a].start = H.ipool + x.start[a];
+      d.ax[a].coinc = H.ipool + x.coinc[a];
+      d.ax[a].flags = H.bpool + x.flags[a];
+      d.ax[a].w = H.dpool + x.w[a];
+      d.ax[a].k = x.k[a];
+      d.ax[a].n = x.n[a];
+    }
+    xf.push_back(d);
+  }
+  if ((rc = upload(&H.dxfer, xf, H.bytes, err))) return rc;
+
+  H.nslots = 2 * L + 2;
+  ENG_CHECK(hipMalloc(&H.norms, size_t(H.nslots) * kNormSubSlots * sizeof(unsigned long long)));
+  ENG_CHECK(hipMalloc(&H.iters, sizeof(int)));
+  ENG_CHECK(hipMemsetAsync(H.iters, 0, sizeof(int), s));
+  H.bytes += size_t(H.nslots) * kNormSubSlots * sizeof(unsigned long long) + sizeof(int);
+  launch_zero_u64(s, H.norms, (long long)H.nslots * kNormSubSlots);
+  H.eps_coarse = 2 * DBL_EPSILON;
+  H.eps_solv = p.tol < H.eps_coarse ? H.eps_coarse : p.tol;
+  // over-relaxation of the coarsest-level loop, tuned on the 13^3-21^3
+  // coarsest boxes the hierarchy produces (17^3: 1.9)
+  H.coarse_omega = 2.0 - 1.6 / double(H.hbox[size_t(g.level_first[0])].n[0] - 1);
+
+  PunctureArgs &pa = H.pa;
+  pa.n = p.num_bh;
+  pa.eq = p.eq;
+  for (int i = 0; i < p.num_bh; ++i) {
+    const oll_puncture &q = p.bh[i];
+    pa.p[i] = Puncture{q.m, q.x, q.y, q.z, q.px, q.py, q.pz, q.sx, q.sy, q.sz};
+  }
+  pa.ps_case = p.ps_case;
+  pa.ps_r0 = p.ps_r0;
+  pa.ps_isigma = 1 / p.ps_sigma;
+  pa.ps_phi0 = (p.ps_a2 * p.ps_a2 / (1 + p.ps_a2 * p.ps_a2)) * p.ps_phi0;
+  pa.ps_d1 = M_PI * pa.ps_phi0 * pa.ps_phi0 * pa.ps_isigma * pa.ps_isigma;
+  pa.ps_e1 = 1 / (16 * p.ps_a2);
+  pa.ps_e2 = 4 * std::sqrt(M_PI / 3);
+  pa.ps_de1 = 4 * M_PI * pa.ps_phi0 * pa.ps_phi0 * pa.ps_isigma * pa.ps_isigma;
+  return hier_reset(H, s, err);
+}
+
+int hier_reset(Hier &H, hipStream_t s, std::string &err) {
+  for (const DevBox &b : H.hbox) launch_coefficients_box(s, b, H.pa, H.prob.robin_a);
+  ENG_CHECK(hipGetLastError());
+  ENG_CHECK(hipStreamSynchronize(s));
+  H.cycle = 0;
+  H.res = H.res_prev = 1e12;
+  H.done = false;
+  H.status = 0;
+  H.hist.clear();
+  H.last_slots.clear();
+  H.coarse_sweeps = 0;
+  return OLL_OK;
+}
+
+void hier_release(Hier &H) {
+  (void)hipFree(H.slab);
+  (void)hipFree(H.dbox);
+  (void)hipFree(H.ipool);
+  (void)hipFree(H.bpool);
+  (void)hipFree(H.dpool);
+  (void)hipFree(H.dxfer);
+  (void)hipFree(H.norms);
+  (void)hipFree(H.iters);
+  H.slab = nullptr;
+  H.dbox = nullptr;
+  H.ipool = nullptr;
+  H.bpool = nullptr;
+  H.dpool = nullptr;
+  H.dxfer = nullptr;
+  H.norms = nullptr;
+  H.iters = nullptr;
+}
+
+// ---------------------------------------------------------------------
+// schedules
+// ---------------------------------------------------------------------
+
+void Schedule::clear() {
+  launches.clear();
+  blob.clear();
+  uploaded = false;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+}
+
+void release_schedule(Schedule &S) {
+  S.clear();
+  (void)hipFree(S.dblob);
+  S.dblob = nullptr;
+  S.dcap = 0;
+}
+
+namespace {
+
+struct Builder {
+  Schedule &S;
+
+  // row items of one level of one member appended to `items`
+  static void rows(const Hier &H, int level, unsigned long long *norm, std::vector<RowItem> &items,
+                   int &total, int &maxrow) {
+    const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
+    for (int b = f; b < f + n; ++b) {
+      const DevBox &B = H.hbox[size_t(b)];
+      items.push_back(RowItem{H.dbox + b, norm, total, interior_rows(B)});
+      total += interior_rows(B);
+      maxrow = std::max(maxrow, B.n[0] - 2 * B.R);
+    }
+  }
+
+  void zero(const std::vector<Hier *> &M) {
+    std::vector<unsigned long long *> ptrs;
+    int count = 0;
+    for (Hier *H : M) {
+      ptrs.push_back(H->norms);
+      count = std::max(count, H->nslots * kNormSubSlots);
+    }
+    Launch L{Launch::ZERO};
+    L.off = S.add(ptrs);
+    L.n = int(ptrs.size());
+    L.total = count;
+    S.launches.push_back(L);
+  }
+
+  // one smoothing phase (nu sweeps) of level lv[h] of each member
+  void smooth(const std::vector<Hier *> &M, const std::vector<int> &lv, int nu, bool post) {
+    std::map<std::pair<int, int>, std::vector<int>> groups;  // (order, eq) -> members
+    for (size_t h = 0; h < M.size(); ++h)
+      if (lv[h] >= 0) groups[{M[h]->level_order(lv[h]), M[h]->prob.eq}].push_back(int(h));
+    for (auto &g : groups) {
+      const int order = g.first.first, nc = order / 2 + 1;
+      // members are packed into launches of < 2^24 threads (the kernel's
+      // exact float division of the thread index)
+      struct Chunk {
+        std::vector<RowItem> items;
+        int total = 0, maxrow = 0;
+        double bytes[5] = {0, 0, 0, 0, 0};
+      };
+      std::vector<Chunk> chunks(1);
+      for (int h : g.second) {
+        const Hier &H = *M[size_t(h)];
+        const int l = lv[size_t(h)];
+        std::vector<RowItem> mine;
+        int mtotal = 0, mmax = 0;
+        rows(H, l, H.slot(post ? Hier::slot_post(l) : Hier::slot_pre(l)), mine, mtotal, mmax);
+        Chunk *c = &chunks.back();
+        const int mx = std::max(c->maxrow, mmax);
+        if (!c->items.empty() &&
+            (long long)(c->total + mtotal) * ((mx + nc - 1) / nc) >= (1LL << 24)) {
+          chunks.emplace_back();
+          c = &chunks.back();
+        }
+        for (RowItem r : mine) {
+          r.row0 += c->total;
+          c->items.push_back(r);
+        }
+        c->total += mtotal;
+        c->maxrow = std::max(c->maxrow, mmax);
+        const double per = H.prob.eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : 40.0;
+        for (int q = 0; q < nc; ++q) c->bytes[q] += per * H.colour_pts[size_t(l)][size_t(q)];
+      }
+      std::vector<size_t> offs;
+      for (const Chunk &c : chunks) offs.push_back(S.add(c.items));
+      for (int s = 0; s < nu; ++s)
+        for (int q = 0; q < nc; ++q)
+          for (size_t ci = 0; ci < chunks.size(); ++ci) {
+            Launch L{Launch::SMOOTH};
+            L.order = order;
+            L.eq = g.first.second;
+            L.colour = q;
+            L.off = offs[ci];
+            L.n = int(chunks[ci].items.size());
+            L.total = chunks[ci].total;
+            L.max_row = chunks[ci].maxrow;
+            L.bytes = chunks[ci].bytes[q];
+            S.launches.push_back(L);
+          }
+    }
+  }
+
+  void residual(const std::vector<Hier *> &M, const std::vector<int> &lv, int write_defect,
+                bool with_norm) {
+    std::map<std::pair<int, int>, std::vector<int>> groups;
+    for (size_t h = 0; h < M.size(); ++h)
+      if (lv[h] >= 0) groups[{M[h]->level_order(lv[h]), M[h]->prob.eq}].push_back(int(h));
+    for (auto &g : groups) {
+      std::vector<RowItem> items;
+      int total = 0, maxrow = 0;
+      for (int h : g.second) {
+        const Hier &H = *M[size_t(h)];
+        rows(H, lv[size_t(h)], with_norm ? H.slot(H.slot_scratch()) : nullptr, items, total, maxrow);
+      }
+      Launch L{Launch::RESIDUAL};
+      L.order = g.first.first;
+      L.eq = g.first.second;
+      L.mode = write_defect;
+      L.off = S.add(items);
+      L.n = int(items.size());
+      L.total = total;
+      L.max_row = maxrow;
+      S.launches.push_back(L);
+    }
+  }
+
+  void robin(const std::vector<Hier *> &M, const std::vector<int> &lv, int mode) {
+    std::vector<RobinItem> items;
+    int total = 0;
+    for (size_t h = 0; h < M.size(); ++h) {
+      if (lv[h] < 0) continue;
+      const Hier &H = *M[h];
+      const int f = H.geo.level_first[size_t(lv[h])], n = H.geo.level_count[size_t(lv[h])];
+      for (int b = f; b < f + n; ++b) {
+        const int nc = face_columns(H.hbox[size_t(b)]);
+        items.push_back(RobinItem{H.dbox + b, total, nc});
+        total += nc;
+      }
+    }
+    if (items.empty()) return;
+    Launch L{Launch::ROBIN};
+    L.mode = mode;
+    L.off = S.add(items);
+    L.n = int(items.size());
+    L.total = total;
+    S.launches.push_back(L);
+  }
+
+  // point items of the target boxes of level tl[h] with transfer refs
+  // X[k][t] (k = the member's fine level of the pair)
+  static void points(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
+                     bool shell, int mg, std::vector<PtItem> &items, long long &total) {
+    const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
+    for (int t = 0; t < n; ++t) {
+      const DevBox &B = H.hbox[size_t(f + t)];
+      const Hier::XRef r = X[size_t(k)][size_t(t)];
+      const long long np = shell ? shell_pts(B) : npts(B);
+      items.push_back(PtItem{H.dbox + f + t, H.dxfer + r.off, r.cnt, mg, total, np});
+      total += np;
+    }
+  }
+
+  void transfer(Launch::Kind kind, const std::vector<Hier *> &M, const std::vector<int> &lv) {
+    // lv[h] = the member's fine level k of the pair (k-1, k)
+    if (kind == Launch::RESTRICT_DEFECT) {
+      std::map<std::pair<int, int>, std::vector<int>> groups;  // coarse (order, eq)
+      for (size_t h = 0; h < M.size(); ++h)
+        if (lv[h] >= 1) groups[{M[h]->level_order(lv[h] - 1), M[h]->prob.eq}].push_back(int(h));
+      for (auto &g : groups) {
+        std::vector<PtItem> items;
+        long long total = 0;
+        for (int h : g.second) {
+          const Hier &H = *M[size_t(h)];
+          const int k = lv[size_t(h)];
+          points(H, k - 1, k, H.rw, false, k <= H.geo.mgrid, items, total);
+        }
+        Launch L{kind};
+        L.order = g.first.first;
+        L.eq = g.first.second;
+        L.off = S.add(items);
+        L.n = int(items.size());
+        L.total = total;
+        S.launches.push_back(L);
+      }
+      return;
+    }
+    std::vector<PtItem> items;
+    long long total = 0;
+    for (size_t h = 0; h < M.size(); ++h) {
+      const int k = lv[h];
+      if (k < 1) continue;
+      const Hier &H = *M[h];
+      const int mg = k <= H.geo.mgrid;
+      if (kind == Launch::RESTRICT_U) points(H, k - 1, k, H.ru, false, mg, items, total);
+      if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, items, total);
+      if (kind == Launch::GHOST && !mg) points(H, k, k, H.gf, true, 0, items, total);
+    }
+    if (items.empty()) return;
+    Launch L{kind};
+    L.off = S.add(items);
+    L.n = int(items.size());
+    L.total = total;
+    S.launches.push_back(L);
+  }
+
+  void coarse(const std::vector<Hier *> &M) {
+    std::map<int, std::vector<CoarseItem>> groups;  // eq -> items
+    std::map<int, std::pair<int, int>> sizes;
+    for (Hier *H : M) {
+      const int b0 = H->geo.level_first[0];
+      const DevBox &B = H->hbox[size_t(b0)];
+      groups[H->prob.eq].push_back(CoarseItem{H->dbox + b0, H->slot(Hier::slot_pre(0)), H->iters,
+                                              H->coarse_omega});
+      auto &sz = sizes[H->prob.eq];
+      sz.first = std::max<int>(sz.first, int(npts(B)));
+      sz.second = std::max<int>(sz.second, (B.n[0] - 2) * (B.n[1] - 2) * (B.n[2] - 2));
+    }
+    for (auto &g : groups) {
+      Launch L{Launch::COARSE};
+      L.eq = g.first;
+      L.off = S.add(g.second);
+      L.n = int(g.second.size());
+      L.max_pts = sizes[g.first].first;
+      L.max_int = sizes[g.first].second;
+      S.launches.push_back(L);
+    }
+  }
+};
+
+}  // namespace
+
+int build_vcycle_from(const std::vector<Hier *> &M, const std::vector<int> &start, Schedule &S,
+                      std::string &err) {
+  S.clear();
+  if (M.empty()) return OLL_OK;
+  for (Hier *H : M)
+    if (H->prob.nu1 != M[0]->prob.nu1 || H->prob.nu2 != M[0]->prob.nu2)
+      return err = "members of one schedule must share nu1 and nu2", OLL_EINVAL;
+  for (Hier *H : M) {
+    const DevBox &B = H->hbox[size_t(H->geo.level_first[0])];
+    const size_t lds = 2 * sizeof(double) * size_t(npts(B));
+    if (lds + 2048 > size_t(kCoarseLdsLimit) ||
+        (B.n[0] - 2) * (B.n[1] - 2) * (B.n[2] - 2) > 1024 * kCoarseOwn)
+      return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;
+  }
+  Builder b{S};
+  b.zero(M);
+  int dmax = 0;
+  for (size_t h = 0; h < M.size(); ++h) dmax = std::max(dmax, start[h]);
+  const int nu1 = M[0]->prob.nu1, nu2 = M[0]->prob.nu2;
+  std::vector<int> lv(M.size()), mglv(M.size());
+  for (int d = 0; d < dmax; ++d) {
+    // level of each member at this depth (-1: idle)
+    for (size_t h = 0; h < M.size(); ++h) {
+      lv[h] = (start[h] - d >= 1) ? start[h] - d : -1;
+      mglv[h] = (lv[h] >= 1 && lv[h] <= M[h]->geo.mgrid) ? lv[h] : -1;
+    }
+    b.robin(M, mglv, ROBIN_UPDATE);
+    b.smooth(M, lv, nu1, false);
+    b.residual(M, lv, 1, false);
+    b.robin(M, mglv, ROBIN_DEFECT_W);
+    b.transfer(Launch::RESTRICT_U, M, lv);
+    b.transfer(Launch::RESTRICT_DEFECT, M, lv);
+  }
+  b.coarse(M);
+  for (int d = dmax - 1; d >= 0; --d) {
+    for (size_t h = 0; h < M.size(); ++h) {
+      lv[h] = (start[h] - d >= 1) ? start[h] - d : -1;
+      mglv[h] = (lv[h] >= 1 && lv[h] <= M[h]->geo.mgrid) ? lv[h] : -1;
+    }
+    b.transfer(Launch::PROLONG, M, lv);
+    b.transfer(Launch::GHOST, M, lv);
+    b.robin(M, mglv, ROBIN_UPDATE);
+    b.smooth(M, lv, nu2, true);
+  }
+  return OLL_OK;
+}
+
+int build_vcycle(const std::vector<Hier *> &M, Schedule &S, std::string &err) {
+  std::vector<int> start;
+  for (Hier *H : M) start.push_back(H->top());
+  return build_vcycle_from(M, start, S, err);
+}
+
+int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err) {
+  (void)err;
+  S.clear();
+  std::vector<Hier *> M{&H};
+  std::vector<int> lv{level};
+  Builder b{S};
+  if (stage == STAGE_SMOOTH || stage == STAGE_RESIDUAL) b.zero(M);
+  if (stage == STAGE_SMOOTH) {
+    // the scratch slot collects the norm of these sweeps
+    const size_t before = S.launches.size();
+    b.smooth(M, lv, nu, false);
+    std::vector<RowItem> items;
+    int total = 0, maxrow = 0;
+    Builder::rows(H, level, H.slot(H.slot_scratch()), items, total, maxrow);
+    const size_t off = S.add(items);
+    for (size_t i = before; i < S.launches.size(); ++i) S.launches[i].off = off;
+  } else if (stage == STAGE_RESIDUAL) {
+    b.residual(M, lv, 0, true);
+  } else if (stage == STAGE_ROBIN) {
+    b.robin(M, lv, ROBIN_UPDATE);
+  } else {
+    b.robin(M, lv, ROBIN_OPERATOR_R);
+  }
+  return OLL_OK;
+}
+
+int upload_schedule(Schedule &S, hipStream_t s, std::string &err) {
+  if (S.blob.size() > S.dcap) {
+    (void)hipFree(S.dblob);
+    S.dblob = nullptr;
+    const size_t cap = std::max<size_t>(S.blob.size() * 2, 4096);
+    ENG_CHECK(hipMalloc(&S.dblob, cap));
+    S.dcap = cap;
+  }
+  if (!S.blob.empty())
+    ENG_CHECK(hipMemcpyAsync(S.dblob, S.blob.data(), S.blob.size(), hipMemcpyHostToDevice, s));
+  S.uploaded = true;
+  return OLL_OK;
+}
+
+namespace {
+
+hipEvent_t timing_event(Timing &t) {
+  if (t.used == t.ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    t.ev.push_back(e);
+  }
+  return t.ev[t.used++];
+}
+
+int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
+  for (const Launch &L : S.launches) {
+    const void *items = S.dblob + L.off;
+    hipEvent_t e1 = nullptr;
+    if (t && t->on && L.kind == Launch::SMOOTH) {
+      hipEvent_t e0 = timing_event(*t);
+      e1 = timing_event(*t);
+      if (e0) (void)hipEventRecord(e0, s);
+    }
+    switch (L.kind) {
+      case Launch::ZERO:
+        launch_zero_lists(s, reinterpret_cast<unsigned long long *const *>(items), L.n, int(L.total));
+        break;
+      case Launch::ROBIN:
+        launch_robin(s, reinterpret_cast<const RobinItem *>(items), L.n, int(L.total), L.mode);
+        break;
+      case Launch::SMOOTH:
+        launch_smooth(s, reinterpret_cast<const RowItem *>(items), L.n, int(L.total), L.max_row,
+                      L.order, L.eq, L.colour);
+        break;
+      case Launch::RESIDUAL:
+        launch_residual(s, reinterpret_cast<const RowItem *>(items), L.n, int(L.total), L.max_row,
+                        L.order, L.eq, L.mode);
+        break;
+      case Launch::RESTRICT_U:
+        launch_restrict_u(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);
+        break;
+      case Launch::RESTRICT_DEFECT:
+        launch_restrict_defect(s, reinterpret_cast<const PtItem *>(items), L.n, L.total, L.order, L.eq);
+        break;
+      case Launch::COARSE:
+        if (!launch_coarse_solve(s, reinterpret_cast<const CoarseItem *>(items), L.n, L.max_pts,
+                                 L.max_int, L.eq, 2 * DBL_EPSILON, 100000))
+          return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;
+        break;
+      case Launch::PROLONG:
+        launch_prolong(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);
+        break;
+      case Launch::GHOST:
+        launch_ghost_fill(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);
+        break;
+    }
+    if (e1) {
+      (void)hipEventRecord(e1, s);
+      t->bytes += L.bytes;
+    }
+  }
+  ENG_CHECK(hipGetLastError());
+  return OLL_OK;
+}
+
+}  // namespace
+
+int run_schedule(Schedule &S, hipStream_t s, Timing *t, bool graphs, std::string &err) {
+  int rc;
+  if (!S.uploaded && (rc = upload_schedule(S, s, err))) return rc;
+  const bool timed = t && t->on;
+  if (!graphs || timed) return issue(S, s, t, err);
+  if (!S.exec) {
+    ENG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = issue(S, s, nullptr, err);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+      return OLL_EDEVICE;
+    }
+    S.graph = g;
+    ENG_CHECK(hipGraphInstantiate(&S.exec, S.graph, nullptr, nullptr, 0));
+  }
+  ENG_CHECK(hipGraphLaunch(S.exec, s));
+  return OLL_OK;
+}
+
+int read_norms(Hier &H, hipStream_t s, std::string &err) {
+  H.raw.resize(size_t(H.nslots) * kNormSubSlots);
+  ENG_CHECK(hipMemcpyAsync(H.raw.data(), H.norms, H.raw.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+  ENG_CHECK(hipMemcpyAsync(&H.raw_iters, H.iters, sizeof(int), hipMemcpyDeviceToHost, s));
+  return OLL_OK;
+}
+
+void decode_norms(Hier &H) {
+  H.last_slots.assign(size_t(H.nslots), 0.0);
+  for (int sl = 0; sl < H.nslots; ++sl) {
+    unsigned long long m = 0;
+    for (int q = 0; q < kNormSubSlots; ++q) m = std::max(m, H.raw[size_t(sl) * kNormSubSlots + q]);
+    double d;
+    std::memcpy(&d, &m, sizeof d);
+    H.last_slots[size_t(sl)] = d;
+  }
+}
+
+int finish_cycle(Hier &H, hipStream_t s, std::string &err) {
+  int rc = read_norms(H, s, err);
+  if (rc) return rc;
+  ENG_CHECK(hipStreamSynchronize(s));
+  return apply_cycle(H, err);
+}
+
+int apply_cycle(Hier &H, std::string &err) {
+  decode_norms(H);
+  H.coarse_sweeps += H.raw_iters;
+  const int top = H.top();
+  for (int k = top; k >= 0; --k) {
+    H.hist.push_back(H.cycle);
+    H.hist.push_back(H.last_slots[size_t(Hier::slot_pre(k))]);
+  }
+  H.res_prev = H.res;
+  H.res = H.last_slots[size_t(Hier::slot_post(top))];
+  ++H.cycle;
+  for (int k = 0; k <= top; ++k)
+    if (std::isinf(H.last_slots[size_t(Hier::slot_pre(k))]) ||
+        std::isinf(H.last_slots[size_t(Hier::slot_post(k))])) {
+      H.status = OLL_ENAN;
+      H.done = true;
+      err = "NAN check error in the smoother";
+      return OLL_ENAN;
+    }
+  if (!(H.res > H.eps_solv && H.res < H.res_prev)) H.done = true;
+  if (H.prob.max_cycles > 0 && H.cycle >= H.prob.max_cycles) H.done = true;
+  return OLL_OK;
+}
+
+int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::string &err) {
+  const oll_problem &p = H.prob;
+  const int top = H.top();
+  const int nb = int(H.hbox.size());
+  const int nt = 4 * 25 + 1;
+  // u at each puncture from its finest box (Lagrange order 4);
+  // multigrid.cc:97-116 falls back to the level's first box
+  std::vector<int> pbox(size_t(p.num_bh));
+  std::vector<double> pxyz(size_t(3 * p.num_bh));
+  for (int i = 0; i < p.num_bh; ++i) {
+    pbox[size_t(i)] = H.geo.level_first[size_t(top)] + (i < H.geo.level_count[size_t(top)] ? i : 0);
+    pxyz[size_t(3 * i)] = p.bh[i].x;
+    pxyz[size_t(3 * i + 1)] = p.bh[i].y;
+    pxyz[size_t(3 * i + 2)] = p.bh[i].z;
+  }
+  std::vector<double> ctr(size_t(3 * nb));
+  for (int b = 0; b < nb; ++b)
+    for (int a = 0; a < 3; ++a) ctr[size_t(3 * b + a)] = H.geo.boxes[size_t(b)].c[a];
+  int *d_pbox = nullptr;
+  double *d_pxyz = nullptr, *d_up = nullptr, *d_ctr = nullptr, *d_sph = nullptr;
+  const size_t nsph = surface ? size_t(nb) * nt * nt : 0;
+  ENG_CHECK(hipMalloc(&d_pbox, pbox.size() * sizeof(int)));
+  ENG_CHECK(hipMalloc(&d_pxyz, pxyz.size() * sizeof(double)));
+  ENG_CHECK(hipMalloc(&d_up, pbox.size() * sizeof(double)));
+  ENG_CHECK(hipMemcpyAsync(d_pbox, pbox.data(), pbox.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  ENG_CHECK(hipMemcpyAsync(d_pxyz, pxyz.data(), pxyz.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_interp_points(s, H.dbox, F_U, d_pbox, d_pxyz, p.num_bh, 4, d_up);
+  std::vector<double> up(pbox.size()), sph(nsph);
+  if (surface) {
+    ENG_CHECK(hipMalloc(&d_ctr, ctr.size() * sizeof(double)));
+    ENG_CHECK(hipMalloc(&d_sph, nsph * sizeof(double)));
+    ENG_CHECK(hipMemcpyAsync(d_ctr, ctr.data(), ctr.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_adm_sphere(s, H.dbox, nb, d_ctr, d_sph, nt);
+    ENG_CHECK(hipMemcpyAsync(sph.data(), d_sph, nsph * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  ENG_CHECK(hipMemcpyAsync(up.data(), d_up, up.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  ENG_CHECK(hipStreamSynchronize(s));
+  (void)hipFree(d_pbox);
+  (void)hipFree(d_pxyz);
+  (void)hipFree(d_up);
+  (void)hipFree(d_ctr);
+  (void)hipFree(d_sph);
+  // elliptic.cc:2448-2469
+  for (int i = 0; i < p.num_bh && puncture; ++i) {
+    double bhmj = 0;
+    for (int j = 0; j < p.num_bh; ++j)
+      if (j != i) {
+        const double dx = p.bh[i].x - p.bh[j].x, dy = p.bh[i].y - p.bh[j].y, dz = p.bh[i].z - p.bh[j].z;
+        bhmj += 0.5 * p.bh[j].m / std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    puncture[i] = p.bh[i].m * (1 + bhmj + up[size_t(i)]);
+  }
+  // Boole's rule over theta and phi, elliptic.cc:2560-2604
+  if (surface) {
+    const double dtheta = M_PI / (nt - 1), dphi = 2 * M_PI / (nt - 1);
+    for (int b = 0; b < nb; ++b) {
+      const double *Sd = &sph[size_t(b) * nt * nt];
+      const Box &B = H.geo.boxes[size_t(b)];
+      const double r0 = 0.375 * std::min(B.len[0], std::min(B.len[1], B.len[2]));
+      double adm = 0;
+      for (int i = 0; i < nt - 3; i += 4) {
+        double I[5] = {0, 0, 0, 0, 0};
+        for (int j = 0; j < nt - 3; j += 4)
+          for (int q = 0; q < 5; ++q) {
+            const double *row = Sd + size_t(i + q) * nt;
+            I[q] += 7 * row[j] + 32 * row[j + 1] + 12 * row[j + 2] + 32 * row[j + 3] + 7 * row[j + 4];
+          }
+        for (int q = 0; q < 5; ++q) I[q] *= 2 * dphi / 45;
+        adm += 7 * I[0] + 32 * I[1] + 12 * I[2] + 32 * I[3] + 7 * I[4];
+      }
+      adm *= -r0 * r0 * dtheta / (45 * M_PI);
+      surface[b] = adm;
+    }
+  }
+  return OLL_OK;
+}
+
+}  // namespace oll

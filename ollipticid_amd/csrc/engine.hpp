@@ -1,0 +1,137 @@
+// Host engine: one Hier per request (its box hierarchy resident in HBM) and
+// Schedules, the launch sequences of one V-cycle for a set of hierarchies.
+//
+// A schedule aligns its members by depth below their top level: at depth d
+// every member works on its own level top - d, so one launch per (stage,
+// stencil order, equation, colour) covers the boxes of all members.  The
+// coarsest-level solves of all members run together, one block each.  The
+// control flow of one member is the reference's FASN recursion
+// (elliptic.cc:1469-1968); the outer loop and stopping rule are Vcycle's
+// (elliptic.cc:1251-1342).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/olliptic_amd.h"
+#include "geometry.hpp"
+#include "kernels.hpp"
+
+namespace oll {
+
+struct Hier {
+  oll_problem prob{};
+  Hierarchy geo;
+  std::vector<DevBox> hbox;      // host copies, device field pointers
+  DevBox *dbox = nullptr;        // device copies
+  double *slab = nullptr;
+  int *ipool = nullptr;
+  unsigned char *bpool = nullptr;
+  double *dpool = nullptr;
+  DevXfer *dxfer = nullptr;
+  size_t bytes = 0;              // HBM held
+  struct XRef {
+    int off = 0, cnt = 0;        // into dxfer
+  };
+  // [fine level k][target box within its level]
+  std::vector<std::vector<XRef>> ru, rw, pv, gf;
+  unsigned long long *norms = nullptr;  // nslots * kNormSubSlots
+  int nslots = 0;
+  int *iters = nullptr;
+  PunctureArgs pa{};
+  double eps_solv = 0, eps_coarse = 0, coarse_omega = 1;
+  std::vector<std::vector<double>> colour_pts;  // [level][colour]
+  // V-cycle state
+  int cycle = 0;
+  double res = 1e12, res_prev = 1e12;
+  bool done = false;
+  int status = 0;                // 0 ok, <0 error code
+  std::vector<double> hist;      // (cycle, pre-smoothing norm) pairs
+  std::vector<double> last_slots;
+  long long coarse_sweeps = 0;
+  std::vector<unsigned long long> raw;  // staging of the norm slots
+  int raw_iters = 0;
+
+  int top() const { return geo.levels - 1; }
+  int level_order(int l) const { return hbox[size_t(geo.level_first[size_t(l)])].order; }
+  unsigned long long *slot(int s) const { return norms + size_t(s) * kNormSubSlots; }
+  static int slot_pre(int k) { return 2 * k; }
+  static int slot_post(int k) { return 2 * k + 1; }
+  int slot_scratch() const { return nslots - 1; }
+};
+
+// builds geometry, allocates, uploads transfer tables, evaluates the
+// coefficients; returns an OLL_* code and sets err
+int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err);
+void hier_release(Hier &H);
+int hier_reset(Hier &H, hipStream_t s, std::string &err);
+// host-only footprint of a problem's hierarchy (bytes of HBM)
+int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err);
+
+struct Launch {
+  enum Kind { ZERO, ROBIN, SMOOTH, RESIDUAL, RESTRICT_U, RESTRICT_DEFECT, COARSE, PROLONG, GHOST };
+  Kind kind;
+  int order = 2, eq = 0, colour = 0, mode = 0;
+  size_t off = 0;     // items in the schedule blob
+  int n = 0;          // items
+  long long total = 0;
+  int max_row = 0;
+  int max_pts = 0, max_int = 0;
+  double bytes = 0;   // algorithmic bytes (smoother launches)
+};
+
+struct Schedule {
+  std::vector<Launch> launches;
+  std::vector<unsigned char> blob;
+  unsigned char *dblob = nullptr;
+  size_t dcap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool uploaded = false;
+
+  template <typename T>
+  size_t add(const std::vector<T> &items) {
+    size_t off = (blob.size() + 255) & ~size_t(255);
+    blob.resize(off + items.size() * sizeof(T));
+    if (!items.empty()) std::memcpy(blob.data() + off, items.data(), items.size() * sizeof(T));
+    return off;
+  }
+  void clear();
+};
+
+struct Timing {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  double bytes = 0;
+};
+
+// the full V-cycle of every member; all members must share nu1, nu2
+int build_vcycle(const std::vector<Hier *> &members, Schedule &S, std::string &err);
+// single stages of one hierarchy on one level (tests and tools)
+int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err);
+enum Stage { STAGE_SMOOTH = 0, STAGE_RESIDUAL, STAGE_ROBIN, STAGE_BOUNDARY_OP };
+
+int upload_schedule(Schedule &S, hipStream_t s, std::string &err);
+// runs the schedule; with graphs = true it captures once and replays
+int run_schedule(Schedule &S, hipStream_t s, Timing *t, bool graphs, std::string &err);
+void release_schedule(Schedule &S);
+
+// enqueue the copy of a member's norm slots (read_norms), then after a
+// stream synchronization decode them into H.last_slots (decode_norms);
+// apply_cycle also records the V-cycle history and applies the stopping
+// rule; finish_cycle does all three for one member
+int read_norms(Hier &H, hipStream_t s, std::string &err);
+void decode_norms(Hier &H);
+int apply_cycle(Hier &H, std::string &err);
+int finish_cycle(Hier &H, hipStream_t s, std::string &err);
+// V-cycle schedule whose member h starts at level start[h] (FASN(start))
+int build_vcycle_from(const std::vector<Hier *> &members, const std::vector<int> &start,
+                      Schedule &S, std::string &err);
+// ADM masses (elliptic.cc:2447-2611); surface may be null
+int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::string &err);
+
+}  // namespace oll

@@ -1,9 +1,10 @@
 // CDNA4 (gfx950) kernels of the FAS puncture solver.
 //
-// All stencils here are axis stars of radius R = order/2, so the points are
-// split into R+1 colours (i+j+k) mod (R+1): no point reads another point of
-// its own colour, and one launch updates one colour of every box of a level
-// in parallel.  Arithmetic is float64 throughout.
+// All stencils are axis stars of radius R = order/2, so the points split
+// into R+1 colours (i+j+k) mod (R+1) with no same-colour neighbours: one
+// launch updates one colour of every listed box.  A launch works on a
+// device-resident item list whose boxes may belong to many hierarchies
+// (requests); arithmetic is float64 throughout.
 //
 // Reference counterparts (WuShichao/OllipticID):
 //   k_smooth            elliptic.cc:1126-1242 Smooth, elliptic.h:405-445 ops
@@ -104,35 +105,28 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return r;
 }
 
-// locate the box of a flat interior row index within a level
-__device__ __forceinline__ int find_box_row(const LevelView &lv, int row) {
-  int b = 0;
-  while (b + 1 < lv.nbox && row >= lv.row0[b + 1]) ++b;
-  return b;
-}
-
-__device__ __forceinline__ int find_box_pt(const LevelView &lv, long long pt) {
-  int b = 0;
-  while (b + 1 < lv.nbox && pt >= lv.pt0[b + 1]) ++b;
-  return b;
-}
-
-// Norm slots: every logical norm is kNormSub consecutive u64 words; blocks
-// spread their atomics over them and the host takes the max.  A block only
-// issues its atomic when it raises the value it reads.
-constexpr int kNormSub = 8;
-
+// Norm slots: a logical norm is kNormSubSlots u64 words; blocks spread
+// their atomics over them (the host takes the max) and only issue an atomic
+// when it raises the value read.  Non-negative doubles order like their
+// bit patterns.
 __device__ __forceinline__ void norm_update(unsigned long long *slot, double v) {
-  unsigned long long *s = slot + (blockIdx.x & (kNormSub - 1));
+  unsigned long long *s = slot + (blockIdx.x & (kNormSubSlots - 1));
   const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
   if (bits > __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(s, bits);
 }
 
-// ------------------------------------------------------------------
-// smoother: one colour of one Newton-Gauss-Seidel sweep on every box of a
-// level.  Thread g owns colour point t = g mod per_row of interior row
-// g / per_row: i = i_first(row) + t (R+1).
-// ------------------------------------------------------------------
+// max over the lanes that share an item, one atomic per item per wave
+__device__ __forceinline__ void item_norm_update(unsigned long long *slot, int item, double v) {
+  const int item0 = __shfl(item, 0, 64);
+  const unsigned long long diff = __ballot(item != item0);
+  if (diff == 0ull) {
+    v = warp_max(v);
+    if ((threadIdx.x & 63) == 0 && v > 0.0 && slot) norm_update(slot, v);
+  } else if (v > 0.0 && slot) {
+    norm_update(slot, v);
+  }
+}
+
 // exact g / d for g < 2^24 via a float reciprocal and one correction step
 __device__ __forceinline__ unsigned fast_div(unsigned g, unsigned d, float inv_d) {
   unsigned q = unsigned(float(g) * inv_d);
@@ -141,27 +135,64 @@ __device__ __forceinline__ unsigned fast_div(unsigned g, unsigned d, float inv_d
   return q;
 }
 
-// wave-level max, then one (read-checked) atomic per wave
-__device__ __forceinline__ void wave_norm_update(unsigned long long *slot, double v) {
-  v = warp_max(v);
-  if ((threadIdx.x & 63) == 0 && v > 0.0) norm_update(slot, v);
+__device__ __forceinline__ int find_row_item(const RowItem *items, int n, int row) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].row0 <= row) lo = mid; else hi = mid - 1;
+  }
+  return lo;
 }
 
+__device__ __forceinline__ int find_pt_item(const PtItem *items, int n, long long pt) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].p0 <= pt) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// coordinate of index i, without contraction so it equals the host value
+__device__ __forceinline__ double coord(const DevBox &B, int a, int i) {
+  return __dadd_rn(B.lo[a], __dmul_rn((double)i, B.d[a]));
+}
+
+__device__ __forceinline__ bool interior(const DevBox &B, int i, int j, int k) {
+  const int R = B.R;
+  return i >= R && i < B.n[0] - R && j >= R && j < B.n[1] - R && k >= R && k < B.n[2] - R;
+}
+
+__device__ __forceinline__ void point_ijk(const DevBox &B, long long q, int &i, int &j, int &k) {
+  const unsigned n0 = unsigned(B.n[0]), n1 = unsigned(B.n[1]);
+  const unsigned qq = unsigned(q);
+  const unsigned plane = n0 * n1;
+  k = int(qq / plane);
+  const unsigned r = qq - unsigned(k) * plane;
+  j = int(r / n0);
+  i = int(r - unsigned(j) * n0);
+}
+
+// ------------------------------------------------------------------
+// smoother: one colour of one Newton-Gauss-Seidel sweep on every listed
+// box.  Thread g owns colour point t = g mod per_row of global interior row
+// g / per_row: i = i_first(row) + t (R+1).
+// ------------------------------------------------------------------
 template <int ORD, int EQ>
-__global__ __launch_bounds__(256) void k_smooth(LevelView lv, const DevBox *__restrict__ boxes,
-                                                 Fields F, int colour, unsigned per_row,
-                                                 float inv_per_row,
-                                                 unsigned long long *__restrict__ norm) {
+__global__ __launch_bounds__(256) void k_smooth(const RowItem *__restrict__ items, int nitems,
+                                                 unsigned total_rows, unsigned per_row,
+                                                 float inv_per_row, int colour) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
   const unsigned g = blockIdx.x * 256u + threadIdx.x;
   const unsigned row = fast_div(g, per_row, inv_per_row);
   const int t = int(g - row * per_row);
   double rmax = 0.0;
-  if (row < unsigned(lv.row0[lv.nbox])) {
-    const int bi = find_box_row(lv, int(row));
-    const DevBox &B = boxes[lv.box[bi]];
-    const unsigned r = row - unsigned(lv.row0[bi]);
+  int it = 0;
+  if (row < total_rows) {
+    it = find_row_item(items, nitems, int(row));
+    const DevBox &B = *items[it].box;
+    const unsigned r = row - unsigned(items[it].row0);
     const unsigned nyi = unsigned(B.n[1] - 2 * R);
     const unsigned kq = fast_div(r, nyi, 1.0f / float(nyi));
     const int k = R + int(kq);
@@ -170,57 +201,56 @@ __global__ __launch_bounds__(256) void k_smooth(LevelView lv, const DevBox *__re
     const int i = first + t * NC;
     if (i < B.n[0] - R) {
       const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-      const long long p = B.off + i + sy * j + sz * k;
-      double *__restrict__ u = F.f[F_U];
+      const long long p = i + sy * j + sz * k;
+      double *__restrict__ u = B.f[F_U];
       const double lu = lap<ORD>(u, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
-      const double cu = F.f[F_C][p] + u[p];
+      const double cu = B.f[F_C][p] + u[p];
       double N, dN;
-      nonlinear<EQ>(lu, B.dulap, F.f[F_B][p], cu, EQ ? F.f[F_D][p] : 0.0,
-                    EQ ? F.f[F_E][p] : 0.0, N, dN);
-      const double res = N - F.f[F_RHO][p];
+      nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], cu, EQ ? B.f[F_D][p] : 0.0,
+                    EQ ? B.f[F_E][p] : 0.0, N, dN);
+      const double res = N - B.f[F_RHO][p];
       const double un = u[p] - res / dN;
       u[p] = un;
       rmax = fabs(res);
       if (!(un == un)) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
     }
   }
-  wave_norm_update(norm, rmax);
+  item_norm_update(row < total_rows ? items[it].norm : nullptr, it, rmax);
 }
 
 // residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u)
 template <int ORD, int EQ>
-__global__ __launch_bounds__(256) void k_residual(LevelView lv, const DevBox *__restrict__ boxes,
-                                                   Fields F, int write_defect,
-                                                   unsigned long long *__restrict__ norm) {
+__global__ __launch_bounds__(256) void k_residual(const RowItem *__restrict__ items, int nitems,
+                                                   unsigned total_rows, int write_defect) {
   constexpr int R = ORD / 2;
-  __shared__ double sh[4];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int row = blockIdx.x * 4 + ty;
+  const unsigned row = blockIdx.x * 4u + ty;
   double rmax = 0.0;
-  if (row < lv.row0[lv.nbox]) {
-    const int bi = find_box_row(lv, row);
-    const DevBox &B = boxes[lv.box[bi]];
-    const int r = row - lv.row0[bi];
-    const int nyi = B.n[1] - 2 * R;
-    const int j = R + r % nyi, k = R + r / nyi;
+  int it = 0;
+  if (row < total_rows) {
+    it = find_row_item(items, nitems, int(row));
+    const DevBox &B = *items[it].box;
+    const unsigned r = row - unsigned(items[it].row0);
+    const unsigned nyi = unsigned(B.n[1] - 2 * R);
+    const int k = R + int(r / nyi);
+    const int j = R + int(r % nyi);
     const int i = R + blockIdx.y * 64 + tx;
     if (i < B.n[0] - R) {
       const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-      const long long p = B.off + i + sy * j + sz * k;
-      const double *__restrict__ u = F.f[F_U];
+      const long long p = i + sy * j + sz * k;
+      const double *__restrict__ u = B.f[F_U];
       const double lu = lap<ORD>(u, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
       double N, dN;
-      nonlinear<EQ>(lu, B.dulap, F.f[F_B][p], F.f[F_C][p] + u[p], EQ ? F.f[F_D][p] : 0.0,
-                    EQ ? F.f[F_E][p] : 0.0, N, dN);
+      nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
+                    EQ ? B.f[F_E][p] : 0.0, N, dN);
       if (write_defect)
-        F.f[F_W][p] = F.f[F_RHO][p] - N;
+        B.f[F_W][p] = B.f[F_RHO][p] - N;
       else
-        F.f[F_R][p] = N;
+        B.f[F_R][p] = N;
       rmax = fabs(N);
     }
   }
-  const double m = block_max(rmax, sh);
-  if (norm && threadIdx.x == 0 && m > 0.0) norm_update(norm, m);
+  item_norm_update(row < total_rows ? items[it].norm : nullptr, it, rmax);
 }
 
 // ------------------------------------------------------------------
@@ -254,13 +284,7 @@ __device__ __forceinline__ double deriv(const double *__restrict__ u, long long 
           3.0 * u[p - 4 * s]) * id / 12.0;
 }
 
-// coordinate of index i, without contraction so it equals the host value
-__device__ __forceinline__ double coord(const DevBox &B, int a, int i) {
-  return __dadd_rn(B.lo[a], __dmul_rn((double)i, B.d[a]));
-}
-
-// Box-local views of the fields a Robin stage touches (point (0,0,0) of the
-// box at index 0).  u may live in global memory or in LDS.
+// box-local views of the fields a Robin stage touches; u may be in LDS
 struct BoxData {
   double *u;
   const double *rho;
@@ -268,19 +292,19 @@ struct BoxData {
   double *w;
 };
 
-__device__ __forceinline__ BoxData box_data(const DevBox &B, Fields F) {
+__device__ __forceinline__ BoxData box_data(const DevBox &B) {
   BoxData D;
-  D.u = F.f[F_U] + B.off;
-  D.rho = F.f[F_RHO] + B.off;
-  D.r = F.f[F_R] + B.off;
-  D.w = F.f[F_W] + B.off;
+  D.u = B.f[F_U];
+  D.rho = B.f[F_RHO];
+  D.r = B.f[F_R];
+  D.w = B.f[F_W];
   return D;
 }
 
 // Robin update / operator at one shell point; `mask` has bit a set for
 // each axis a whose index lies in the ghost shell.
 __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, int j, int k,
-                                            int mask, int mode, double ipn) {
+                                            int mask, int mode) {
   const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
   const long long p = i + sy * j + sz * k;
   double *u = D.u;
@@ -297,10 +321,10 @@ __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, i
     sF += dv / X[a];
     sdF += cf / X[a];
   }
-  const double op = r2 * ipn * sF + u[p];
+  const double op = r2 * B.ipn * sF + u[p];
   if (mode == ROBIN_UPDATE) {
     const double Fv = op - D.rho[p];
-    const double dF = r2 * ipn * sdF + 1.0;
+    const double dF = r2 * B.ipn * sdF + 1.0;
     u[p] = u[p] - Fv / dF;
   } else if (mode == ROBIN_OPERATOR_R) {
     D.r[p] = op;
@@ -309,10 +333,14 @@ __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, i
   }
 }
 
-// face column t (0 .. 2(nj nk + ni nk + ni nj) - 1) of a box: all R layers,
-// inner -> outer
-__device__ __forceinline__ void robin_face_column(const DevBox &B, BoxData D, int t, int mode,
-                                                  double ipn) {
+__device__ __forceinline__ int face_columns(const DevBox &B) {
+  const int R = B.R;
+  const int ni = B.n[0] - 2 * R, nj = B.n[1] - 2 * R, nk = B.n[2] - 2 * R;
+  return 2 * (nj * nk + ni * nk + ni * nj);
+}
+
+// face column t of a box: all R layers, inner -> outer
+__device__ __forceinline__ void robin_face_column(const DevBox &B, BoxData D, int t, int mode) {
   const int R = B.R;
   const int ni = B.n[0] - 2 * R, nj = B.n[1] - 2 * R, nk = B.n[2] - 2 * R;
   const int fx = nj * nk, fy = ni * nk, fz = ni * nj;
@@ -337,15 +365,8 @@ __device__ __forceinline__ void robin_face_column(const DevBox &B, BoxData D, in
     } else {
       i = R + a0; j = R + a1; k = g;
     }
-    robin_point(B, D, i, j, k, 1 << axis, mode, ipn);
+    robin_point(B, D, i, j, k, 1 << axis, mode);
   }
-}
-
-// faces: one thread per (face, column); layers inner -> outer
-__global__ __launch_bounds__(256) void k_robin_faces(const DevBox *__restrict__ boxes, int box,
-                                                      Fields F, int mode, double ipn) {
-  const DevBox &B = boxes[box];
-  robin_face_column(B, box_data(B, F), blockIdx.x * blockDim.x + threadIdx.x, mode, ipn);
 }
 
 // ghost index of position q (0..R-1) on side `hi` of an axis, in the
@@ -355,8 +376,8 @@ __device__ __forceinline__ int shell_index(int q, int hi, int n, int R) {
   return hi ? n - R + q : R - 1 - q;
 }
 
-// edges then corners, one block per box (faces must be done)
-__device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode, double ipn) {
+// edges then corners of one box, by one block (faces must be done)
+__device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode) {
   const int R = B.R;
   const int nint[3] = {B.n[0] - 2 * R, B.n[1] - 2 * R, B.n[2] - 2 * R};
   // edges XY(4) run along z, XZ(4) along y, YZ(4) along x; within an edge
@@ -385,137 +406,53 @@ __device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode, double
         } else {  // YiZi, YiZf, YfZi, YfZf: z outer, y inner; along x
           j = shell_index(qb, hA, B.n[1], R); k = shell_index(qa, hB, B.n[2], R); i = R + s; mask = 6;
         }
-        robin_point(B, D, i, j, k, mask, mode, ipn);
+        robin_point(B, D, i, j, k, mask, mode);
       }
   }
   __syncthreads();
   // corners: k outer, j, i inner (domain.cc:592-695)
   if (threadIdx.x < 8) {
     const int c = threadIdx.x;
-    // reference corner order XiYiZi, XiYiZf, XfYiZi, XfYiZf, XiYfZi, ...
     const int hz = c & 1, hx = (c >> 1) & 1, hy = (c >> 2) & 1;
     for (int qk = 0; qk < R; ++qk)
       for (int qj = 0; qj < R; ++qj)
         for (int qi = 0; qi < R; ++qi)
           robin_point(B, D, shell_index(qi, hx, B.n[0], R), shell_index(qj, hy, B.n[1], R),
-                      shell_index(qk, hz, B.n[2], R), 7, mode, ipn);
+                      shell_index(qk, hz, B.n[2], R), 7, mode);
   }
 }
 
-__global__ __launch_bounds__(1024) void k_robin_edges_corners(const DevBox *__restrict__ boxes,
-                                                               int box, Fields F, int mode,
-                                                               double ipn) {
-  robin_edges_corners(boxes[box], box_data(boxes[box], F), mode, ipn);
-}
-
-// ------------------------------------------------------------------
-// transfers
-// ------------------------------------------------------------------
-
-struct AxisRef {
-  const int *start, *coinc;
-  const unsigned char *flags;
-  const double *w;
-  int k;
-};
-
-__device__ __forceinline__ AxisRef axis_ref(const XferView &xv, const DevAxis &a) {
-  AxisRef r;
-  r.start = xv.ipool + a.start_off;
-  r.coinc = xv.ipool + a.coinc_off;
-  r.flags = xv.bpool + a.flags_off;
-  r.w = xv.dpool + a.w_off;
-  r.k = a.k;
-  return r;
-}
-
-__device__ __forceinline__ bool in_region(const DevBox &B, int i, int j, int k, int region) {
-  if (region == REG_ALL) return true;
-  const int R = B.R;
-  const bool inside = i >= R && i < B.n[0] - R && j >= R && j < B.n[1] - R && k >= R &&
-                      k < B.n[2] - R;
-  return region == REG_INSIDE ? inside : !inside;
-}
-
-// tensor interpolation of field f of box S (reference accumulation order,
-// interpolation.cc:257-296)
-__device__ double tensor_interp(const DevBox &S, const double *__restrict__ f, const AxisRef ax[3],
-                                const int t[3]) {
-  const int kx = ax[0].k;
-  const int s0 = ax[0].start[t[0]], s1 = ax[1].start[t[1]], s2 = ax[2].start[t[2]];
-  const double *wx = ax[0].w + (long long)t[0] * kx;
-  const double *wy = ax[1].w + (long long)t[1] * ax[1].k;
-  const double *wz = ax[2].w + (long long)t[2] * ax[2].k;
-  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
-  double acc = 0.0;
-  for (int I = 0; I < kx; ++I) {
-    double uj = 0.0;
-    for (int J = 0; J < ax[1].k; ++J) {
-      double uk = 0.0;
-      const long long base = S.off + (s0 + I) + sy * (s1 + J) + sz * (long long)s2;
-      for (int K = 0; K < ax[2].k; ++K) uk += f[base + sz * K] * wz[K];
-      uj += uk * wy[J];
-    }
-    acc += uj * wx[I];
+// faces: one thread per (box, face column)
+__global__ __launch_bounds__(256) void k_robin_faces(const RobinItem *__restrict__ items, int n,
+                                                      int total, int mode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].col0 <= g) lo = mid; else hi = mid - 1;
   }
-  return acc;
+  const DevBox &B = *items[lo].box;
+  robin_face_column(B, box_data(B), g - items[lo].col0, mode);
 }
 
-// trilinear interpolation of (u - ww) of box S (interpolation.cc:125-182)
-__device__ __forceinline__ double trilinear_corr(const DevBox &S, const double *__restrict__ u,
-                                                 const double *__restrict__ ww, const AxisRef ax[3],
-                                                 const int t[3]) {
-  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
-  const long long b = S.off + ax[0].start[t[0]] + sy * ax[1].start[t[1]] + sz * ax[2].start[t[2]];
-  const double dxi = ax[0].w[2 * t[0]], dxf = ax[0].w[2 * t[0] + 1];
-  const double dyi = ax[1].w[2 * t[1]], dyf = ax[1].w[2 * t[1] + 1];
-  const double dzi = ax[2].w[2 * t[2]], dzf = ax[2].w[2 * t[2] + 1];
-#define V(o) (u[b + (o)] - ww[b + (o)])
-  return ((V(0) * dxi + V(1) * dxf) * dyi + (V(sy) * dxi + V(sy + 1) * dxf) * dyf) * dzi +
-         ((V(sz) * dxi + V(sz + 1) * dxf) * dyi + (V(sz + sy) * dxi + V(sz + sy + 1) * dxf) * dyf) *
-             dzf;
-#undef V
-}
-
-// u restriction: coarse INSIDE points inside a fine box's non-ghost part
-__global__ __launch_bounds__(256) void k_restrict_u(XferView xv, const DevBox *__restrict__ boxes,
-                                                     Fields F) {
-  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= xv.tgt.pt0[xv.tgt.nbox]) return;
-  const int bi = find_box_pt(xv.tgt, pt);
-  const DevBox &T = boxes[xv.tgt.box[bi]];
-  const long long q = pt - xv.tgt.pt0[bi];
-  const int i = int(q % T.n[0]), j = int((q / T.n[0]) % T.n[1]), k = int(q / ((long long)T.n[0] * T.n[1]));
-  if (!in_region(T, i, j, k, REG_INSIDE)) return;
-  const int t[3] = {i, j, k};
-  for (int x = xv.x0[bi + 1] - 1; x >= xv.x0[bi]; --x) {
-    const DevXfer &X = xv.xfer[x];
-    AxisRef ax[3] = {axis_ref(xv, X.ax[0]), axis_ref(xv, X.ax[1]), axis_ref(xv, X.ax[2])};
-    if (!((ax[0].flags[i] & ax[1].flags[j] & ax[2].flags[k]) & 1)) continue;  // all=false: nb only
-    const DevBox &S = boxes[X.source];
-    double v;
-    if ((ax[0].flags[i] & ax[1].flags[j] & ax[2].flags[k]) & 4) {
-      v = F.f[F_U][S.off + ax[0].coinc[i] + (long long)S.n[0] * ax[1].coinc[j] +
-                   (long long)S.n[0] * S.n[1] * ax[2].coinc[k]];
-    } else {
-      v = tensor_interp(S, F.f[F_U], ax, t);
-    }
-    F.f[F_U][T.off + q] = v;
-    return;
-  }
+// edges and corners: one block per box
+__global__ __launch_bounds__(1024) void k_robin_edges_corners(const RobinItem *__restrict__ items,
+                                                               int mode) {
+  const DevBox &B = *items[blockIdx.x].box;
+  robin_edges_corners(B, box_data(B), mode);
 }
 
 // Robin operator value at a shell point (no update)
-__device__ __forceinline__ double robin_op_at(const DevBox &B, Fields F, int i, int j, int k,
-                                              double ipn) {
+__device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int k) {
   const int R = B.R;
   int mask = 0;
   if (i < R || i >= B.n[0] - R) mask |= 1;
   if (j < R || j >= B.n[1] - R) mask |= 2;
   if (k < R || k >= B.n[2] - R) mask |= 4;
   const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  const long long p = B.off + i + sy * j + sz * k;
-  const double *__restrict__ u = F.f[F_U];
+  const long long p = i + sy * j + sz * k;
+  const double *__restrict__ u = B.f[F_U];
   const int ijk[3] = {i, j, k};
   const long long st[3] = {1, sy, sz};
   double X[3];
@@ -527,42 +464,127 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, Fields F, int i, 
     double cf;
     sF += deriv(u, p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf) / X[a];
   }
-  return r2 * ipn * sF + u[p];
+  return r2 * B.ipn * sF + u[p];
+}
+
+// ------------------------------------------------------------------
+// transfers
+// ------------------------------------------------------------------
+
+__device__ __forceinline__ long long lin(const DevBox &S, int i, int j, int k) {
+  return i + (long long)S.n[0] * (j + (long long)S.n[1] * k);
+}
+
+// tensor Lagrange interpolation of f over box S (reference accumulation
+// order, interpolation.cc:257-296); the nodes of coincident axes are
+// skipped, their weights being exactly one unit vector
+__device__ double tensor_interp(const DevBox &S, const double *__restrict__ f, const DevAxis *ax,
+                                const int t[3], const unsigned fl[3]) {
+  int s0[3], kk[3];
+  const double *w[3];
+  for (int a = 0; a < 3; ++a) {
+    if (fl[a] & 4) {
+      s0[a] = ax[a].coinc[t[a]];
+      kk[a] = 1;
+      w[a] = nullptr;
+    } else {
+      s0[a] = ax[a].start[t[a]];
+      kk[a] = ax[a].k;
+      w[a] = ax[a].w + (long long)t[a] * ax[a].k;
+    }
+  }
+  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
+  double acc = 0.0;
+  for (int I = 0; I < kk[0]; ++I) {
+    double uj = 0.0;
+    for (int J = 0; J < kk[1]; ++J) {
+      double uk = 0.0;
+      const long long base = (s0[0] + I) + sy * (s0[1] + J) + sz * (long long)s0[2];
+      for (int K = 0; K < kk[2]; ++K) uk += f[base + sz * K] * (w[2] ? w[2][K] : 1.0);
+      uj += uk * (w[1] ? w[1][J] : 1.0);
+    }
+    acc += uj * (w[0] ? w[0][I] : 1.0);
+  }
+  return acc;
+}
+
+// trilinear interpolation of (u - ww) of box S (interpolation.cc:125-182)
+__device__ __forceinline__ double trilinear_corr(const DevBox &S, const DevAxis *ax, const int t[3]) {
+  const double *__restrict__ u = S.f[F_U];
+  const double *__restrict__ ww = S.f[F_WW];
+  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
+  const long long b = ax[0].start[t[0]] + sy * ax[1].start[t[1]] + sz * ax[2].start[t[2]];
+  const double dxi = ax[0].w[2 * t[0]], dxf = ax[0].w[2 * t[0] + 1];
+  const double dyi = ax[1].w[2 * t[1]], dyf = ax[1].w[2 * t[1] + 1];
+  const double dzi = ax[2].w[2 * t[2]], dzf = ax[2].w[2 * t[2] + 1];
+#define V(o) (u[b + (o)] - ww[b + (o)])
+  return ((V(0) * dxi + V(1) * dxf) * dyi + (V(sy) * dxi + V(sy + 1) * dxf) * dyf) * dzi +
+         ((V(sz) * dxi + V(sz + 1) * dxf) * dyi + (V(sz + sy) * dxi + V(sz + sy + 1) * dxf) * dyf) *
+             dzf;
+#undef V
+}
+
+__device__ __forceinline__ unsigned axis_flags(const DevXfer &X, const int t[3], unsigned fl[3]) {
+  fl[0] = X.ax[0].flags[t[0]];
+  fl[1] = X.ax[1].flags[t[1]];
+  fl[2] = X.ax[2].flags[t[2]];
+  return fl[0] & fl[1] & fl[2];
+}
+
+// u restriction: coarse INSIDE points inside a fine box's non-ghost part
+// (all = false), injection at coincident points
+__global__ __launch_bounds__(256) void k_restrict_u(const PtItem *__restrict__ items, int n,
+                                                     long long total) {
+  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= total) return;
+  const PtItem &I = items[find_pt_item(items, n, pt)];
+  const DevBox &T = *I.box;
+  int i, j, k;
+  point_ijk(T, pt - I.p0, i, j, k);
+  if (!interior(T, i, j, k)) return;
+  const int t[3] = {i, j, k};
+  for (int x = I.nsrc - 1; x >= 0; --x) {
+    const DevXfer &X = I.src[x];
+    unsigned fl[3];
+    if (!(axis_flags(X, t, fl) & 1)) continue;
+    T.f[F_U][pt - I.p0] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
+    return;
+  }
 }
 
 // Coarse FAS source: W_c = restriction of the fine defect (full weighting
 // where the coarse point coincides with a fine point whose 3x3x3
 // neighbourhood is fine interior, injection / Lagrange-2 elsewhere), then
-// rho_c = W_c + R_c with R_c = N(u_c) inside, Robin operator on the shell
-// (multigrid levels).  Refinement levels overwrite rho_c only where the
-// point is inside both boxes' non-ghost parts (ffunction.cc:2808-2849).
+// rho_c = W_c + R_c with R_c = N(u_c) inside, the Robin operator on the
+// shell (multigrid levels).  Refinement levels overwrite rho_c only where
+// the point is inside both boxes' non-ghost parts (ffunction.cc:2808-2849).
 // Also saves ww_c = u_c on every coarse point.
 template <int ORD, int EQ>
-__global__ __launch_bounds__(256) void k_restrict_defect(XferView xv, const DevBox *__restrict__ boxes,
-                                                          Fields F, int mg, double ipn) {
+__global__ __launch_bounds__(256) void k_restrict_defect(const PtItem *__restrict__ items, int n,
+                                                          long long total) {
   const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= xv.tgt.pt0[xv.tgt.nbox]) return;
-  const int bi = find_box_pt(xv.tgt, pt);
-  const DevBox &T = boxes[xv.tgt.box[bi]];
-  const long long q = pt - xv.tgt.pt0[bi];
-  const int i = int(q % T.n[0]), j = int((q / T.n[0]) % T.n[1]), k = int(q / ((long long)T.n[0] * T.n[1]));
-  const long long p = T.off + q;
-  F.f[F_WW][p] = F.f[F_U][p];
-  const bool inside = in_region(T, i, j, k, REG_INSIDE);
-  if (!mg && !inside) return;
+  if (pt >= total) return;
+  const PtItem &I = items[find_pt_item(items, n, pt)];
+  const DevBox &T = *I.box;
+  const long long p = pt - I.p0;
+  int i, j, k;
+  point_ijk(T, p, i, j, k);
+  T.f[F_WW][p] = T.f[F_U][p];
+  const bool inside = interior(T, i, j, k);
+  if (!I.mg && !inside) return;
   const int t[3] = {i, j, k};
   bool covered = false;
   double wc = 0.0;
-  for (int x = xv.x0[bi + 1] - 1; x >= xv.x0[bi]; --x) {
-    const DevXfer &X = xv.xfer[x];
-    AxisRef ax[3] = {axis_ref(xv, X.ax[0]), axis_ref(xv, X.ax[1]), axis_ref(xv, X.ax[2])};
-    const unsigned f = ax[0].flags[i] & ax[1].flags[j] & ax[2].flags[k];
-    if (!((f & 1) || (mg && (f & 2)))) continue;
-    const DevBox &S = boxes[X.source];
-    const double *__restrict__ w = F.f[F_W];
+  for (int x = I.nsrc - 1; x >= 0; --x) {
+    const DevXfer &X = I.src[x];
+    unsigned fl[3];
+    const unsigned f = axis_flags(X, t, fl);
+    if (!((f & 1) || (I.mg && (f & 2)))) continue;
+    const DevBox &S = *X.src;
+    const double *__restrict__ w = S.f[F_W];
     if (f & 8) {
       const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
-      const long long c = S.off + ax[0].coinc[i] + sy * ax[1].coinc[j] + sz * ax[2].coinc[k];
+      const long long c = lin(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
       double acc = 0.0;
       for (int dz = -1; dz <= 1; ++dz) {
         const double wz = dz == 0 ? 0.5 : 0.25;
@@ -573,62 +595,59 @@ __global__ __launch_bounds__(256) void k_restrict_defect(XferView xv, const DevB
         }
       }
       wc = acc;
-    } else if (f & 4) {
-      wc = w[S.off + ax[0].coinc[i] + (long long)S.n[0] * ax[1].coinc[j] +
-             (long long)S.n[0] * S.n[1] * ax[2].coinc[k]];
     } else {
-      wc = tensor_interp(S, w, ax, t);
+      wc = tensor_interp(S, w, X.ax, t, fl);
     }
     covered = true;
     break;
   }
-  if (!covered && !mg) return;
-  if (covered) F.f[F_W][p] = wc; else wc = F.f[F_W][p];
+  if (!covered && !I.mg) return;
+  if (covered) T.f[F_W][p] = wc; else wc = T.f[F_W][p];
   double Rc;
   if (inside) {
     const long long sy = T.n[0], sz = (long long)T.n[0] * T.n[1];
-    const double *__restrict__ u = F.f[F_U];
+    const double *__restrict__ u = T.f[F_U];
     double dN;
-    nonlinear<EQ>(lap<ORD>(u, p, sy, sz, T.idx2[0], T.idx2[1], T.idx2[2]), T.dulap, F.f[F_B][p],
-                  F.f[F_C][p] + u[p], EQ ? F.f[F_D][p] : 0.0, EQ ? F.f[F_E][p] : 0.0, Rc, dN);
+    nonlinear<EQ>(lap<ORD>(u, p, sy, sz, T.idx2[0], T.idx2[1], T.idx2[2]), T.dulap, T.f[F_B][p],
+                  T.f[F_C][p] + u[p], EQ ? T.f[F_D][p] : 0.0, EQ ? T.f[F_E][p] : 0.0, Rc, dN);
   } else {
-    Rc = robin_op_at(T, F, i, j, k, ipn);
+    Rc = robin_op_at(T, i, j, k);
   }
-  F.f[F_RHO][p] = wc + Rc;
+  T.f[F_RHO][p] = wc + Rc;
 }
 
-// prolongation of the coarse correction and u += v
-__global__ __launch_bounds__(256) void k_prolong(XferView xv, const DevBox *__restrict__ boxes,
-                                                  Fields F, int region) {
+// prolongation of the coarse correction and u += v (all points on
+// multigrid levels, interior points on refinement levels)
+__global__ __launch_bounds__(256) void k_prolong(const PtItem *__restrict__ items, int n,
+                                                  long long total) {
   const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= xv.tgt.pt0[xv.tgt.nbox]) return;
-  const int bi = find_box_pt(xv.tgt, pt);
-  const DevBox &T = boxes[xv.tgt.box[bi]];
-  const long long q = pt - xv.tgt.pt0[bi];
-  const int i = int(q % T.n[0]), j = int((q / T.n[0]) % T.n[1]), k = int(q / ((long long)T.n[0] * T.n[1]));
-  if (!in_region(T, i, j, k, region)) return;
+  if (pt >= total) return;
+  const PtItem &I = items[find_pt_item(items, n, pt)];
+  const DevBox &T = *I.box;
+  const long long p = pt - I.p0;
+  int i, j, k;
+  point_ijk(T, p, i, j, k);
+  if (!I.mg && !interior(T, i, j, k)) return;
   const int t[3] = {i, j, k};
-  for (int x = xv.x0[bi + 1] - 1; x >= xv.x0[bi]; --x) {
-    const DevXfer &X = xv.xfer[x];
-    AxisRef ax[3] = {axis_ref(xv, X.ax[0]), axis_ref(xv, X.ax[1]), axis_ref(xv, X.ax[2])};
-    const unsigned f = ax[0].flags[i] & ax[1].flags[j] & ax[2].flags[k];
+  for (int x = I.nsrc - 1; x >= 0; --x) {
+    const DevXfer &X = I.src[x];
+    unsigned fl[3];
+    const unsigned f = axis_flags(X, t, fl);
     if (!(f & 3)) continue;  // all = true
-    const DevBox &S = boxes[X.source];
+    const DevBox &S = *X.src;
     double v;
     if (f & 4) {
-      const long long c = S.off + ax[0].coinc[i] + (long long)S.n[0] * ax[1].coinc[j] +
-                          (long long)S.n[0] * S.n[1] * ax[2].coinc[k];
-      v = F.f[F_U][c] - F.f[F_WW][c];
+      const long long c = lin(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      v = S.f[F_U][c] - S.f[F_WW][c];
     } else {
-      v = trilinear_corr(S, F.f[F_U], F.f[F_WW], ax, t);
+      v = trilinear_corr(S, X.ax, t);
     }
-    F.f[F_U][T.off + q] += v;
+    T.f[F_U][p] += v;
     return;
   }
 }
 
-// shell point q (0 .. n0 n1 n2 - (n0-2R)(n1-2R)(n2-2R) - 1) of a box:
-// the R-thick z slabs first, then y slabs, then x slabs
+// shell point q of a box: the R-thick z slabs first, then y slabs, then x slabs
 __device__ __forceinline__ void shell_point(const DevBox &B, unsigned q, int &i, int &j, int &k) {
   const unsigned R = unsigned(B.R);
   const unsigned n0 = unsigned(B.n[0]), n1 = unsigned(B.n[1]), n2 = unsigned(B.n[2]);
@@ -663,98 +682,56 @@ __device__ __forceinline__ void shell_point(const DevBox &B, unsigned q, int &i,
   i = s ? int(n0 - R + l) : int(l);
 }
 
-// Lagrange tensor interpolation that skips the nodes of coincident axes
-// (their weights are exactly one unit vector)
-__device__ double tensor_interp_sparse(const DevBox &S, const double *__restrict__ f,
-                                       const AxisRef ax[3], const int t[3], unsigned fl0,
-                                       unsigned fl1, unsigned fl2) {
-  int s0[3], kk[3];
-  const double *w[3];
-  const unsigned fl[3] = {fl0, fl1, fl2};
-  for (int a = 0; a < 3; ++a) {
-    if (fl[a] & 4) {
-      s0[a] = ax[a].coinc[t[a]];
-      kk[a] = 1;
-      w[a] = nullptr;
-    } else {
-      s0[a] = ax[a].start[t[a]];
-      kk[a] = ax[a].k;
-      w[a] = ax[a].w + (long long)t[a] * ax[a].k;
-    }
-  }
-  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
-  double acc = 0.0;
-  for (int I = 0; I < kk[0]; ++I) {
-    double uj = 0.0;
-    for (int J = 0; J < kk[1]; ++J) {
-      double uk = 0.0;
-      const long long base = S.off + (s0[0] + I) + sy * (s0[1] + J) + sz * (long long)s0[2];
-      for (int K = 0; K < kk[2]; ++K) uk += f[base + sz * K] * (w[2] ? w[2][K] : 1.0);
-      uj += uk * (w[1] ? w[1][J] : 1.0);
-    }
-    acc += uj * (w[0] ? w[0][I] : 1.0);
-  }
-  return acc;
-}
-
-// ghost shell of refinement boxes from the coarser level (Lagrange order+1);
-// one thread per shell point
-__global__ __launch_bounds__(256) void k_ghost_fill(XferView xv, const DevBox *__restrict__ boxes,
-                                                     Fields F) {
+// ghost shell of refinement boxes from the coarser level (Lagrange
+// order+1); one thread per shell point
+__global__ __launch_bounds__(256) void k_ghost_fill(const PtItem *__restrict__ items, int n,
+                                                     long long total) {
   const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nb = xv.tgt.nbox;
-  if (pt >= xv.tgt.shell0[nb]) return;
-  int bi = 0;
-  while (bi + 1 < nb && pt >= xv.tgt.shell0[bi + 1]) ++bi;
-  const DevBox &T = boxes[xv.tgt.box[bi]];
+  if (pt >= total) return;
+  const PtItem &I = items[find_pt_item(items, n, pt)];
+  const DevBox &T = *I.box;
   int i, j, k;
-  shell_point(T, unsigned(pt - xv.tgt.shell0[bi]), i, j, k);
+  shell_point(T, unsigned(pt - I.p0), i, j, k);
   const int t[3] = {i, j, k};
-  for (int x = xv.x0[bi + 1] - 1; x >= xv.x0[bi]; --x) {
-    const DevXfer &X = xv.xfer[x];
-    AxisRef ax[3] = {axis_ref(xv, X.ax[0]), axis_ref(xv, X.ax[1]), axis_ref(xv, X.ax[2])};
-    const unsigned f0 = ax[0].flags[i], f1 = ax[1].flags[j], f2 = ax[2].flags[k];
-    const unsigned f = f0 & f1 & f2;
+  for (int x = I.nsrc - 1; x >= 0; --x) {
+    const DevXfer &X = I.src[x];
+    unsigned fl[3];
+    const unsigned f = axis_flags(X, t, fl);
     if (!(f & 3)) continue;
-    const DevBox &S = boxes[X.source];
-    const double v = tensor_interp_sparse(S, F.f[F_U], ax, t, f0, f1, f2);
-    F.f[F_U][T.off + i + (long long)T.n[0] * j + (long long)T.n[0] * T.n[1] * k] = v;
+    T.f[F_U][lin(T, i, j, k)] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
     return;
   }
 }
 
 // ------------------------------------------------------------------
 // coarsest level: {Robin, one multi-colour sweep} until the sweep norm is
-// <= eps or stops changing (elliptic.cc:1782-1816); one block, one launch.
-// u lives in LDS for the whole loop.  Interior updates are over-relaxed by
-// omega (omega = 1 is the reference's Gauss-Seidel-Newton); the loop's fixed
+// <= eps or stops changing (elliptic.cc:1782-1816); one block per
+// hierarchy, u and rho in LDS for the whole loop, b, c (d, e) of the owned
+// interior points in registers.  Interior updates are over-relaxed by omega
+// (omega = 1 is the reference's Gauss-Seidel-Newton); the loop's fixed
 // point, the coarse solution, does not depend on omega.
 // ------------------------------------------------------------------
-constexpr int kCoarseOwn = 8;  // interior points per thread (1024 threads)
-
 template <int EQ>
-__global__ __launch_bounds__(1024) void k_coarse_solve(const DevBox *__restrict__ boxes, int box,
-                                                        Fields F, double ipn, double eps,
-                                                        double omega, int max_iter,
-                                                        unsigned long long *__restrict__ norm,
-                                                        int *__restrict__ iters) {
+__global__ __launch_bounds__(1024) void k_coarse_solve(const CoarseItem *__restrict__ items,
+                                                        double eps, int max_iter) {
   extern __shared__ double lds[];
   __shared__ double sh[16];
   __shared__ double s_norm;
-  const DevBox &B = boxes[box];
+  const CoarseItem &It = items[blockIdx.x];
+  const DevBox &B = *It.box;
+  const double omega = It.omega;
   const int R = B.R;  // 1: the coarsest level always uses order 2
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int np = n0 * n1 * n2;
-  double *su = lds;        // u
-  double *srho = lds + np; // rho
+  double *su = lds;
+  double *srho = lds + np;
   const int ni = n0 - 2 * R, nj = n1 - 2 * R, nk = n2 - 2 * R;
   const long long sy = n0, sz = (long long)n0 * n1;
-  double *__restrict__ ug = F.f[F_U] + B.off;
+  double *__restrict__ ug = B.f[F_U];
   for (int q = threadIdx.x; q < np; q += blockDim.x) {
     su[q] = ug[q];
-    srho[q] = F.f[F_RHO][B.off + q];
+    srho[q] = B.f[F_RHO][q];
   }
-  // owned interior points: t = threadIdx.x + m * blockDim.x
   const int nint = ni * nj * nk;
   int own_p[kCoarseOwn], own_c[kCoarseOwn];
   double own_b[kCoarseOwn], own_cc[kCoarseOwn], own_d[kCoarseOwn], own_e[kCoarseOwn];
@@ -769,11 +746,11 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(const DevBox *__restrict_
       const int p = i + n0 * (j + n1 * k);
       own_p[m] = p;
       own_c[m] = (i + j + k) % (R + 1);
-      own_b[m] = F.f[F_B][B.off + p];
-      own_cc[m] = F.f[F_C][B.off + p];
+      own_b[m] = B.f[F_B][p];
+      own_cc[m] = B.f[F_C][p];
       if (EQ) {
-        own_d[m] = F.f[F_D][B.off + p];
-        own_e[m] = F.f[F_E][B.off + p];
+        own_d[m] = B.f[F_D][p];
+        own_e[m] = B.f[F_E][p];
       }
     }
   }
@@ -783,14 +760,14 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(const DevBox *__restrict_
   D.rho = srho;
   D.r = nullptr;
   D.w = nullptr;
-  const int nface = 2 * (nj * nk + ni * nk + ni * nj);
+  const int nface = face_columns(B);
   double prev = 1.0, cur = 1.0;
   int it = 0;
   do {
     prev = cur;
-    for (int t = threadIdx.x; t < nface; t += blockDim.x) robin_face_column(B, D, t, ROBIN_UPDATE, ipn);
+    for (int t = threadIdx.x; t < nface; t += blockDim.x) robin_face_column(B, D, t, ROBIN_UPDATE);
     __syncthreads();
-    robin_edges_corners(B, D, ROBIN_UPDATE, ipn);
+    robin_edges_corners(B, D, ROBIN_UPDATE);
     __syncthreads();
     double m = 0.0;
     for (int col = 0; col <= R; ++col) {
@@ -818,13 +795,13 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(const DevBox *__restrict_
   } while (cur > eps && prev != cur && it < max_iter && !isinf(cur));
   for (int q = threadIdx.x; q < np; q += blockDim.x) ug[q] = su[q];
   if (threadIdx.x == 0) {
-    *norm = (unsigned long long)__double_as_longlong(cur);
-    *iters = it;
+    *It.norm = (unsigned long long)__double_as_longlong(cur);
+    *It.iters = it;
   }
 }
 
 // ------------------------------------------------------------------
-// setup: coefficients b, c [,d, e]; rho = robin value on the shell, 0
+// setup: coefficients b, c [,d, e]; rho = Robin value on the shell, 0
 // inside; u = 0
 // ------------------------------------------------------------------
 __device__ double punctures_c(const PunctureArgs &pa, double X, double Y, double Z) {
@@ -919,25 +896,22 @@ __device__ double sf_e(const PunctureArgs &pa, double r) {
   return 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_coefficients(DevBox B, Fields F, PunctureArgs pa,
-                                                       double robin_a) {
+__global__ __launch_bounds__(256) void k_coefficients(DevBox B, PunctureArgs pa, double robin_a) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long np = (long long)B.n[0] * B.n[1] * B.n[2];
   if (q >= np) return;
-  const int i = int(q % B.n[0]), j = int((q / B.n[0]) % B.n[1]), k = int(q / ((long long)B.n[0] * B.n[1]));
+  int i, j, k;
+  point_ijk(B, q, i, j, k);
   const double x = coord(B, 0, i), y = coord(B, 1, j), z = coord(B, 2, k);
-  const long long p = B.off + q;
-  F.f[F_B][p] = punctures_b(pa, x, y, z);
-  F.f[F_C][p] = punctures_c(pa, x, y, z);
+  B.f[F_B][q] = punctures_b(pa, x, y, z);
+  B.f[F_C][q] = punctures_c(pa, x, y, z);
   if (pa.eq == 1) {
     const double r = sqrt(x * x + y * y + z * z);
-    F.f[F_D][p] = sf_d(pa, r);
-    F.f[F_E][p] = sf_e(pa, r);
+    B.f[F_D][q] = sf_d(pa, r);
+    B.f[F_E][q] = sf_e(pa, r);
   }
-  const int R = B.R;
-  const bool inside = i >= R && i < B.n[0] - R && j >= R && j < B.n[1] - R && k >= R && k < B.n[2] - R;
-  F.f[F_RHO][p] = inside ? 0.0 : robin_a;
-  F.f[F_U][p] = 0.0;
+  B.f[F_RHO][q] = interior(B, i, j, k) ? 0.0 : robin_a;
+  B.f[F_U][q] = 0.0;
 }
 
 // ------------------------------------------------------------------
@@ -963,23 +937,23 @@ __device__ int cell_of(const DevBox &B, int a, double x) {
   return i + 1;
 }
 
-__device__ double dU_at(const DevBox &B, Fields F, const double ctr[3], int i, int j, int k) {
+__device__ double dU_at(const DevBox &B, const double ctr[3], int i, int j, int k) {
   const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  const long long p = B.off + i + sy * j + sz * k;
+  const long long p = i + sy * j + sz * k;
   const int ijk[3] = {i, j, k};
   const long long st[3] = {1, sy, sz};
   double X[3], g[3];
   for (int a = 0; a < 3; ++a) {
     X[a] = coord(B, a, ijk[a]) - ctr[a];
     double cf;
-    g[a] = deriv(F.f[F_C], p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf) +
-           deriv(F.f[F_U], p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf);
+    g[a] = deriv(B.f[F_C], p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf) +
+           deriv(B.f[F_U], p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf);
   }
   const double ir = 1.0 / sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
   return ir * (X[0] * g[0] + X[1] * g[1] + X[2] * g[2]);
 }
 
-__global__ __launch_bounds__(256) void k_adm_sphere(const DevBox *__restrict__ boxes, Fields F,
+__global__ __launch_bounds__(256) void k_adm_sphere(const DevBox *__restrict__ boxes,
                                                      const double *__restrict__ centres,
                                                      double *__restrict__ sph, int nt) {
   const int box = blockIdx.y;
@@ -987,9 +961,9 @@ __global__ __launch_bounds__(256) void k_adm_sphere(const DevBox *__restrict__ b
   if (q >= nt * nt) return;
   const DevBox &B = boxes[box];
   const double ctr[3] = {centres[3 * box], centres[3 * box + 1], centres[3 * box + 2]};
-  double L0 = coord(B, 0, B.n[0] - 1) - B.lo[0];
-  double L1 = coord(B, 1, B.n[1] - 1) - B.lo[1];
-  double L2 = coord(B, 2, B.n[2] - 1) - B.lo[2];
+  const double L0 = coord(B, 0, B.n[0] - 1) - B.lo[0];
+  const double L1 = coord(B, 1, B.n[1] - 1) - B.lo[1];
+  const double L2 = coord(B, 2, B.n[2] - 1) - B.lo[2];
   const double r0 = 0.375 * fmin(L0, fmin(L1, L2));
   const int it = q / nt, ip = q % nt;
   const double dtheta = M_PI / (nt - 1), dphi = 2 * M_PI / (nt - 1);
@@ -1004,7 +978,7 @@ __global__ __launch_bounds__(256) void k_adm_sphere(const DevBox *__restrict__ b
   }
   double v;
   if (coinc) {
-    v = dU_at(B, F, ctr, c[0], c[1], c[2]);
+    v = dU_at(B, ctr, c[0], c[1], c[2]);
   } else {
     for (int a = 0; a < 3; ++a) c[a] = c[a] == 0 ? 1 : c[a];
     const double fx = (X[0] - coord(B, 0, c[0] - 1)) * B.idx[0];
@@ -1012,24 +986,24 @@ __global__ __launch_bounds__(256) void k_adm_sphere(const DevBox *__restrict__ b
     const double fz = (X[2] - coord(B, 2, c[2] - 1)) * B.idx[2];
     const double gx = 1 - fx, gy = 1 - fy, gz = 1 - fz;
     const int I = c[0], J = c[1], K = c[2];
-    v = ((dU_at(B, F, ctr, I - 1, J - 1, K - 1) * gx + dU_at(B, F, ctr, I, J - 1, K - 1) * fx) * gy +
-         (dU_at(B, F, ctr, I - 1, J, K - 1) * gx + dU_at(B, F, ctr, I, J, K - 1) * fx) * fy) * gz +
-        ((dU_at(B, F, ctr, I - 1, J - 1, K) * gx + dU_at(B, F, ctr, I, J - 1, K) * fx) * gy +
-         (dU_at(B, F, ctr, I - 1, J, K) * gx + dU_at(B, F, ctr, I, J, K) * fx) * fy) * fz;
+    v = ((dU_at(B, ctr, I - 1, J - 1, K - 1) * gx + dU_at(B, ctr, I, J - 1, K - 1) * fx) * gy +
+         (dU_at(B, ctr, I - 1, J, K - 1) * gx + dU_at(B, ctr, I, J, K - 1) * fx) * fy) * gz +
+        ((dU_at(B, ctr, I - 1, J - 1, K) * gx + dU_at(B, ctr, I, J - 1, K) * fx) * gy +
+         (dU_at(B, ctr, I - 1, J, K) * gx + dU_at(B, ctr, I, J, K) * fx) * fy) * fz;
   }
   sph[(long long)box * nt * nt + q] = v * sin(th);
 }
 
 // Lagrange interpolation of one field at arbitrary points (one box per
 // point), interpolation.cc:189-303
-__global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__ boxes, Fields F,
-                                                       int field, const int *__restrict__ pbox,
+__global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__ boxes, int field,
+                                                       const int *__restrict__ pbox,
                                                        const double *__restrict__ pxyz, int npts,
                                                        int ord, double *__restrict__ out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npts) return;
   const DevBox &B = boxes[pbox[q]];
-  const double *__restrict__ f = F.f[field];
+  const double *__restrict__ f = B.f[field];
   const double X[3] = {pxyz[3 * q], pxyz[3 * q + 1], pxyz[3 * q + 2]};
   int c[3], s[3];
   bool coinc = true;
@@ -1039,7 +1013,7 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
   }
   const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
   if (coinc) {
-    out[q] = f[B.off + c[0] + sy * c[1] + sz * c[2]];
+    out[q] = f[c[0] + sy * c[1] + sz * c[2]];
     return;
   }
   double w[3][10];
@@ -1061,7 +1035,7 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
     double uj = 0.0;
     for (int J = 0; J <= ord; ++J) {
       double uk = 0.0;
-      for (int K = 0; K <= ord; ++K) uk += f[B.off + (s[0] + I) + sy * (s[1] + J) + sz * (s[2] + K)] * w[2][K];
+      for (int K = 0; K <= ord; ++K) uk += f[(s[0] + I) + sy * (s[1] + J) + sz * (s[2] + K)] * w[2][K];
       uj += uk * w[1][J];
     }
     acc += uj * w[0][I];
@@ -1069,31 +1043,51 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
   out[q] = acc;
 }
 
-__global__ void k_zero_u64(unsigned long long *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_zero_u64(unsigned long long *p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }
 
-__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
+// zero `count` words at each of n pointers (one block per pointer)
+__global__ void k_zero_lists(unsigned long long *const *ptrs, int count) {
+  unsigned long long *p = ptrs[blockIdx.x];
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0ull;
 }
 
 inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1) / b); }
 
-inline bool empty_grid(dim3 g) { return g.x == 0 || g.y == 0 || g.z == 0; }
+template <int ORD, int EQ>
+void smooth_impl(hipStream_t s, const RowItem *items, int n, int rows, int max_row_len, int colour) {
+  constexpr int NC = ORD / 2 + 1;
+  const unsigned per_row = unsigned((max_row_len + NC - 1) / NC);
+  const long long threads = (long long)rows * per_row;
+  if (threads == 0) return;
+  hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(ceil_div(threads, 256)), dim3(256), 0, s, items, n,
+                     unsigned(rows), per_row, 1.0f / float(per_row), colour);
+}
+
+template <int ORD, int EQ>
+void residual_impl(hipStream_t s, const RowItem *items, int n, int rows, int max_row_len,
+                   int write_defect) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(ceil_div(rows, 4), ceil_div(max_row_len, 64)),
+                     dim3(256), 0, s, items, n, unsigned(rows), write_defect);
+}
+
+template <int ORD, int EQ>
+void restrict_defect_impl(hipStream_t s, const PtItem *items, int n, long long total) {
+  if (total == 0) return;
+  hipLaunchKernelGGL((k_restrict_defect<ORD, EQ>), dim3(ceil_div(total, 256)), dim3(256), 0, s,
+                     items, n, total);
+}
+
+constexpr int kLdsLimit = 160 * 1024;
 
 }  // namespace
 
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
-
-void launch_coefficients_box(hipStream_t s, const DevBox &B, Fields F, const PunctureArgs &pa,
-                             double robin_a) {
-  const long long np = (long long)B.n[0] * B.n[1] * B.n[2];
-  hipLaunchKernelGGL(k_coefficients, dim3(ceil_div(np, 256)), dim3(256), 0, s, B, F, pa, robin_a);
-}
 
 #define OLL_DISPATCH_ORDER(ORDER, EQ, KERNEL, ...)                    \
   do {                                                                \
@@ -1114,122 +1108,91 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, Fields F, const Pun
     }                                                                 \
   } while (0)
 
-namespace {
-template <int ORD, int EQ>
-void smooth_impl(hipStream_t s, const LevelView &lv, const DevBox *boxes, Fields F, int colour,
-                 unsigned long long *norm) {
-  constexpr int NC = ORD / 2 + 1;
-  const long long rows = lv.row0[lv.nbox];
-  const unsigned per_row = unsigned((lv.max_row_len + NC - 1) / NC);
-  const dim3 grid(ceil_div(rows * per_row, 256));
-  if (empty_grid(grid)) return;
-  hipLaunchKernelGGL((k_smooth<ORD, EQ>), grid, dim3(256), 0, s, lv, boxes, F, colour, per_row,
-                     1.0f / float(per_row), norm);
+void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa,
+                             double robin_a) {
+  const long long np = (long long)B.n[0] * B.n[1] * B.n[2];
+  hipLaunchKernelGGL(k_coefficients, dim3(ceil_div(np, 256)), dim3(256), 0, s, B, pa, robin_a);
 }
 
-template <int ORD, int EQ>
-void residual_impl(hipStream_t s, const LevelView &lv, const DevBox *boxes, Fields F,
-                   int write_defect, unsigned long long *norm) {
-  const int rows = lv.row0[lv.nbox];
-  dim3 grid(ceil_div(rows, 4), ceil_div(lv.max_row_len, 64));
-  if (empty_grid(grid)) return;
-  hipLaunchKernelGGL((k_residual<ORD, EQ>), grid, dim3(256), 0, s, lv, boxes, F, write_defect,
-                     norm);
+void launch_smooth(hipStream_t s, const RowItem *items, int n, int total_rows, int max_row_len,
+                   int order, int eq, int colour) {
+  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, items, n, total_rows, max_row_len, colour);
 }
 
-template <int ORD, int EQ>
-void restrict_defect_impl(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F, int mg,
-                          double ipn) {
-  const long long np = xv.tgt.pt0[xv.tgt.nbox];
-  hipLaunchKernelGGL((k_restrict_defect<ORD, EQ>), dim3(ceil_div(np, 256)), dim3(256), 0, s, xv,
-                     boxes, F, mg, ipn);
-}
-}  // namespace
-
-void launch_smooth(hipStream_t s, const LevelView &lv, const DevBox *boxes, Fields F, int order,
-                   int eq, int colour, unsigned long long *norm) {
-  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, lv, boxes, F, colour, norm);
+void launch_residual(hipStream_t s, const RowItem *items, int n, int total_rows, int max_row_len,
+                     int order, int eq, int write_defect) {
+  OLL_DISPATCH_ORDER(order, eq, residual_impl, s, items, n, total_rows, max_row_len, write_defect);
 }
 
-void launch_residual(hipStream_t s, const LevelView &lv, const DevBox *boxes, Fields F, int order,
-                     int eq, int write_defect, unsigned long long *norm) {
-  OLL_DISPATCH_ORDER(order, eq, residual_impl, s, lv, boxes, F, write_defect, norm);
+void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_robin_faces, dim3(ceil_div(total_cols, 256)), dim3(256), 0, s, items, n,
+                     total_cols, mode);
+  hipLaunchKernelGGL(k_robin_edges_corners, dim3(n), dim3(1024), 0, s, items, mode);
 }
 
-void launch_robin(hipStream_t s, const DevBox *boxes, int box, const DevBox &hb, Fields F, int mode,
-                  double ipn) {
-  const int R = hb.R;
-  const int ni = hb.n[0] - 2 * R, nj = hb.n[1] - 2 * R, nk = hb.n[2] - 2 * R;
-  const long long cols = 2LL * (nj * nk + ni * nk + ni * nj);
-  hipLaunchKernelGGL(k_robin_faces, dim3(ceil_div(cols, 256)), dim3(256), 0, s, boxes, box, F, mode,
-                     ipn);
-  hipLaunchKernelGGL(k_robin_edges_corners, dim3(1), dim3(1024), 0, s, boxes, box, F, mode, ipn);
+void launch_restrict_u(hipStream_t s, const PtItem *items, int n, long long total) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_restrict_u, dim3(ceil_div(total, 256)), dim3(256), 0, s, items, n, total);
 }
 
-void launch_restrict_u(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F) {
-  const long long np = xv.tgt.pt0[xv.tgt.nbox];
-  hipLaunchKernelGGL(k_restrict_u, dim3(ceil_div(np, 256)), dim3(256), 0, s, xv, boxes, F);
+void launch_restrict_defect(hipStream_t s, const PtItem *items, int n, long long total, int order,
+                            int eq) {
+  OLL_DISPATCH_ORDER(order, eq, restrict_defect_impl, s, items, n, total);
 }
 
-void launch_restrict_defect(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F,
-                            int order, int eq, int mg, double ipn) {
-  OLL_DISPATCH_ORDER(order, eq, restrict_defect_impl, s, xv, boxes, F, mg, ipn);
+void launch_prolong(hipStream_t s, const PtItem *items, int n, long long total) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_prolong, dim3(ceil_div(total, 256)), dim3(256), 0, s, items, n, total);
 }
 
-void launch_prolong(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F, int region) {
-  const long long np = xv.tgt.pt0[xv.tgt.nbox];
-  hipLaunchKernelGGL(k_prolong, dim3(ceil_div(np, 256)), dim3(256), 0, s, xv, boxes, F, region);
+void launch_ghost_fill(hipStream_t s, const PtItem *items, int n, long long total) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_ghost_fill, dim3(ceil_div(total, 256)), dim3(256), 0, s, items, n, total);
 }
 
-void launch_ghost_fill(hipStream_t s, const XferView &xv, const DevBox *boxes, Fields F) {
-  const long long np = xv.tgt.shell0[xv.tgt.nbox];
-  if (np == 0) return;
-  hipLaunchKernelGGL(k_ghost_fill, dim3(ceil_div(np, 256)), dim3(256), 0, s, xv, boxes, F);
-}
-
-int coarse_solve_lds_limit() { return 160 * 1024; }
-
-bool launch_coarse_solve(hipStream_t s, const DevBox *boxes, int box, const DevBox &hb, Fields F,
-                         int eq, double ipn, double eps, double omega, int max_iter,
-                         unsigned long long *norm, int *iters) {
-  const size_t lds = 2 * sizeof(double) * size_t(hb.n[0]) * hb.n[1] * hb.n[2];
-  const long long nint = (long long)(hb.n[0] - 2) * (hb.n[1] - 2) * (hb.n[2] - 2);
-  if (lds + 2048 > size_t(coarse_solve_lds_limit()) || nint > 1024LL * kCoarseOwn) return false;
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<0>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, coarse_solve_lds_limit() - 1024);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<1>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, coarse_solve_lds_limit() - 1024);
-    attr_set = true;
-  }
+bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
+                         int eq, double eps, int max_iter) {
+  const size_t lds = 2 * sizeof(double) * size_t(max_pts);
+  if (lds + 2048 > size_t(kLdsLimit) || max_int > 1024 * kCoarseOwn) return false;
+  if (n == 0) return true;
   if (eq)
-    hipLaunchKernelGGL(k_coarse_solve<1>, dim3(1), dim3(1024), lds, s, boxes, box, F, ipn, eps,
-                       omega, max_iter, norm, iters);
+    hipLaunchKernelGGL(k_coarse_solve<1>, dim3(n), dim3(1024), lds, s, items, eps, max_iter);
   else
-    hipLaunchKernelGGL(k_coarse_solve<0>, dim3(1), dim3(1024), lds, s, boxes, box, F, ipn, eps,
-                       omega, max_iter, norm, iters);
+    hipLaunchKernelGGL(k_coarse_solve<0>, dim3(n), dim3(1024), lds, s, items, eps, max_iter);
   return true;
 }
 
-void launch_zero_u64(hipStream_t s, unsigned long long *p, int n) {
+void launch_zero_lists(hipStream_t s, unsigned long long *const *ptrs, int n, int count) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(256), 0, s, ptrs, count);
+}
+
+void kernels_init() {
+  static bool done = false;
+  if (done) return;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<0>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<1>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
+  done = true;
+}
+
+void launch_zero_u64(hipStream_t s, unsigned long long *p, long long n) {
+  if (n == 0) return;
   hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n);
 }
 
-void launch_copy(hipStream_t s, double *dst, const double *src, long long n) {
-  hipLaunchKernelGGL(k_copy, dim3(ceil_div(n, 256)), dim3(256), 0, s, dst, src, n);
-}
-
-void launch_adm_sphere(hipStream_t s, const DevBox *boxes, int nbox, Fields F,
-                       const double *centres, double *sph, int nt) {
-  hipLaunchKernelGGL(k_adm_sphere, dim3(ceil_div(nt * nt, 256), nbox), dim3(256), 0, s, boxes, F,
+void launch_adm_sphere(hipStream_t s, const DevBox *boxes, int nbox, const double *centres,
+                       double *sph, int nt) {
+  hipLaunchKernelGGL(k_adm_sphere, dim3(ceil_div(nt * nt, 256), nbox), dim3(256), 0, s, boxes,
                      centres, sph, nt);
 }
 
-void launch_interp_points(hipStream_t s, const DevBox *boxes, Fields F, int field, const int *pbox,
+void launch_interp_points(hipStream_t s, const DevBox *boxes, int field, const int *pbox,
                           const double *pxyz, int npts, int ord, double *out) {
-  hipLaunchKernelGGL(k_interp_points, dim3(ceil_div(npts, 64)), dim3(64), 0, s, boxes, F, field,
-                     pbox, pxyz, npts, ord, out);
+  hipLaunchKernelGGL(k_interp_points, dim3(ceil_div(npts, 64)), dim3(64), 0, s, boxes, field, pbox,
+                     pxyz, npts, ord, out);
 }
 
 }  // namespace oll

@@ -16,7 +16,8 @@ EQ_PUNCTURES, EQ_PUNCTURES_SCALAR = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 F_U, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW = range(10)
 NFIELDS = 10
-ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EDEVICE", -4: "ENAN", -5: "ENOMEM"}
+ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EDEVICE", -4: "ENAN", -5: "ENOMEM", -6: "EBUSY"}
+EBUSY = -6
 
 
 class Puncture(C.Structure):
@@ -78,6 +79,15 @@ SIGNATURES = {
     "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oll_last_solve_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "oll_reset": (C.c_int, [_vp]),
+    "oll_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "oll_engine_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(_vp)]),
+    "oll_engine_destroy": (None, [_vp]),
+    "oll_engine_admit": (C.c_int, [_vp, _P, C.c_longlong]),
+    "oll_engine_step": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "oll_engine_pop_done": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(Result),
+                                      C.POINTER(C.c_double)]),
+    "oll_engine_stats": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]),
     "oll_last_norms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "oll_last_solve_coarse_sweeps": (C.c_int, [_vp, C.POINTER(C.c_longlong)]),
     "oll_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
@@ -318,6 +328,9 @@ class Hierarchy:
         _check(self.L.oll_last_solve_coarse_sweeps(self.h, C.byref(out)))
         return out.value
 
+    def set_graphs(self, on):
+        _check(self.L.oll_set_graphs(self.h, 1 if on else 0))
+
     def reset(self):
         _check(self.L.oll_reset(self.h))
 
@@ -331,3 +344,57 @@ class Hierarchy:
 
     def device_bytes(self):
         return self.L.oll_device_bytes(self.h)
+
+
+class Engine:
+    """Many resident requests solved together (oll_engine)."""
+
+    def __init__(self, hbm_budget=0, graphs=True):
+        self.L = load()
+        h = _vp()
+        _check(self.L.oll_engine_create(hbm_budget, 1 if graphs else 0, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oll_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, problem, tag):
+        """True when admitted, False when the HBM budget is exhausted."""
+        rc = self.L.oll_engine_admit(self.h, C.byref(problem), tag)
+        if rc == EBUSY:
+            return False
+        _check(rc)
+        return True
+
+    def step(self):
+        n = C.c_int()
+        _check(self.L.oll_engine_step(self.h, C.byref(n)))
+        return n.value
+
+    def pop_done(self):
+        """List of (tag, cycles, residual, adm list or error code)."""
+        out = []
+        while True:
+            tag = C.c_longlong()
+            res = Result()
+            adm = (C.c_double * MAX_BH)()
+            rc = self.L.oll_engine_pop_done(self.h, C.byref(tag), C.byref(res), adm)
+            if rc == 0:
+                return out
+            if rc < 0:
+                out.append((tag.value, res.cycles, res.residual, rc))
+            else:
+                out.append((tag.value, res.cycles, res.residual, list(adm)))
+
+    def stats(self):
+        b, a, d = C.c_size_t(), C.c_int(), C.c_int()
+        _check(self.L.oll_engine_stats(self.h, C.byref(b), C.byref(a), C.byref(d)))
+        return {"bytes_in_use": b.value, "active": a.value, "done_pending": d.value}
