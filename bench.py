@@ -68,12 +68,39 @@ def cpu_baseline(argv, full_cycles):
                       "took %.2f s on 1 host core; scaled x%d" % (full_cycles, t, full_cycles)}
 
 
+def batched(prob, B, cycles_alone, adm_alone, graphs):
+    """B copies of the workload resident together, V-cycles in lockstep (one
+    launch per stage covers all B).  Each copy must reproduce the single solve
+    exactly."""
+    import ollipticid_amd as oa
+
+    e = oa.Engine(graphs=graphs)
+    for t in range(B):
+        assert e.admit(prob, t)
+    t0 = time.perf_counter()
+    while e.step():
+        pass
+    done = e.pop_done()
+    wall = time.perf_counter() - t0
+    assert len(done) == B
+    for tag, cycles, res, adm in done:
+        assert cycles == cycles_alone and adm[:2] == list(adm_alone), (tag, cycles, adm)
+    e.close()
+    return {"batch": B, "solves_per_s": B / wall, "ms_per_batch": 1e3 * wall,
+            "identical_to_single": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="issue every launch from the host instead of replaying a captured "
+                         "V-cycle graph (profilers that cannot trace graph launches)")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="also solve this many copies together in one engine batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the served stream shards across "
@@ -85,6 +112,7 @@ def main():
 
     prob = oa.problem_from_argv(BINARY)
     h = oa.Hierarchy(prob)
+    h.set_graphs(not args.no_graphs)
 
     def step():
         h.reset()
@@ -122,6 +150,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
+        "graphs": not args.no_graphs,
         "data": "synthetic",
         "config": {"workload": "equal-mass binary: NumBH=2, m=0.487209 at y=+-5.5, "
                                "P=+-(-0.0901099,-0.000703975,0), Robin, L=1000, N=65, "
@@ -142,6 +171,8 @@ def main():
                      "launches": launches, "avg_launch_us": 1e3 * avg_ms,
                      "bytes_per_launch": kbytes / launches},
     }
+    if args.batch > 1:
+        out["batched"] = batched(prob, args.batch, cycles, adm, not args.no_graphs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(BINARY, cycles)
     print(json.dumps(out), flush=True)
