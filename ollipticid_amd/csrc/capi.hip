@@ -203,9 +203,15 @@ int oll_get_field(oll_hier *h, int ibox, int field, double *host) {
   if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= NFIELDS)
     return fail(OLL_EINVAL, "box or field out of range");
   const DevBox &b = h->H.hbox[size_t(ibox)];
-  const size_t n = size_t(b.n[0]) * size_t(b.n[1]) * size_t(b.n[2]);
-  CAPI_CHECK(hipMemcpyAsync(host, b.f[field], n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> st(size_t(b.size));
+  CAPI_CHECK(hipMemcpyAsync(st.data(), b.f[field], st.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, h->stream));
   CAPI_CHECK(hipStreamSynchronize(h->stream));
+  // colour-split rows -> plain lexicographic order (i fastest)
+  size_t q = 0;
+  for (int k = 0; k < b.n[2]; ++k)
+    for (int j = 0; j < b.n[1]; ++j)
+      for (int i = 0; i < b.n[0]; ++i) host[q++] = st[size_t(pidx(b, i, j, k))];
   return OLL_OK;
 }
 
@@ -214,8 +220,13 @@ int oll_set_field(oll_hier *h, int ibox, int field, const double *host) {
   if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= NFIELDS)
     return fail(OLL_EINVAL, "box or field out of range");
   const DevBox &b = h->H.hbox[size_t(ibox)];
-  const size_t n = size_t(b.n[0]) * size_t(b.n[1]) * size_t(b.n[2]);
-  CAPI_CHECK(hipMemcpyAsync(b.f[field], host, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  std::vector<double> st(size_t(b.size), 0.0);
+  size_t q = 0;
+  for (int k = 0; k < b.n[2]; ++k)
+    for (int j = 0; j < b.n[1]; ++j)
+      for (int i = 0; i < b.n[0]; ++i) st[size_t(pidx(b, i, j, k))] = host[q++];
+  CAPI_CHECK(hipMemcpyAsync(b.f[field], st.data(), st.size() * sizeof(double),
+                            hipMemcpyHostToDevice, h->stream));
   CAPI_CHECK(hipStreamSynchronize(h->stream));
   return OLL_OK;
 }

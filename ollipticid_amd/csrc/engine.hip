@@ -125,7 +125,10 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err) {
   Hierarchy g;
   if ((rc = geometry(p, g, err))) return rc;
   size_t pts = 0, tables = 0;
-  for (const Box &b : g.boxes) pts += b.npts();
+  for (const Box &b : g.boxes) {
+    const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
+    pts += size_t((layout_size(n, b.R) + 31) & ~31LL);
+  }
   for (int l = 1; l < g.levels; ++l)
     for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
       const Box &B = g.boxes[size_t(g.level_first[size_t(l)] + b)];
@@ -142,12 +145,16 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   H.prob = p;
   if ((rc = geometry(p, H.geo, err))) return rc;
   const Hierarchy &g = H.geo;
-  // slab: field-major, boxes back to back
+  // slab: field-major, boxes back to back (colour-split rows, kernels.hpp)
   long long total = 0;
   std::vector<long long> off;
   for (const Box &b : g.boxes) {
+    if (b.n[0] >= size_t(kMaxAxisPoints) || b.n[1] >= size_t(kMaxAxisPoints) ||
+        b.n[2] >= size_t(kMaxAxisPoints))
+      return err = "a box exceeds 4095 points per axis", OLL_EINVAL;
+    const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
     off.push_back(total);
-    total += (long long)b.npts();
+    total += (layout_size(n, b.R) + 31) & ~31LL;  // 256-byte aligned boxes
   }
   const size_t slab_bytes = size_t(total) * sizeof(double) * NFIELDS;
   hipError_t e = hipMalloc(&H.slab, slab_bytes);
@@ -173,6 +180,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.level = b.level;
     d.dulap = dulap_of(b.order, d.idx2);
     d.ipn = ipn;
+    set_layout(d);
     for (int f = 0; f < NFIELDS; ++f) d.f[f] = H.slab + size_t(f) * size_t(total) + size_t(off[ib]);
     H.hbox.push_back(d);
   }
@@ -517,7 +525,7 @@ struct Builder {
       groups[H->prob.eq].push_back(CoarseItem{H->dbox + b0, H->slot(Hier::slot_pre(0)), H->iters,
                                               H->coarse_omega});
       auto &sz = sizes[H->prob.eq];
-      sz.first = std::max<int>(sz.first, int(npts(B)));
+      sz.first = std::max<int>(sz.first, int(B.size));
       sz.second = std::max<int>(sz.second, (B.n[0] - 2) * (B.n[1] - 2) * (B.n[2] - 2));
     }
     for (auto &g : groups) {
@@ -543,7 +551,7 @@ int build_vcycle_from(const std::vector<Hier *> &M, const std::vector<int> &star
       return err = "members of one schedule must share nu1 and nu2", OLL_EINVAL;
   for (Hier *H : M) {
     const DevBox &B = H->hbox[size_t(H->geo.level_first[0])];
-    const size_t lds = 2 * sizeof(double) * size_t(npts(B));
+    const size_t lds = 2 * sizeof(double) * size_t(B.size);
     if (lds + 2048 > size_t(kCoarseLdsLimit) ||
         (B.n[0] - 2) * (B.n[1] - 2) * (B.n[2] - 2) > 1024 * kCoarseOwn)
       return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;

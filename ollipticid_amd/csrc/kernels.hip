@@ -34,37 +34,58 @@ constexpr double kEps = DBL_EPSILON;
 // point operators
 // ------------------------------------------------------------------
 
+// offsets of the i-neighbours i+m (xp[m-1]) and i-m (xm[m-1]), m = 1..NC-1,
+// of a point in sub-row s of a colour-split row with rl slots per sub-row
+template <int NC>
+__device__ __forceinline__ void xoffs(int s, int rl, int *xp, int *xm) {
+#pragma unroll
+  for (int m = 1; m < NC; ++m) {
+    xp[m - 1] = (s + m < NC) ? m * rl : (m - NC) * rl + 1;
+    xm[m - 1] = (s >= m) ? -m * rl : (NC - m) * rl - 1;
+  }
+}
+
+// axis-star Laplacian at storage index p; i-neighbours at p + xp[m-1] and
+// p + xm[m-1], j- and k-neighbours m * sy and m * sz away
 template <int ORD>
-__device__ __forceinline__ double lap(const double *__restrict__ u, long long p, long long sy,
-                                      long long sz, double ix2, double iy2, double iz2) {
+__device__ __forceinline__ double lap(const double *__restrict__ u, long long p, const int *xp,
+                                      const int *xm, long long sy, long long sz, double ix2,
+                                      double iy2, double iz2) {
   const double u0 = u[p];
+#define UX(m) (u[p + xp[(m) - 1]] + u[p + xm[(m) - 1]])
+#define UY(m) (u[p + (m) * sy] + u[p - (m) * sy])
+#define UZ(m) (u[p + (m) * sz] + u[p - (m) * sz])
   if constexpr (ORD == 2) {
-    return (u[p + 1] - 2.0 * u0 + u[p - 1]) * ix2 + (u[p + sy] - 2.0 * u0 + u[p - sy]) * iy2 +
-           (u[p + sz] - 2.0 * u0 + u[p - sz]) * iz2;
+    return (UX(1) - 2.0 * u0) * ix2 + (UY(1) - 2.0 * u0) * iy2 + (UZ(1) - 2.0 * u0) * iz2;
   } else if constexpr (ORD == 4) {
-    const double ax = -(u[p + 2] + u[p - 2]) + 16.0 * (u[p + 1] + u[p - 1]) - 30.0 * u0;
-    const double ay = -(u[p + 2 * sy] + u[p - 2 * sy]) + 16.0 * (u[p + sy] + u[p - sy]) - 30.0 * u0;
-    const double az = -(u[p + 2 * sz] + u[p - 2 * sz]) + 16.0 * (u[p + sz] + u[p - sz]) - 30.0 * u0;
+    const double ax = -UX(2) + 16.0 * UX(1) - 30.0 * u0;
+    const double ay = -UY(2) + 16.0 * UY(1) - 30.0 * u0;
+    const double az = -UZ(2) + 16.0 * UZ(1) - 30.0 * u0;
     return (ax * ix2 + ay * iy2 + az * iz2) * (1.0 / 12.0);
   } else if constexpr (ORD == 6) {
-    const double ax = 2.0 * (u[p + 3] + u[p - 3]) - 27.0 * (u[p + 2] + u[p - 2]) +
-                      270.0 * (u[p + 1] + u[p - 1]) - 490.0 * u0;
-    const double ay = 2.0 * (u[p + 3 * sy] + u[p - 3 * sy]) - 27.0 * (u[p + 2 * sy] + u[p - 2 * sy]) +
-                      270.0 * (u[p + sy] + u[p - sy]) - 490.0 * u0;
-    const double az = 2.0 * (u[p + 3 * sz] + u[p - 3 * sz]) - 27.0 * (u[p + 2 * sz] + u[p - 2 * sz]) +
-                      270.0 * (u[p + sz] + u[p - sz]) - 490.0 * u0;
+    const double ax = 2.0 * UX(3) - 27.0 * UX(2) + 270.0 * UX(1) - 490.0 * u0;
+    const double ay = 2.0 * UY(3) - 27.0 * UY(2) + 270.0 * UY(1) - 490.0 * u0;
+    const double az = 2.0 * UZ(3) - 27.0 * UZ(2) + 270.0 * UZ(1) - 490.0 * u0;
     return (ax * ix2 + ay * iy2 + az * iz2) * (1.0 / 180.0);
   } else {
-    const double ax = -9.0 * (u[p + 4] + u[p - 4]) + 128.0 * (u[p + 3] + u[p - 3]) -
-                      1008.0 * (u[p + 2] + u[p - 2]) + 8064.0 * (u[p + 1] + u[p - 1]) - 14350.0 * u0;
-    const double ay = -9.0 * (u[p + 4 * sy] + u[p - 4 * sy]) + 128.0 * (u[p + 3 * sy] + u[p - 3 * sy]) -
-                      1008.0 * (u[p + 2 * sy] + u[p - 2 * sy]) + 8064.0 * (u[p + sy] + u[p - sy]) -
-                      14350.0 * u0;
-    const double az = -9.0 * (u[p + 4 * sz] + u[p - 4 * sz]) + 128.0 * (u[p + 3 * sz] + u[p - 3 * sz]) -
-                      1008.0 * (u[p + 2 * sz] + u[p - 2 * sz]) + 8064.0 * (u[p + sz] + u[p - sz]) -
-                      14350.0 * u0;
+    const double ax = -9.0 * UX(4) + 128.0 * UX(3) - 1008.0 * UX(2) + 8064.0 * UX(1) - 14350.0 * u0;
+    const double ay = -9.0 * UY(4) + 128.0 * UY(3) - 1008.0 * UY(2) + 8064.0 * UY(1) - 14350.0 * u0;
+    const double az = -9.0 * UZ(4) + 128.0 * UZ(3) - 1008.0 * UZ(2) + 8064.0 * UZ(1) - 14350.0 * u0;
     return (ax * ix2 + ay * iy2 + az * iz2) * (1.0 / 5040.0);
   }
+#undef UX
+#undef UY
+#undef UZ
+}
+
+// Laplacian of a box at (i, j, k), storage index p
+template <int ORD>
+__device__ __forceinline__ double lap_at(const DevBox &B, const double *__restrict__ u, long long p,
+                                         int i) {
+  constexpr int NC = ORD / 2 + 1;
+  int xp[NC - 1], xm[NC - 1];
+  xoffs<NC>(i % NC, B.rl, xp, xm);
+  return lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
 }
 
 // N(u) = Lap u + b (c+u)^-7 [+ d (c+u) + e (c+u)^5] and dN/du_p
@@ -176,7 +197,9 @@ __device__ __forceinline__ void point_ijk(const DevBox &B, long long q, int &i, 
 // ------------------------------------------------------------------
 // smoother: one colour of one Newton-Gauss-Seidel sweep on every listed
 // box.  Thread g owns colour point t = g mod per_row of global interior row
-// g / per_row: i = i_first(row) + t (R+1).
+// g / per_row: i = i_first(row) + t (R+1), which in the colour-split row is
+// slot i_first / (R+1) + t of sub-row i_first mod (R+1), so consecutive
+// threads touch consecutive elements of every field.
 // ------------------------------------------------------------------
 template <int ORD, int EQ>
 __global__ __launch_bounds__(256) void k_smooth(const RowItem *__restrict__ items, int nitems,
@@ -200,10 +223,12 @@ __global__ __launch_bounds__(256) void k_smooth(const RowItem *__restrict__ item
     const int first = R + ((colour - j - k - R) % NC + 2 * NC) % NC;
     const int i = first + t * NC;
     if (i < B.n[0] - R) {
-      const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-      const long long p = i + sy * j + sz * k;
+      const int sub = first % NC;
+      const long long p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + first / NC + t;
+      int xp[NC - 1], xm[NC - 1];
+      xoffs<NC>(sub, B.rl, xp, xm);
       double *__restrict__ u = B.f[F_U];
-      const double lu = lap<ORD>(u, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
+      const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
       const double cu = B.f[F_C][p] + u[p];
       double N, dN;
       nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], cu, EQ ? B.f[F_D][p] : 0.0,
@@ -218,13 +243,18 @@ __global__ __launch_bounds__(256) void k_smooth(const RowItem *__restrict__ item
   item_norm_update(row < total_rows ? items[it].norm : nullptr, it, rmax);
 }
 
-// residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u)
+// residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u).
+// blockIdx.y = sub-row + NC * (64-slot chunk); each wave covers 64
+// consecutive slots of one sub-row of one row.
 template <int ORD, int EQ>
 __global__ __launch_bounds__(256) void k_residual(const RowItem *__restrict__ items, int nitems,
                                                    unsigned total_rows, int write_defect) {
   constexpr int R = ORD / 2;
+  constexpr int NC = R + 1;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const unsigned row = blockIdx.x * 4u + ty;
+  const int sub = int(blockIdx.y % NC);
+  const int t = int(blockIdx.y / NC) * 64 + tx;
   double rmax = 0.0;
   int it = 0;
   if (row < total_rows) {
@@ -234,12 +264,13 @@ __global__ __launch_bounds__(256) void k_residual(const RowItem *__restrict__ it
     const unsigned nyi = unsigned(B.n[1] - 2 * R);
     const int k = R + int(r / nyi);
     const int j = R + int(r % nyi);
-    const int i = R + blockIdx.y * 64 + tx;
-    if (i < B.n[0] - R) {
-      const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-      const long long p = i + sy * j + sz * k;
+    const int i = sub + t * NC;
+    if (i >= R && i < B.n[0] - R) {
+      const long long p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + t;
+      int xp[NC - 1], xm[NC - 1];
+      xoffs<NC>(sub, B.rl, xp, xm);
       const double *__restrict__ u = B.f[F_U];
-      const double lu = lap<ORD>(u, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
+      const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
       double N, dN;
       nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
                     EQ ? B.f[F_E][p] : 0.0, N, dN);
@@ -257,31 +288,41 @@ __global__ __launch_bounds__(256) void k_residual(const RowItem *__restrict__ it
 // Robin boundary r^2/n sum_d (1/x_d) d_d u + u = S on the ghost shell
 // ------------------------------------------------------------------
 
-// first derivative along an axis at index i (n points, stride s):
+// value of u at offset m along axis a from the point (i, j, k) whose
+// storage index is p
+__device__ __forceinline__ double axis_at(const DevBox &B, const double *__restrict__ u,
+                                          long long p, int a, int i, int m) {
+  if (a == 0) return u[p - ioff(B, i) + ioff(B, i + m)];
+  return u[p + (long long)m * (a == 1 ? (long long)B.rs : B.ps)];
+}
+
+// first derivative along axis a at (i, j, k) (ia = index along a):
 // dx2 (diff_2nd.cc:237-335) for order 2, dx4 (diff_4th.cc:6-89) otherwise.
 // coef = coefficient of u at the point itself.
-__device__ __forceinline__ double deriv(const double *__restrict__ u, long long p, long long s,
-                                        int i, int n, int order, double id, double &coef) {
-  if (order == 2) {
-    if (i == 0) {
+__device__ __forceinline__ double deriv(const DevBox &B, const double *__restrict__ u, long long p,
+                                        int a, int i, int ia, double &coef) {
+  const int n = B.n[a];
+  const double id = B.idx[a];
+#define UA(m) axis_at(B, u, p, a, i, (m))
+  if (B.order == 2) {
+    if (ia == 0) {
       coef = -3.0 * 0.5 * id;
-      return (-3.0 * u[p] + 4.0 * u[p + s] - u[p + 2 * s]) * 0.5 * id;
-    } else if (i == n - 1) {
+      return (-3.0 * u[p] + 4.0 * UA(1) - UA(2)) * 0.5 * id;
+    } else if (ia == n - 1) {
       coef = 3.0 * 0.5 * id;
-      return (3.0 * u[p] - 4.0 * u[p - s] + u[p - 2 * s]) * 0.5 * id;
+      return (3.0 * u[p] - 4.0 * UA(-1) + UA(-2)) * 0.5 * id;
     }
     coef = 0.0;
-    return (u[p + s] - u[p - s]) * 0.5 * id;
+    return (UA(1) - UA(-1)) * 0.5 * id;
   }
   const int nm = (n % 2 == 0) ? n / 2 : (n - 1) / 2;
-  if (i <= nm) {
+  if (ia <= nm) {
     coef = -25.0 * id / 12.0;
-    return -(25.0 * u[p] - 48.0 * u[p + s] + 36.0 * u[p + 2 * s] - 16.0 * u[p + 3 * s] +
-             3.0 * u[p + 4 * s]) * id / 12.0;
+    return -(25.0 * u[p] - 48.0 * UA(1) + 36.0 * UA(2) - 16.0 * UA(3) + 3.0 * UA(4)) * id / 12.0;
   }
   coef = 25.0 * id / 12.0;
-  return (25.0 * u[p] - 48.0 * u[p - s] + 36.0 * u[p - 2 * s] - 16.0 * u[p - 3 * s] +
-          3.0 * u[p - 4 * s]) * id / 12.0;
+  return (25.0 * u[p] - 48.0 * UA(-1) + 36.0 * UA(-2) - 16.0 * UA(-3) + 3.0 * UA(-4)) * id / 12.0;
+#undef UA
 }
 
 // box-local views of the fields a Robin stage touches; u may be in LDS
@@ -305,11 +346,9 @@ __device__ __forceinline__ BoxData box_data(const DevBox &B) {
 // each axis a whose index lies in the ghost shell.
 __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, int j, int k,
                                             int mask, int mode) {
-  const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  const long long p = i + sy * j + sz * k;
+  const long long p = pidx(B, i, j, k);
   double *u = D.u;
   const int ijk[3] = {i, j, k};
-  const long long st[3] = {1, sy, sz};
   double X[3];
   for (int a = 0; a < 3; ++a) X[a] = coord(B, a, ijk[a]);
   const double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
@@ -317,7 +356,7 @@ __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, i
   for (int a = 0; a < 3; ++a) {
     if (!(mask & (1 << a))) continue;
     double cf;
-    const double dv = deriv(u, p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf);
+    const double dv = deriv(B, u, p, a, i, ijk[a], cf);
     sF += dv / X[a];
     sdF += cf / X[a];
   }
@@ -450,11 +489,9 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int
   if (i < R || i >= B.n[0] - R) mask |= 1;
   if (j < R || j >= B.n[1] - R) mask |= 2;
   if (k < R || k >= B.n[2] - R) mask |= 4;
-  const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  const long long p = i + sy * j + sz * k;
+  const long long p = pidx(B, i, j, k);
   const double *__restrict__ u = B.f[F_U];
   const int ijk[3] = {i, j, k};
-  const long long st[3] = {1, sy, sz};
   double X[3];
   for (int a = 0; a < 3; ++a) X[a] = coord(B, a, ijk[a]);
   const double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
@@ -462,7 +499,7 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int
   for (int a = 0; a < 3; ++a) {
     if (!(mask & (1 << a))) continue;
     double cf;
-    sF += deriv(u, p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf) / X[a];
+    sF += deriv(B, u, p, a, i, ijk[a], cf) / X[a];
   }
   return r2 * B.ipn * sF + u[p];
 }
@@ -470,10 +507,6 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int
 // ------------------------------------------------------------------
 // transfers
 // ------------------------------------------------------------------
-
-__device__ __forceinline__ long long lin(const DevBox &S, int i, int j, int k) {
-  return i + (long long)S.n[0] * (j + (long long)S.n[1] * k);
-}
 
 // tensor Lagrange interpolation of f over box S (reference accumulation
 // order, interpolation.cc:257-296); the nodes of coincident axes are
@@ -493,13 +526,14 @@ __device__ double tensor_interp(const DevBox &S, const double *__restrict__ f, c
       w[a] = ax[a].w + (long long)t[a] * ax[a].k;
     }
   }
-  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
+  const long long sy = S.rs, sz = S.ps;
   double acc = 0.0;
   for (int I = 0; I < kk[0]; ++I) {
+    const int xo = ioff(S, s0[0] + I);
     double uj = 0.0;
     for (int J = 0; J < kk[1]; ++J) {
       double uk = 0.0;
-      const long long base = (s0[0] + I) + sy * (s0[1] + J) + sz * (long long)s0[2];
+      const long long base = xo + sy * (s0[1] + J) + sz * (long long)s0[2];
       for (int K = 0; K < kk[2]; ++K) uk += f[base + sz * K] * (w[2] ? w[2][K] : 1.0);
       uj += uk * (w[1] ? w[1][J] : 1.0);
     }
@@ -512,14 +546,16 @@ __device__ double tensor_interp(const DevBox &S, const double *__restrict__ f, c
 __device__ __forceinline__ double trilinear_corr(const DevBox &S, const DevAxis *ax, const int t[3]) {
   const double *__restrict__ u = S.f[F_U];
   const double *__restrict__ ww = S.f[F_WW];
-  const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
-  const long long b = ax[0].start[t[0]] + sy * ax[1].start[t[1]] + sz * ax[2].start[t[2]];
+  const long long sy = S.rs, sz = S.ps;
+  const int x0 = ax[0].start[t[0]];
+  const long long b = pidx(S, x0, ax[1].start[t[1]], ax[2].start[t[2]]);
+  const int x1 = ioff(S, x0 + 1) - ioff(S, x0);
   const double dxi = ax[0].w[2 * t[0]], dxf = ax[0].w[2 * t[0] + 1];
   const double dyi = ax[1].w[2 * t[1]], dyf = ax[1].w[2 * t[1] + 1];
   const double dzi = ax[2].w[2 * t[2]], dzf = ax[2].w[2 * t[2] + 1];
 #define V(o) (u[b + (o)] - ww[b + (o)])
-  return ((V(0) * dxi + V(1) * dxf) * dyi + (V(sy) * dxi + V(sy + 1) * dxf) * dyf) * dzi +
-         ((V(sz) * dxi + V(sz + 1) * dxf) * dyi + (V(sz + sy) * dxi + V(sz + sy + 1) * dxf) * dyf) *
+  return ((V(0) * dxi + V(x1) * dxf) * dyi + (V(sy) * dxi + V(sy + x1) * dxf) * dyf) * dzi +
+         ((V(sz) * dxi + V(sz + x1) * dxf) * dyi + (V(sz + sy) * dxi + V(sz + sy + x1) * dxf) * dyf) *
              dzf;
 #undef V
 }
@@ -547,7 +583,7 @@ __global__ __launch_bounds__(256) void k_restrict_u(const PtItem *__restrict__ i
     const DevXfer &X = I.src[x];
     unsigned fl[3];
     if (!(axis_flags(X, t, fl) & 1)) continue;
-    T.f[F_U][pt - I.p0] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
+    T.f[F_U][pidx(T, i, j, k)] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
     return;
   }
 }
@@ -566,9 +602,9 @@ __global__ __launch_bounds__(256) void k_restrict_defect(const PtItem *__restric
   if (pt >= total) return;
   const PtItem &I = items[find_pt_item(items, n, pt)];
   const DevBox &T = *I.box;
-  const long long p = pt - I.p0;
   int i, j, k;
-  point_ijk(T, p, i, j, k);
+  point_ijk(T, pt - I.p0, i, j, k);
+  const long long p = pidx(T, i, j, k);
   T.f[F_WW][p] = T.f[F_U][p];
   const bool inside = interior(T, i, j, k);
   if (!I.mg && !inside) return;
@@ -583,15 +619,18 @@ __global__ __launch_bounds__(256) void k_restrict_defect(const PtItem *__restric
     const DevBox &S = *X.src;
     const double *__restrict__ w = S.f[F_W];
     if (f & 8) {
-      const long long sy = S.n[0], sz = (long long)S.n[0] * S.n[1];
-      const long long c = lin(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      const long long sy = S.rs, sz = S.ps;
+      const int ci = X.ax[0].coinc[i];
+      const long long c = pidx(S, ci, X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      const int xo = ioff(S, ci);
+      const int xm = ioff(S, ci - 1) - xo, xp = ioff(S, ci + 1) - xo;
       double acc = 0.0;
       for (int dz = -1; dz <= 1; ++dz) {
         const double wz = dz == 0 ? 0.5 : 0.25;
         for (int dy = -1; dy <= 1; ++dy) {
           const double wy = dy == 0 ? 0.5 : 0.25;
           const long long r = c + sy * dy + sz * dz;
-          acc += wz * wy * (0.25 * w[r - 1] + 0.5 * w[r] + 0.25 * w[r + 1]);
+          acc += wz * wy * (0.25 * w[r + xm] + 0.5 * w[r] + 0.25 * w[r + xp]);
         }
       }
       wc = acc;
@@ -605,10 +644,9 @@ __global__ __launch_bounds__(256) void k_restrict_defect(const PtItem *__restric
   if (covered) T.f[F_W][p] = wc; else wc = T.f[F_W][p];
   double Rc;
   if (inside) {
-    const long long sy = T.n[0], sz = (long long)T.n[0] * T.n[1];
     const double *__restrict__ u = T.f[F_U];
     double dN;
-    nonlinear<EQ>(lap<ORD>(u, p, sy, sz, T.idx2[0], T.idx2[1], T.idx2[2]), T.dulap, T.f[F_B][p],
+    nonlinear<EQ>(lap_at<ORD>(T, u, p, i), T.dulap, T.f[F_B][p],
                   T.f[F_C][p] + u[p], EQ ? T.f[F_D][p] : 0.0, EQ ? T.f[F_E][p] : 0.0, Rc, dN);
   } else {
     Rc = robin_op_at(T, i, j, k);
@@ -624,9 +662,8 @@ __global__ __launch_bounds__(256) void k_prolong(const PtItem *__restrict__ item
   if (pt >= total) return;
   const PtItem &I = items[find_pt_item(items, n, pt)];
   const DevBox &T = *I.box;
-  const long long p = pt - I.p0;
   int i, j, k;
-  point_ijk(T, p, i, j, k);
+  point_ijk(T, pt - I.p0, i, j, k);
   if (!I.mg && !interior(T, i, j, k)) return;
   const int t[3] = {i, j, k};
   for (int x = I.nsrc - 1; x >= 0; --x) {
@@ -637,12 +674,12 @@ __global__ __launch_bounds__(256) void k_prolong(const PtItem *__restrict__ item
     const DevBox &S = *X.src;
     double v;
     if (f & 4) {
-      const long long c = lin(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      const long long c = pidx(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
       v = S.f[F_U][c] - S.f[F_WW][c];
     } else {
       v = trilinear_corr(S, X.ax, t);
     }
-    T.f[F_U][p] += v;
+    T.f[F_U][pidx(T, i, j, k)] += v;
     return;
   }
 }
@@ -698,7 +735,7 @@ __global__ __launch_bounds__(256) void k_ghost_fill(const PtItem *__restrict__ i
     unsigned fl[3];
     const unsigned f = axis_flags(X, t, fl);
     if (!(f & 3)) continue;
-    T.f[F_U][lin(T, i, j, k)] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
+    T.f[F_U][pidx(T, i, j, k)] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
     return;
   }
 }
@@ -722,30 +759,31 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(const CoarseItem *__restr
   const double omega = It.omega;
   const int R = B.R;  // 1: the coarsest level always uses order 2
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int np = n0 * n1 * n2;
+  const int np = int(B.size);  // the LDS copy keeps the colour-split layout
   double *su = lds;
   double *srho = lds + np;
   const int ni = n0 - 2 * R, nj = n1 - 2 * R, nk = n2 - 2 * R;
-  const long long sy = n0, sz = (long long)n0 * n1;
   double *__restrict__ ug = B.f[F_U];
   for (int q = threadIdx.x; q < np; q += blockDim.x) {
     su[q] = ug[q];
     srho[q] = B.f[F_RHO][q];
   }
   const int nint = ni * nj * nk;
-  int own_p[kCoarseOwn], own_c[kCoarseOwn];
+  int own_p[kCoarseOwn], own_c[kCoarseOwn], own_s[kCoarseOwn];
   double own_b[kCoarseOwn], own_cc[kCoarseOwn], own_d[kCoarseOwn], own_e[kCoarseOwn];
 #pragma unroll
   for (int m = 0; m < kCoarseOwn; ++m) {
     const int t = threadIdx.x + m * blockDim.x;
     own_p[m] = -1;
     own_c[m] = -1;
+    own_s[m] = 0;
     own_b[m] = own_cc[m] = own_d[m] = own_e[m] = 0.0;
     if (t < nint) {
       const int i = R + t % ni, j = R + (t / ni) % nj, k = R + t / (ni * nj);
-      const int p = i + n0 * (j + n1 * k);
+      const int p = int(pidx(B, i, j, k));
       own_p[m] = p;
       own_c[m] = (i + j + k) % (R + 1);
+      own_s[m] = i & 1;
       own_b[m] = B.f[F_B][p];
       own_cc[m] = B.f[F_C][p];
       if (EQ) {
@@ -775,7 +813,9 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(const CoarseItem *__restr
       for (int q = 0; q < kCoarseOwn; ++q) {
         if (own_c[q] != col) continue;
         const long long p = own_p[q];
-        const double lu = lap<2>(su, p, sy, sz, B.idx2[0], B.idx2[1], B.idx2[2]);
+        int xp[1], xm[1];
+        xoffs<2>(own_s[q], B.rl, xp, xm);
+        const double lu = lap<2>(su, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
         double N, dN;
         nonlinear<EQ>(lu, B.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
         const double res = N - srho[p];
@@ -902,16 +942,17 @@ __global__ __launch_bounds__(256) void k_coefficients(DevBox B, PunctureArgs pa,
   if (q >= np) return;
   int i, j, k;
   point_ijk(B, q, i, j, k);
+  const long long p = pidx(B, i, j, k);
   const double x = coord(B, 0, i), y = coord(B, 1, j), z = coord(B, 2, k);
-  B.f[F_B][q] = punctures_b(pa, x, y, z);
-  B.f[F_C][q] = punctures_c(pa, x, y, z);
+  B.f[F_B][p] = punctures_b(pa, x, y, z);
+  B.f[F_C][p] = punctures_c(pa, x, y, z);
   if (pa.eq == 1) {
     const double r = sqrt(x * x + y * y + z * z);
-    B.f[F_D][q] = sf_d(pa, r);
-    B.f[F_E][q] = sf_e(pa, r);
+    B.f[F_D][p] = sf_d(pa, r);
+    B.f[F_E][p] = sf_e(pa, r);
   }
-  B.f[F_RHO][q] = interior(B, i, j, k) ? 0.0 : robin_a;
-  B.f[F_U][q] = 0.0;
+  B.f[F_RHO][p] = interior(B, i, j, k) ? 0.0 : robin_a;
+  B.f[F_U][p] = 0.0;
 }
 
 // ------------------------------------------------------------------
@@ -938,16 +979,13 @@ __device__ int cell_of(const DevBox &B, int a, double x) {
 }
 
 __device__ double dU_at(const DevBox &B, const double ctr[3], int i, int j, int k) {
-  const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
-  const long long p = i + sy * j + sz * k;
+  const long long p = pidx(B, i, j, k);
   const int ijk[3] = {i, j, k};
-  const long long st[3] = {1, sy, sz};
   double X[3], g[3];
   for (int a = 0; a < 3; ++a) {
     X[a] = coord(B, a, ijk[a]) - ctr[a];
     double cf;
-    g[a] = deriv(B.f[F_C], p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf) +
-           deriv(B.f[F_U], p, st[a], ijk[a], B.n[a], B.order, B.idx[a], cf);
+    g[a] = deriv(B, B.f[F_C], p, a, i, ijk[a], cf) + deriv(B, B.f[F_U], p, a, i, ijk[a], cf);
   }
   const double ir = 1.0 / sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
   return ir * (X[0] * g[0] + X[1] * g[1] + X[2] * g[2]);
@@ -1011,9 +1049,9 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
     c[a] = cell_of(B, a, X[a]);
     if (!(fabs(X[a] - coord(B, a, c[a])) <= 3 * kEps)) coinc = false;
   }
-  const long long sy = B.n[0], sz = (long long)B.n[0] * B.n[1];
+  const long long sy = B.rs, sz = B.ps;
   if (coinc) {
-    out[q] = f[c[0] + sy * c[1] + sz * c[2]];
+    out[q] = f[pidx(B, c[0], c[1], c[2])];
     return;
   }
   double w[3][10];
@@ -1032,10 +1070,11 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
   }
   double acc = 0.0;
   for (int I = 0; I <= ord; ++I) {
+    const int xo = ioff(B, s[0] + I);
     double uj = 0.0;
     for (int J = 0; J <= ord; ++J) {
       double uk = 0.0;
-      for (int K = 0; K <= ord; ++K) uk += f[(s[0] + I) + sy * (s[1] + J) + sz * (s[2] + K)] * w[2][K];
+      for (int K = 0; K <= ord; ++K) uk += f[xo + sy * (s[1] + J) + sz * (s[2] + K)] * w[2][K];
       uj += uk * w[1][J];
     }
     acc += uj * w[0][I];
@@ -1069,8 +1108,11 @@ void smooth_impl(hipStream_t s, const RowItem *items, int n, int rows, int max_r
 template <int ORD, int EQ>
 void residual_impl(hipStream_t s, const RowItem *items, int n, int rows, int max_row_len,
                    int write_defect) {
+  constexpr int R = ORD / 2, NC = R + 1;
   if (rows == 0) return;
-  hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(ceil_div(rows, 4), ceil_div(max_row_len, 64)),
+  // max_row_len = interior points per row; slots per sub-row incl. ghosts
+  const int rl = (max_row_len + 2 * R + NC - 1) / NC;
+  hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(ceil_div(rows, 4), NC * ceil_div(rl, 64)),
                      dim3(256), 0, s, items, n, unsigned(rows), write_defect);
 }
 
@@ -1153,7 +1195,7 @@ void launch_ghost_fill(hipStream_t s, const PtItem *items, int n, long long tota
 
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
                          int eq, double eps, int max_iter) {
-  const size_t lds = 2 * sizeof(double) * size_t(max_pts);
+  const size_t lds = 2 * sizeof(double) * size_t(max_pts);  // max_pts: largest B.size
   if (lds + 2048 > size_t(kLdsLimit) || max_int > 1024 * kCoarseOwn) return false;
   if (n == 0) return true;
   if (eq)

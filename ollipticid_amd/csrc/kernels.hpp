@@ -1,10 +1,15 @@
 // Device descriptors and host-side launchers of the CDNA4 kernels.
 //
 // HBM layout: one slab per hierarchy (request).  Each field (u, b, c, d, e,
-// rho, r, v, w, ww) of a box is a contiguous float64 array in plain
-// lexicographic order (i fastest) including the ghost shell of width R; a
-// DevBox carries the base pointers of its fields, so one launch can work on
-// boxes of many hierarchies.  Launches read device-resident item lists.
+// rho, r, v, w, ww) of a box is a contiguous float64 array covering the box
+// including its ghost shell of width R.  Rows (fixed j, k) are stored
+// colour-split: a row holds nc = R+1 sub-rows of rl = ceil(n0/nc) slots and
+// point i sits in sub-row i mod nc at slot i / nc.  The points of one colour
+// (i+j+k) mod nc of a row are then contiguous, so a colour pass of the
+// smoother reads and writes unit-stride runs; neighbours along j and k are
+// rs and ps elements away, neighbours along i are in the adjacent sub-rows.
+// A DevBox carries the base pointers of its fields, so one launch can work
+// on boxes of many hierarchies.  Launches read device-resident item lists.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,8 +31,41 @@ struct DevBox {
   double idx2[3];    // (1/d)^2
   double dulap;      // diagonal of the Laplacian stencil
   double ipn;        // 1 / Robin fall-off power
+  int nc;            // colours = sub-rows per row (R+1)
+  int rl;            // slots per sub-row, ceil(n0 / nc)
+  int rs;            // row stride, nc * rl
+  unsigned ncm;      // ceil(65536 / nc): i / nc == (i * ncm) >> 16 for i < 4096
+  long long ps;      // plane stride, rs * n1
+  long long size;    // elements per field, ps * n2
   double *f[NFIELDS];
 };
+
+constexpr int kMaxAxisPoints = 4096;
+
+__host__ __device__ inline int split_div(const DevBox &B, int i) {
+  return int((unsigned(i) * B.ncm) >> 16);
+}
+// offset of point i within its row
+__host__ __device__ inline int ioff(const DevBox &B, int i) {
+  const int q = split_div(B, i);
+  return (i - q * B.nc) * B.rl + q;
+}
+__host__ __device__ inline long long pidx(const DevBox &B, int i, int j, int k) {
+  return (long long)k * B.ps + (long long)j * B.rs + ioff(B, i);
+}
+// fills the layout members from n, R
+inline void set_layout(DevBox &B) {
+  B.nc = B.R + 1;
+  B.rl = (B.n[0] + B.nc - 1) / B.nc;
+  B.rs = B.nc * B.rl;
+  B.ncm = (65536u + unsigned(B.nc) - 1u) / unsigned(B.nc);
+  B.ps = (long long)B.rs * B.n[1];
+  B.size = B.ps * B.n[2];
+}
+inline long long layout_size(const int n[3], int R) {
+  const int nc = R + 1;
+  return (long long)nc * ((n[0] + nc - 1) / nc) * n[1] * n[2];
+}
 
 // one axis of a transfer table (geometry.hpp AxisTable)
 struct DevAxis {
