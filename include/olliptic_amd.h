@@ -38,7 +38,13 @@ extern "C" {
 #define OLL_EQ_PUNCTURES 0
 #define OLL_EQ_PUNCTURES_SCALAR 1
 
-/* arithmetic of the correction cycles */
+/* arithmetic of the smoothing blocks.  FP64: the reference's Newton-Gauss-
+ * Seidel sweeps in float64.  MIXED: each block of nu sweeps computes the
+ * float64 defect rho - N(u) and dN/du once, relaxes the linearised
+ * correction equation nu times in float32, and adds the correction to u in
+ * float64; restriction, prolongation, Robin, the coarsest-level solve and
+ * the coefficients stay float64.  Both converge to the same discrete
+ * solution (the float64 defect vanishes there). */
 #define OLL_PREC_FP64 0
 #define OLL_PREC_MIXED 1
 
@@ -158,7 +164,8 @@ int oll_reset(oll_hier *h);
 /* Smoother launch timing: when on, every smoother launch is bracketed by HIP
  * events on the handle's stream; oll_kernel_timing returns the number of
  * launches, their summed device time and their algorithmic bytes (40 B per
- * updated point: u, b, c, rho read and u written; 56 B with d, e). */
+ * updated point: u, b, c, rho read and u written; 56 B with d, e; 16 B for
+ * the FP32 correction sweeps of OLL_PREC_MIXED: e, q, r read, e written). */
 int oll_set_kernel_timing(oll_hier *h, int on);
 int oll_kernel_timing(oll_hier *h, int *launches, double *total_ms, double *bytes);
 /* V-cycles of oll_solve are captured once as a HIP graph and replayed
@@ -179,12 +186,27 @@ int oll_engine_create(size_t hbm_budget, int use_graphs, oll_engine **out);
 void oll_engine_destroy(oll_engine *e);
 /* OLL_EBUSY when the request does not fit the remaining budget. */
 int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag);
+/* Moves a built hierarchy (oll_hier_create, or one handed out by
+ * oll_engine_pop_hier) into the engine as a new request, reset to the
+ * initial guess u = 0: its HBM stays resident, nothing is rebuilt.  h keeps
+ * no device memory afterwards but must still be destroyed by the caller. */
+int oll_engine_admit_hier(oll_engine *e, oll_hier *h, long long tag);
 int oll_engine_step(oll_engine *e, int *active_after);
 /* Pops one finished request: returns 1 and fills tag, result and the
  * puncture ADM masses (num_bh values), 0 when none is finished, <0 on error
  * (the request's own failure, e.g. OLL_ENAN).  Frees its HBM. */
 int oll_engine_pop_done(oll_engine *e, long long *tag, oll_result *res, double *adm);
+/* Like oll_engine_pop_done, but hands the solved hierarchy to the caller
+ * (oll_get_field, oll_adm_mass, ...) instead of freeing it; *out is then
+ * destroyed with oll_hier_destroy.  Its HBM leaves the engine's budget. */
+int oll_engine_pop_hier(oll_engine *e, long long *tag, oll_result *res, oll_hier **out);
 int oll_engine_stats(const oll_engine *e, size_t *bytes_in_use, int *active, int *done_pending);
+/* smoother launch timing over the engine's V-cycles, as oll_set_kernel_timing
+ * (while on, V-cycles are issued launch by launch instead of graph replays);
+ * bytes are algorithmic: 40 B (FP64) or 16 B (FP32: e, q, r read, e written)
+ * per updated point */
+int oll_engine_set_kernel_timing(oll_engine *e, int on);
+int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, double *bytes);
 
 #ifdef __cplusplus
 }

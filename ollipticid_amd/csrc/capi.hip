@@ -400,6 +400,7 @@ struct oll_engine {
   std::map<std::pair<int, int>, Schedule> sched;
   bool dirty = true;
   long long cycles_run = 0;
+  Timing timing;
 };
 
 extern "C" {
@@ -426,6 +427,7 @@ void oll_engine_destroy(oll_engine *e) {
   for (auto &kv : e->sched) release_schedule(kv.second);
   for (auto &m : e->active) hier_release(*m.H);
   for (auto &m : e->done) hier_release(*m.H);
+  for (hipEvent_t ev : e->timing.ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(e->stream);
   delete e;
 }
@@ -451,6 +453,27 @@ int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag) {
   return OLL_OK;
 }
 
+int oll_engine_admit_hier(oll_engine *e, oll_hier *h, long long tag) {
+  if (!e || !h) return fail(OLL_EINVAL, "null argument");
+  if (!h->H.slab) return fail(OLL_EINVAL, "empty hierarchy handle");
+  if (e->budget && e->in_use + h->H.bytes > e->budget) return fail(OLL_EBUSY, "HBM budget exhausted");
+  CAPI_CHECK(hipStreamSynchronize(h->stream));
+  release_schedule(h->vcycle);
+  release_schedule(h->scratch);
+  oll_engine::Member m;
+  m.H.reset(new (std::nothrow) Hier());
+  if (!m.H) return fail(OLL_ENOMEM, "host allocation failed");
+  m.tag = tag;
+  *m.H = std::move(h->H);
+  h->H = Hier();  // the handle keeps no device memory
+  std::string err;
+  const int rc = hier_reset(*m.H, e->stream, err);
+  e->in_use += m.H->bytes;
+  e->active.push_back(std::move(m));
+  e->dirty = true;
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
 int oll_engine_step(oll_engine *e, int *active_after) {
   if (!e) return fail(OLL_EINVAL, "null handle");
   std::string err;
@@ -465,7 +488,7 @@ int oll_engine_step(oll_engine *e, int *active_after) {
     e->dirty = false;
   }
   for (auto &kv : e->sched)
-    if ((rc = run_schedule(kv.second, e->stream, nullptr, e->graphs, err))) return fail(rc, err);
+    if ((rc = run_schedule(kv.second, e->stream, &e->timing, e->graphs, err))) return fail(rc, err);
   for (auto &m : e->active)
     if ((rc = read_norms(*m.H, e->stream, err))) return fail(rc, err);
   CAPI_CHECK(hipStreamSynchronize(e->stream));
@@ -508,6 +531,59 @@ int oll_engine_pop_done(oll_engine *e, long long *tag, oll_result *res, double *
   hier_release(H);
   if (H.status) return fail(H.status, "NAN check error in the smoother");
   return rc ? rc : 1;
+}
+
+int oll_engine_pop_hier(oll_engine *e, long long *tag, oll_result *res, oll_hier **out) {
+  if (!e || !out) return fail(OLL_EINVAL, "null argument");
+  *out = nullptr;
+  if (e->done.empty()) return 0;
+  oll_hier *h = new (std::nothrow) oll_hier();
+  if (!h) return fail(OLL_ENOMEM, "host allocation failed");
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(OLL_EDEVICE, "hipStreamCreate failed");
+  }
+  (void)hipEventCreate(&h->ev0);
+  (void)hipEventCreate(&h->ev1);
+  oll_engine::Member m = std::move(e->done.front());
+  e->done.pop_front();
+  if (tag) *tag = m.tag;
+  if (res) {
+    res->cycles = m.H->cycle;
+    res->residual = m.H->res;
+    res->hist_len = int(m.H->hist.size() / 2);
+  }
+  e->in_use -= std::min(e->in_use, m.H->bytes);
+  const int status = m.H->status;
+  h->H = std::move(*m.H);  // device memory changes owner; nothing is freed
+  *m.H = Hier();
+  *out = h;
+  if (status) return fail(status, "NAN check error in the smoother");
+  return 1;
+}
+
+int oll_engine_set_kernel_timing(oll_engine *e, int on) {
+  if (!e) return fail(OLL_EINVAL, "null handle");
+  CAPI_CHECK(hipStreamSynchronize(e->stream));
+  e->timing.on = on != 0;
+  e->timing.used = 0;
+  e->timing.bytes = 0;
+  return OLL_OK;
+}
+
+int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, double *bytes) {
+  if (!e) return fail(OLL_EINVAL, "null handle");
+  CAPI_CHECK(hipStreamSynchronize(e->stream));
+  double tot = 0;
+  for (size_t i = 0; i + 1 < e->timing.used; i += 2) {
+    float ms = 0;
+    CAPI_CHECK(hipEventElapsedTime(&ms, e->timing.ev[i], e->timing.ev[i + 1]));
+    tot += ms;
+  }
+  if (launches) *launches = int(e->timing.used / 2);
+  if (total_ms) *total_ms = tot;
+  if (bytes) *bytes = e->timing.bytes;
+  return OLL_OK;
 }
 
 int oll_engine_stats(const oll_engine *e, size_t *bytes_in_use, int *active, int *done_pending) {

@@ -54,7 +54,8 @@ int validate(const oll_problem &p, std::string &err) {
   if (p.eq == OLL_EQ_PUNCTURES)
     for (int i = 0; i < p.num_bh; ++i)
       if (p.bh[i].m <= 0) return err = "Wrong punture mass parameters", OLL_EINVAL;
-  if (p.precision != OLL_PREC_FP64) return err = "only OLL_PREC_FP64 is available", OLL_EINVAL;
+  if (p.precision != OLL_PREC_FP64 && p.precision != OLL_PREC_MIXED)
+    return err = "precision must be OLL_PREC_FP64 or OLL_PREC_MIXED", OLL_EINVAL;
   return OLL_OK;
 }
 
@@ -135,7 +136,8 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err) {
       const size_t nsrc = size_t(g.level_count[size_t(l - 1)]);
       tables += nsrc * (B.n[0] + B.n[1] + B.n[2]) * (9 + 8 * size_t(p.order + 2)) * 2;
     }
-  bytes = pts * sizeof(double) * NFIELDS + tables + (64 << 10);
+  bytes = pts * (sizeof(double) * NFIELDS + (p.precision == OLL_PREC_MIXED ? sizeof(float) * NFIELDS32 : 0)) +
+          tables + (64 << 10);
   return OLL_OK;
 }
 
@@ -164,6 +166,18 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   }
   H.bytes += slab_bytes;
   ENG_CHECK(hipMemsetAsync(H.slab, 0, slab_bytes, s));
+  if (p.precision == OLL_PREC_MIXED) {
+    // e must read zero on the ghost shell: the whole slab starts zeroed and
+    // the kernels write interior points only
+    const size_t b32 = size_t(total) * sizeof(float) * NFIELDS32;
+    e = hipMalloc(&H.slab32, b32);
+    if (e != hipSuccess) {
+      err = std::string("hipMalloc slab32: ") + hipGetErrorString(e);
+      return OLL_ENOMEM;
+    }
+    H.bytes += b32;
+    ENG_CHECK(hipMemsetAsync(H.slab32, 0, b32, s));
+  }
   const double ipn = 1.0 / double(p.robin_n);
   for (size_t ib = 0; ib < g.boxes.size(); ++ib) {
     const Box &b = g.boxes[ib];
@@ -182,6 +196,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.ipn = ipn;
     set_layout(d);
     for (int f = 0; f < NFIELDS; ++f) d.f[f] = H.slab + size_t(f) * size_t(total) + size_t(off[ib]);
+    for (int f = 0; f < NFIELDS32; ++f)
+      d.g[f] = H.slab32 ? H.slab32 + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     H.hbox.push_back(d);
   }
   for (int l = 0; l < g.levels; ++l) {
@@ -197,9 +213,9 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
         for (int j = X.R; j < X.n[1] - X.R; ++j)
           for (int i = X.R; i < X.n[0] - X.R; ++i) cnt[size_t((i + j + k) % nc)] += 1.0;
     }
-    // the smoother maps threads to points with 24-bit exact float division
-    if ((long long)rows * ((maxrow + nc - 1) / nc) >= (1LL << 24))
-      return err = "a level exceeds 2^24 points per colour", OLL_EINVAL;
+    // row kernels divide row indices with 24-bit exact float division
+    if ((long long)rows * nc >= (1LL << 24))
+      return err = "a level exceeds 2^24 interior rows", OLL_EINVAL;
     H.colour_pts.push_back(cnt);
   }
   if ((rc = upload(&H.dbox, H.hbox, H.bytes, err))) return rc;
@@ -288,6 +304,7 @@ int hier_reset(Hier &H, hipStream_t s, std::string &err) {
 
 void hier_release(Hier &H) {
   (void)hipFree(H.slab);
+  (void)hipFree(H.slab32);
   (void)hipFree(H.dbox);
   (void)hipFree(H.ipool);
   (void)hipFree(H.bpool);
@@ -296,6 +313,7 @@ void hier_release(Hier &H) {
   (void)hipFree(H.norms);
   (void)hipFree(H.iters);
   H.slab = nullptr;
+  H.slab32 = nullptr;
   H.dbox = nullptr;
   H.ipool = nullptr;
   H.bpool = nullptr;
@@ -331,15 +349,29 @@ namespace {
 struct Builder {
   Schedule &S;
 
-  // row items of one level of one member appended to `items`
-  static void rows(const Hier &H, int level, unsigned long long *norm, std::vector<RowItem> &items,
-                   int &total, int &maxrow) {
+  // threads per row of a level: colour points per interior row (colour
+  // passes) or slots per colour sub-row (whole-row kernels)
+  static int per_row(const Hier &H, int level, bool subrows) {
+    const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
+    int m = 0;
+    for (int b = f; b < f + n; ++b) {
+      const DevBox &B = H.hbox[size_t(b)];
+      m = std::max(m, subrows ? B.rl : (B.n[0] - 2 * B.R + B.nc - 1) / B.nc);
+    }
+    return m;
+  }
+
+  // row blocks of one level of one member appended to `out`: the interior
+  // rows (or colour sub-rows) of each box in runs of kRowThreads / pr
+  static void row_blocks(const Hier &H, int level, unsigned long long *norm, bool subrows, int pr,
+                         std::vector<RowBlock> &out) {
+    const int rpb = std::max(1, kRowThreads / pr);
     const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
     for (int b = f; b < f + n; ++b) {
       const DevBox &B = H.hbox[size_t(b)];
-      items.push_back(RowItem{H.dbox + b, norm, total, interior_rows(B)});
-      total += interior_rows(B);
-      maxrow = std::max(maxrow, B.n[0] - 2 * B.R);
+      const int rows = interior_rows(B) * (subrows ? B.nc : 1);
+      for (int r0 = 0; r0 < rows; r0 += rpb)
+        out.push_back(RowBlock{H.dbox + b, norm, r0, std::min(rpb, rows - r0)});
     }
   }
 
@@ -357,59 +389,65 @@ struct Builder {
     S.launches.push_back(L);
   }
 
-  // one smoothing phase (nu sweeps) of level lv[h] of each member
-  void smooth(const std::vector<Hier *> &M, const std::vector<int> &lv, int nu, bool post) {
-    std::map<std::pair<int, int>, std::vector<int>> groups;  // (order, eq) -> members
+  // one smoothing phase (nu sweeps) of level lv[h] of each member; FP64
+  // members get nu * (R+1) colour passes, mixed-precision members a defect
+  // launch, nu * (R+1) FP32 colour passes and an update launch
+  void smooth(const std::vector<Hier *> &M, const std::vector<int> &lv, int nu, bool post,
+              int norm_slot = -1) {
+    if (nu <= 0) return;
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;  // (order, eq, mixed) -> members
     for (size_t h = 0; h < M.size(); ++h)
-      if (lv[h] >= 0) groups[{M[h]->level_order(lv[h]), M[h]->prob.eq}].push_back(int(h));
+      if (lv[h] >= 0)
+        groups[{M[h]->level_order(lv[h]), M[h]->prob.eq, M[h]->mixed() ? 1 : 0}].push_back(int(h));
     for (auto &g : groups) {
-      const int order = g.first.first, nc = order / 2 + 1;
-      // members are packed into launches of < 2^24 threads (the kernel's
-      // exact float division of the thread index)
-      struct Chunk {
-        std::vector<RowItem> items;
-        int total = 0, maxrow = 0;
-        double bytes[5] = {0, 0, 0, 0, 0};
+      const int order = std::get<0>(g.first), eq = std::get<1>(g.first);
+      const bool mixed = std::get<2>(g.first) != 0;
+      const int nc = order / 2 + 1;
+      auto slot = [&](const Hier &H, int l) {
+        return H.slot(norm_slot >= 0 ? norm_slot : post ? Hier::slot_post(l) : Hier::slot_pre(l));
       };
-      std::vector<Chunk> chunks(1);
+      int pc = 0, ps = 0;  // threads per row: colour passes, whole-row kernels
+      for (int h : g.second) {
+        pc = std::max(pc, per_row(*M[size_t(h)], lv[size_t(h)], false));
+        ps = std::max(ps, per_row(*M[size_t(h)], lv[size_t(h)], true));
+      }
+      std::vector<RowBlock> cb, sb;
+      double bytes[5] = {0, 0, 0, 0, 0};
       for (int h : g.second) {
         const Hier &H = *M[size_t(h)];
         const int l = lv[size_t(h)];
-        std::vector<RowItem> mine;
-        int mtotal = 0, mmax = 0;
-        rows(H, l, H.slot(post ? Hier::slot_post(l) : Hier::slot_pre(l)), mine, mtotal, mmax);
-        Chunk *c = &chunks.back();
-        const int mx = std::max(c->maxrow, mmax);
-        if (!c->items.empty() &&
-            (long long)(c->total + mtotal) * ((mx + nc - 1) / nc) >= (1LL << 24)) {
-          chunks.emplace_back();
-          c = &chunks.back();
-        }
-        for (RowItem r : mine) {
-          r.row0 += c->total;
-          c->items.push_back(r);
-        }
-        c->total += mtotal;
-        c->maxrow = std::max(c->maxrow, mmax);
-        const double per = H.prob.eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : 40.0;
-        for (int q = 0; q < nc; ++q) c->bytes[q] += per * H.colour_pts[size_t(l)][size_t(q)];
+        row_blocks(H, l, mixed ? nullptr : slot(H, l), false, pc, cb);
+        if (mixed) row_blocks(H, l, slot(H, l), true, ps, sb);
+        // algorithmic bytes per updated point: FP64 reads u, b, c, rho
+        // (+ d, e) and writes u; FP32 reads e, q, r and writes e
+        const double per = mixed ? 16.0 : (eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : 40.0);
+        for (int q = 0; q < nc; ++q) bytes[q] += per * H.colour_pts[size_t(l)][size_t(q)];
       }
-      std::vector<size_t> offs;
-      for (const Chunk &c : chunks) offs.push_back(S.add(c.items));
+      const size_t coff = S.add(cb);
+      const size_t soff = mixed ? S.add(sb) : 0;
+      auto whole = [&](Launch::Kind kind) {
+        Launch L{kind};
+        L.order = order;
+        L.eq = eq;
+        L.off = soff;
+        L.n = int(sb.size());
+        L.max_row = ps;
+        S.launches.push_back(L);
+      };
+      if (mixed) whole(Launch::DEFECT32);
       for (int s = 0; s < nu; ++s)
-        for (int q = 0; q < nc; ++q)
-          for (size_t ci = 0; ci < chunks.size(); ++ci) {
-            Launch L{Launch::SMOOTH};
-            L.order = order;
-            L.eq = g.first.second;
-            L.colour = q;
-            L.off = offs[ci];
-            L.n = int(chunks[ci].items.size());
-            L.total = chunks[ci].total;
-            L.max_row = chunks[ci].maxrow;
-            L.bytes = chunks[ci].bytes[q];
-            S.launches.push_back(L);
-          }
+        for (int q = 0; q < nc; ++q) {
+          Launch L{mixed ? Launch::SMOOTH32 : Launch::SMOOTH};
+          L.order = order;
+          L.eq = eq;
+          L.colour = q;
+          L.off = coff;
+          L.n = int(cb.size());
+          L.max_row = pc;
+          L.bytes = bytes[q];
+          S.launches.push_back(L);
+        }
+      if (mixed) whole(Launch::UPDATE32);
     }
   }
 
@@ -419,20 +457,21 @@ struct Builder {
     for (size_t h = 0; h < M.size(); ++h)
       if (lv[h] >= 0) groups[{M[h]->level_order(lv[h]), M[h]->prob.eq}].push_back(int(h));
     for (auto &g : groups) {
-      std::vector<RowItem> items;
-      int total = 0, maxrow = 0;
+      int ps = 0;
+      for (int h : g.second) ps = std::max(ps, per_row(*M[size_t(h)], lv[size_t(h)], true));
+      std::vector<RowBlock> blocks;
       for (int h : g.second) {
         const Hier &H = *M[size_t(h)];
-        rows(H, lv[size_t(h)], with_norm ? H.slot(H.slot_scratch()) : nullptr, items, total, maxrow);
+        row_blocks(H, lv[size_t(h)], with_norm ? H.slot(H.slot_scratch()) : nullptr, true, ps,
+                   blocks);
       }
       Launch L{Launch::RESIDUAL};
       L.order = g.first.first;
       L.eq = g.first.second;
       L.mode = write_defect;
-      L.off = S.add(items);
-      L.n = int(items.size());
-      L.total = total;
-      L.max_row = maxrow;
+      L.off = S.add(blocks);
+      L.n = int(blocks.size());
+      L.max_row = ps;
       S.launches.push_back(L);
     }
   }
@@ -604,13 +643,7 @@ int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string 
   if (stage == STAGE_SMOOTH || stage == STAGE_RESIDUAL) b.zero(M);
   if (stage == STAGE_SMOOTH) {
     // the scratch slot collects the norm of these sweeps
-    const size_t before = S.launches.size();
-    b.smooth(M, lv, nu, false);
-    std::vector<RowItem> items;
-    int total = 0, maxrow = 0;
-    Builder::rows(H, level, H.slot(H.slot_scratch()), items, total, maxrow);
-    const size_t off = S.add(items);
-    for (size_t i = before; i < S.launches.size(); ++i) S.launches[i].off = off;
+    b.smooth(M, lv, nu, false, H.slot_scratch());
   } else if (stage == STAGE_RESIDUAL) {
     b.residual(M, lv, 0, true);
   } else if (stage == STAGE_ROBIN) {
@@ -650,7 +683,7 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
   for (const Launch &L : S.launches) {
     const void *items = S.dblob + L.off;
     hipEvent_t e1 = nullptr;
-    if (t && t->on && L.kind == Launch::SMOOTH) {
+    if (t && t->on && (L.kind == Launch::SMOOTH || L.kind == Launch::SMOOTH32)) {
       hipEvent_t e0 = timing_event(*t);
       e1 = timing_event(*t);
       if (e0) (void)hipEventRecord(e0, s);
@@ -663,12 +696,22 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         launch_robin(s, reinterpret_cast<const RobinItem *>(items), L.n, int(L.total), L.mode);
         break;
       case Launch::SMOOTH:
-        launch_smooth(s, reinterpret_cast<const RowItem *>(items), L.n, int(L.total), L.max_row,
-                      L.order, L.eq, L.colour);
+        launch_smooth(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq,
+                      L.colour);
+        break;
+      case Launch::DEFECT32:
+        launch_defect32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq);
+        break;
+      case Launch::SMOOTH32:
+        launch_smooth32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order,
+                        L.colour);
+        break;
+      case Launch::UPDATE32:
+        launch_update32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order);
         break;
       case Launch::RESIDUAL:
-        launch_residual(s, reinterpret_cast<const RowItem *>(items), L.n, int(L.total), L.max_row,
-                        L.order, L.eq, L.mode);
+        launch_residual(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq,
+                        L.mode);
         break;
       case Launch::RESTRICT_U:
         launch_restrict_u(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);

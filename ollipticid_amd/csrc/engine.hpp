@@ -28,6 +28,7 @@ struct Hier {
   std::vector<DevBox> hbox;      // host copies, device field pointers
   DevBox *dbox = nullptr;        // device copies
   double *slab = nullptr;
+  float *slab32 = nullptr;       // mixed precision only
   int *ipool = nullptr;
   unsigned char *bpool = nullptr;
   double *dpool = nullptr;
@@ -57,6 +58,7 @@ struct Hier {
 
   int top() const { return geo.levels - 1; }
   int level_order(int l) const { return hbox[size_t(geo.level_first[size_t(l)])].order; }
+  bool mixed() const { return prob.precision == OLL_PREC_MIXED; }
   unsigned long long *slot(int s) const { return norms + size_t(s) * kNormSubSlots; }
   static int slot_pre(int k) { return 2 * k; }
   static int slot_post(int k) { return 2 * k + 1; }
@@ -72,15 +74,18 @@ int hier_reset(Hier &H, hipStream_t s, std::string &err);
 int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err);
 
 struct Launch {
-  enum Kind { ZERO, ROBIN, SMOOTH, RESIDUAL, RESTRICT_U, RESTRICT_DEFECT, COARSE, PROLONG, GHOST };
+  enum Kind {
+    ZERO, ROBIN, SMOOTH, RESIDUAL, RESTRICT_U, RESTRICT_DEFECT, COARSE, PROLONG, GHOST,
+    DEFECT32, SMOOTH32, UPDATE32
+  };
   Kind kind;
   int order = 2, eq = 0, colour = 0, mode = 0;
   size_t off = 0;     // items in the schedule blob
   int n = 0;          // items
   long long total = 0;
-  int max_row = 0;
+  int max_row = 0;    // row kernels: threads per row
   int max_pts = 0, max_int = 0;
-  double bytes = 0;   // algorithmic bytes (smoother launches)
+  double bytes = 0;   // algorithmic bytes (smoother launches, FP64 or FP32)
 };
 
 struct Schedule {
@@ -102,6 +107,7 @@ struct Schedule {
   void clear();
 };
 
+// events around every smoother launch (SMOOTH or SMOOTH32) when on
 struct Timing {
   bool on = false;
   std::vector<hipEvent_t> ev;

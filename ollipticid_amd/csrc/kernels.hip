@@ -47,31 +47,30 @@ __device__ __forceinline__ void xoffs(int s, int rl, int *xp, int *xm) {
 
 // axis-star Laplacian at storage index p; i-neighbours at p + xp[m-1] and
 // p + xm[m-1], j- and k-neighbours m * sy and m * sz away
-template <int ORD>
-__device__ __forceinline__ double lap(const double *__restrict__ u, long long p, const int *xp,
-                                      const int *xm, long long sy, long long sz, double ix2,
-                                      double iy2, double iz2) {
-  const double u0 = u[p];
+template <int ORD, typename T = double>
+__device__ __forceinline__ T lap(const T *__restrict__ u, long long p, const int *xp,
+                                 const int *xm, long long sy, long long sz, T ix2, T iy2, T iz2) {
+  const T u0 = u[p];
 #define UX(m) (u[p + xp[(m) - 1]] + u[p + xm[(m) - 1]])
 #define UY(m) (u[p + (m) * sy] + u[p - (m) * sy])
 #define UZ(m) (u[p + (m) * sz] + u[p - (m) * sz])
   if constexpr (ORD == 2) {
-    return (UX(1) - 2.0 * u0) * ix2 + (UY(1) - 2.0 * u0) * iy2 + (UZ(1) - 2.0 * u0) * iz2;
+    return (UX(1) - T(2) * u0) * ix2 + (UY(1) - T(2) * u0) * iy2 + (UZ(1) - T(2) * u0) * iz2;
   } else if constexpr (ORD == 4) {
-    const double ax = -UX(2) + 16.0 * UX(1) - 30.0 * u0;
-    const double ay = -UY(2) + 16.0 * UY(1) - 30.0 * u0;
-    const double az = -UZ(2) + 16.0 * UZ(1) - 30.0 * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (1.0 / 12.0);
+    const T ax = -UX(2) + T(16) * UX(1) - T(30) * u0;
+    const T ay = -UY(2) + T(16) * UY(1) - T(30) * u0;
+    const T az = -UZ(2) + T(16) * UZ(1) - T(30) * u0;
+    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(12));
   } else if constexpr (ORD == 6) {
-    const double ax = 2.0 * UX(3) - 27.0 * UX(2) + 270.0 * UX(1) - 490.0 * u0;
-    const double ay = 2.0 * UY(3) - 27.0 * UY(2) + 270.0 * UY(1) - 490.0 * u0;
-    const double az = 2.0 * UZ(3) - 27.0 * UZ(2) + 270.0 * UZ(1) - 490.0 * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (1.0 / 180.0);
+    const T ax = T(2) * UX(3) - T(27) * UX(2) + T(270) * UX(1) - T(490) * u0;
+    const T ay = T(2) * UY(3) - T(27) * UY(2) + T(270) * UY(1) - T(490) * u0;
+    const T az = T(2) * UZ(3) - T(27) * UZ(2) + T(270) * UZ(1) - T(490) * u0;
+    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(180));
   } else {
-    const double ax = -9.0 * UX(4) + 128.0 * UX(3) - 1008.0 * UX(2) + 8064.0 * UX(1) - 14350.0 * u0;
-    const double ay = -9.0 * UY(4) + 128.0 * UY(3) - 1008.0 * UY(2) + 8064.0 * UY(1) - 14350.0 * u0;
-    const double az = -9.0 * UZ(4) + 128.0 * UZ(3) - 1008.0 * UZ(2) + 8064.0 * UZ(1) - 14350.0 * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (1.0 / 5040.0);
+    const T ax = T(-9) * UX(4) + T(128) * UX(3) - T(1008) * UX(2) + T(8064) * UX(1) - T(14350) * u0;
+    const T ay = T(-9) * UY(4) + T(128) * UY(3) - T(1008) * UY(2) + T(8064) * UY(1) - T(14350) * u0;
+    const T az = T(-9) * UZ(4) + T(128) * UZ(3) - T(1008) * UZ(2) + T(8064) * UZ(1) - T(14350) * u0;
+    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(5040));
   }
 #undef UX
 #undef UY
@@ -136,33 +135,12 @@ __device__ __forceinline__ void norm_update(unsigned long long *slot, double v) 
   if (bits > __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(s, bits);
 }
 
-// max over the lanes that share an item, one atomic per item per wave
-__device__ __forceinline__ void item_norm_update(unsigned long long *slot, int item, double v) {
-  const int item0 = __shfl(item, 0, 64);
-  const unsigned long long diff = __ballot(item != item0);
-  if (diff == 0ull) {
-    v = warp_max(v);
-    if ((threadIdx.x & 63) == 0 && v > 0.0 && slot) norm_update(slot, v);
-  } else if (v > 0.0 && slot) {
-    norm_update(slot, v);
-  }
-}
-
 // exact g / d for g < 2^24 via a float reciprocal and one correction step
 __device__ __forceinline__ unsigned fast_div(unsigned g, unsigned d, float inv_d) {
   unsigned q = unsigned(float(g) * inv_d);
   if (q * d > g) --q;
   else if ((q + 1) * d <= g) ++q;
   return q;
-}
-
-__device__ __forceinline__ int find_row_item(const RowItem *items, int n, int row) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].row0 <= row) lo = mid; else hi = mid - 1;
-  }
-  return lo;
 }
 
 __device__ __forceinline__ int find_pt_item(const PtItem *items, int n, long long pt) {
@@ -194,94 +172,194 @@ __device__ __forceinline__ void point_ijk(const DevBox &B, long long q, int &i, 
   i = int(r - unsigned(j) * n0);
 }
 
-// ------------------------------------------------------------------
-// smoother: one colour of one Newton-Gauss-Seidel sweep on every listed
-// box.  Thread g owns colour point t = g mod per_row of global interior row
-// g / per_row: i = i_first(row) + t (R+1), which in the colour-split row is
-// slot i_first / (R+1) + t of sub-row i_first mod (R+1), so consecutive
-// threads touch consecutive elements of every field.
-// ------------------------------------------------------------------
-template <int ORD, int EQ>
-__global__ __launch_bounds__(256) void k_smooth(const RowItem *__restrict__ items, int nitems,
-                                                 unsigned total_rows, unsigned per_row,
-                                                 float inv_per_row, int colour) {
-  constexpr int R = ORD / 2;
-  constexpr int NC = R + 1;
-  const unsigned g = blockIdx.x * 256u + threadIdx.x;
-  const unsigned row = fast_div(g, per_row, inv_per_row);
-  const int t = int(g - row * per_row);
-  double rmax = 0.0;
-  int it = 0;
-  if (row < total_rows) {
-    it = find_row_item(items, nitems, int(row));
-    const DevBox &B = *items[it].box;
-    const unsigned r = row - unsigned(items[it].row0);
-    const unsigned nyi = unsigned(B.n[1] - 2 * R);
-    const unsigned kq = fast_div(r, nyi, 1.0f / float(nyi));
-    const int k = R + int(kq);
-    const int j = R + int(r - kq * nyi);
-    const int first = R + ((colour - j - k - R) % NC + 2 * NC) % NC;
-    const int i = first + t * NC;
-    if (i < B.n[0] - R) {
-      const int sub = first % NC;
-      const long long p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + first / NC + t;
-      int xp[NC - 1], xm[NC - 1];
-      xoffs<NC>(sub, B.rl, xp, xm);
-      double *__restrict__ u = B.f[F_U];
-      const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
-      const double cu = B.f[F_C][p] + u[p];
-      double N, dN;
-      nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], cu, EQ ? B.f[F_D][p] : 0.0,
-                    EQ ? B.f[F_E][p] : 0.0, N, dN);
-      const double res = N - B.f[F_RHO][p];
-      const double un = u[p] - res / dN;
-      u[p] = un;
-      rmax = fabs(res);
-      if (!(un == un)) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
-    }
-  }
-  item_norm_update(row < total_rows ? items[it].norm : nullptr, it, rmax);
+// block-wide max of v (every thread must call), one norm update per block
+__device__ __forceinline__ void block_norm(unsigned long long *slot, double v) {
+  __shared__ double sh[kRowThreads / 64];
+  const double m = block_max(v, sh);
+  if (threadIdx.x == 0 && slot && m > 0.0) norm_update(slot, m);
 }
 
-// residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u).
-// blockIdx.y = sub-row + NC * (64-slot chunk); each wave covers 64
-// consecutive slots of one sub-row of one row.
+// interior row r of box B -> (j, k)
+template <int R>
+__device__ __forceinline__ void row_jk(const DevBox &B, unsigned r, int &j, int &k) {
+  const unsigned nyi = unsigned(B.n[1] - 2 * R);
+  const unsigned kq = fast_div(r, nyi, 1.0f / float(nyi));
+  k = R + int(kq);
+  j = R + int(r - kq * nyi);
+}
+
+// colour passes: the thread's point of colour `colour` in the block's rows:
+// i = i_first(row) + t (R+1), which in the colour-split row is slot
+// i_first / (R+1) + t of sub-row i_first mod (R+1).  Returns false when the
+// thread has no point.
+template <int R>
+__device__ __forceinline__ bool colour_point(const RowBlock &Bk, unsigned per_row, int colour,
+                                             long long &p, int &sub) {
+  constexpr int NC = R + 1;
+  const unsigned lr = fast_div(threadIdx.x, per_row, 1.0f / float(per_row));
+  const int t = int(threadIdx.x - lr * per_row);
+  if (lr >= unsigned(Bk.rows)) return false;
+  const DevBox &B = *Bk.box;
+  int j, k;
+  row_jk<R>(B, unsigned(Bk.row0) + lr, j, k);
+  const int first = R + ((colour - j - k - R) % NC + 2 * NC) % NC;
+  if (first + t * NC >= B.n[0] - R) return false;
+  sub = first % NC;
+  p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + first / NC + t;
+  return true;
+}
+
+// whole-row kernels: the thread's slot t of sub-row `sub` of an interior
+// row; false when the slot is a ghost or padding point
+template <int R>
+__device__ __forceinline__ bool row_slot(const RowBlock &Bk, unsigned per_row, long long &p,
+                                         int &sub) {
+  constexpr int NC = R + 1;
+  const unsigned lr = fast_div(threadIdx.x, per_row, 1.0f / float(per_row));
+  const int t = int(threadIdx.x - lr * per_row);
+  if (lr >= unsigned(Bk.rows)) return false;
+  const DevBox &B = *Bk.box;
+  const unsigned sr = unsigned(Bk.row0) + lr;
+  const unsigned row = sr / NC;
+  sub = int(sr - row * NC);
+  const int i = sub + t * NC;
+  if (i < R || i >= B.n[0] - R || t >= B.rl) return false;
+  int j, k;
+  row_jk<R>(B, row, j, k);
+  p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + t;
+  return true;
+}
+
+// ------------------------------------------------------------------
+// smoother: one colour of one Newton-Gauss-Seidel sweep on the rows of
+// every listed block
+// ------------------------------------------------------------------
 template <int ORD, int EQ>
-__global__ __launch_bounds__(256) void k_residual(const RowItem *__restrict__ items, int nitems,
-                                                   unsigned total_rows, int write_defect) {
+__global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
+                                                        unsigned per_row, int colour) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const unsigned row = blockIdx.x * 4u + ty;
-  const int sub = int(blockIdx.y % NC);
-  const int t = int(blockIdx.y / NC) * 64 + tx;
+  const RowBlock Bk = blocks[blockIdx.x];
   double rmax = 0.0;
-  int it = 0;
-  if (row < total_rows) {
-    it = find_row_item(items, nitems, int(row));
-    const DevBox &B = *items[it].box;
-    const unsigned r = row - unsigned(items[it].row0);
-    const unsigned nyi = unsigned(B.n[1] - 2 * R);
-    const int k = R + int(r / nyi);
-    const int j = R + int(r % nyi);
-    const int i = sub + t * NC;
-    if (i >= R && i < B.n[0] - R) {
-      const long long p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + t;
-      int xp[NC - 1], xm[NC - 1];
-      xoffs<NC>(sub, B.rl, xp, xm);
-      const double *__restrict__ u = B.f[F_U];
-      const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
-      double N, dN;
-      nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
-                    EQ ? B.f[F_E][p] : 0.0, N, dN);
-      if (write_defect)
-        B.f[F_W][p] = B.f[F_RHO][p] - N;
-      else
-        B.f[F_R][p] = N;
-      rmax = fabs(N);
-    }
+  long long p;
+  int sub;
+  if (colour_point<R>(Bk, per_row, colour, p, sub)) {
+    const DevBox &B = *Bk.box;
+    int xp[NC - 1], xm[NC - 1];
+    xoffs<NC>(sub, B.rl, xp, xm);
+    double *__restrict__ u = B.f[F_U];
+    const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
+    const double cu = B.f[F_C][p] + u[p];
+    double N, dN;
+    nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], cu, EQ ? B.f[F_D][p] : 0.0,
+                  EQ ? B.f[F_E][p] : 0.0, N, dN);
+    const double res = N - B.f[F_RHO][p];
+    const double un = u[p] - res / dN;
+    u[p] = un;
+    rmax = fabs(res);
+    if (!(un == un)) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
   }
-  item_norm_update(row < total_rows ? items[it].norm : nullptr, it, rmax);
+  block_norm(Bk.norm, rmax);
+}
+
+// residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u)
+template <int ORD, int EQ>
+__global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__restrict__ blocks,
+                                                          unsigned per_row, int write_defect) {
+  constexpr int R = ORD / 2;
+  constexpr int NC = R + 1;
+  const RowBlock Bk = blocks[blockIdx.x];
+  double rmax = 0.0;
+  long long p;
+  int sub;
+  if (row_slot<R>(Bk, per_row, p, sub)) {
+    const DevBox &B = *Bk.box;
+    int xp[NC - 1], xm[NC - 1];
+    xoffs<NC>(sub, B.rl, xp, xm);
+    const double *__restrict__ u = B.f[F_U];
+    const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
+    double N, dN;
+    nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
+                  EQ ? B.f[F_E][p] : 0.0, N, dN);
+    if (write_defect)
+      B.f[F_W][p] = B.f[F_RHO][p] - N;
+    else
+      B.f[F_R][p] = N;
+    rmax = fabs(N);
+  }
+  block_norm(Bk.norm, rmax);
+}
+
+// ------------------------------------------------------------------
+// mixed precision.  One smoothing block of nu sweeps becomes: the FP64
+// defect r = rho - N(u) and the linearised diagonal q = dN/du - dulap of
+// the current u (k_defect32, also the block's norm), nu multi-colour
+// Gauss-Seidel sweeps in FP32 on the correction equation
+//   Lap e + q e = r,   e = 0 on the ghost shell          (k_smooth32)
+// and the FP64 update u += e (k_update32).  At a converged u the defect is
+// zero, e stays zero and u is unchanged: the fixed point is the FP64 one.
+// All three use the residual's launch shape (64 consecutive slots of one
+// sub-row per wave) or the smoother's (one colour point per thread).
+// ------------------------------------------------------------------
+template <int ORD, int EQ>
+__global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__restrict__ blocks,
+                                                          unsigned per_row) {
+  constexpr int R = ORD / 2;
+  constexpr int NC = R + 1;
+  const RowBlock Bk = blocks[blockIdx.x];
+  double rmax = 0.0;
+  long long p;
+  int sub;
+  if (row_slot<R>(Bk, per_row, p, sub)) {
+    const DevBox &B = *Bk.box;
+    int xp[NC - 1], xm[NC - 1];
+    xoffs<NC>(sub, B.rl, xp, xm);
+    const double *__restrict__ u = B.f[F_U];
+    const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
+    double N, dN;
+    nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
+                  EQ ? B.f[F_E][p] : 0.0, N, dN);
+    const double res = B.f[F_RHO][p] - N;
+    B.g[G_R][p] = float(res);
+    B.g[G_Q][p] = float(dN - B.dulap);
+    B.g[G_E][p] = 0.0f;
+    rmax = fabs(res);
+    if (!(res == res)) rmax = __longlong_as_double(0x7ff0000000000000LL);
+  }
+  block_norm(Bk.norm, rmax);
+}
+
+template <int ORD>
+__global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__restrict__ blocks,
+                                                          unsigned per_row, int colour) {
+  constexpr int R = ORD / 2;
+  constexpr int NC = R + 1;
+  const RowBlock Bk = blocks[blockIdx.x];
+  long long p;
+  int sub;
+  if (!colour_point<R>(Bk, per_row, colour, p, sub)) return;
+  const DevBox &B = *Bk.box;
+  int xp[NC - 1], xm[NC - 1];
+  xoffs<NC>(sub, B.rl, xp, xm);
+  float *__restrict__ e = B.g[G_E];
+  const float le = lap<ORD, float>(e, p, xp, xm, B.rs, B.ps, float(B.idx2[0]), float(B.idx2[1]),
+                                   float(B.idx2[2]));
+  const float q = B.g[G_Q][p];
+  const float ep = e[p];
+  const float res = le + q * ep - B.g[G_R][p];
+  e[p] = ep - res / (float(B.dulap) + q);
+}
+
+template <int ORD>
+__global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__restrict__ blocks,
+                                                          unsigned per_row) {
+  constexpr int R = ORD / 2;
+  const RowBlock Bk = blocks[blockIdx.x];
+  long long p;
+  int sub;
+  if (!row_slot<R>(Bk, per_row, p, sub)) return;
+  const DevBox &B = *Bk.box;
+  B.f[F_U][p] += double(B.g[G_E][p]);
 }
 
 // ------------------------------------------------------------------
@@ -1096,24 +1174,38 @@ __global__ void k_zero_lists(unsigned long long *const *ptrs, int count) {
 inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1) / b); }
 
 template <int ORD, int EQ>
-void smooth_impl(hipStream_t s, const RowItem *items, int n, int rows, int max_row_len, int colour) {
-  constexpr int NC = ORD / 2 + 1;
-  const unsigned per_row = unsigned((max_row_len + NC - 1) / NC);
-  const long long threads = (long long)rows * per_row;
-  if (threads == 0) return;
-  hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(ceil_div(threads, 256)), dim3(256), 0, s, items, n,
-                     unsigned(rows), per_row, 1.0f / float(per_row), colour);
+void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+                     unsigned(per_row), colour);
 }
 
 template <int ORD, int EQ>
-void residual_impl(hipStream_t s, const RowItem *items, int n, int rows, int max_row_len,
-                   int write_defect) {
-  constexpr int R = ORD / 2, NC = R + 1;
-  if (rows == 0) return;
-  // max_row_len = interior points per row; slots per sub-row incl. ghosts
-  const int rl = (max_row_len + 2 * R + NC - 1) / NC;
-  hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(ceil_div(rows, 4), NC * ceil_div(rl, 64)),
-                     dim3(256), 0, s, items, n, unsigned(rows), write_defect);
+void residual_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int write_defect) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+                     unsigned(per_row), write_defect);
+}
+
+template <int ORD, int EQ>
+void defect32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_defect32<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+                     unsigned(per_row));
+}
+
+template <int ORD, int EQ>
+void smooth32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+                     unsigned(per_row), colour);
+}
+
+template <int ORD, int EQ>
+void update32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_update32<ORD>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+                     unsigned(per_row));
 }
 
 template <int ORD, int EQ>
@@ -1156,14 +1248,28 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs 
   hipLaunchKernelGGL(k_coefficients, dim3(ceil_div(np, 256)), dim3(256), 0, s, B, pa, robin_a);
 }
 
-void launch_smooth(hipStream_t s, const RowItem *items, int n, int total_rows, int max_row_len,
-                   int order, int eq, int colour) {
-  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, items, n, total_rows, max_row_len, colour);
+void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                   int eq, int colour) {
+  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour);
 }
 
-void launch_residual(hipStream_t s, const RowItem *items, int n, int total_rows, int max_row_len,
-                     int order, int eq, int write_defect) {
-  OLL_DISPATCH_ORDER(order, eq, residual_impl, s, items, n, total_rows, max_row_len, write_defect);
+void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                     int eq, int write_defect) {
+  OLL_DISPATCH_ORDER(order, eq, residual_impl, s, blocks, nblocks, per_row, write_defect);
+}
+
+void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                     int eq) {
+  OLL_DISPATCH_ORDER(order, eq, defect32_impl, s, blocks, nblocks, per_row);
+}
+
+void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                     int colour) {
+  OLL_DISPATCH_ORDER(order, 0, smooth32_impl, s, blocks, nblocks, per_row, colour);
+}
+
+void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order) {
+  OLL_DISPATCH_ORDER(order, 0, update32_impl, s, blocks, nblocks, per_row);
 }
 
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode) {

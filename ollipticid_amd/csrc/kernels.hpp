@@ -19,6 +19,9 @@
 namespace oll {
 
 enum Field { F_U = 0, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW, NFIELDS };
+// float32 fields of the mixed-precision smoother (same element layout):
+// correction e, linearised diagonal q = dN/du - dulap, defect r = rho - N(u)
+enum Field32 { G_E = 0, G_Q, G_R, NFIELDS32 };
 
 struct DevBox {
   int n[3];          // points per axis incl. ghosts
@@ -38,6 +41,7 @@ struct DevBox {
   long long ps;      // plane stride, rs * n1
   long long size;    // elements per field, ps * n2
   double *f[NFIELDS];
+  float *g[NFIELDS32];  // null unless the hierarchy runs mixed precision
 };
 
 constexpr int kMaxAxisPoints = 4096;
@@ -83,13 +87,19 @@ struct DevXfer {
   DevAxis ax[3];
 };
 
-// row-parallel launches (smoother, residual): item owns rows row0..row0+rows
-struct RowItem {
+// row-parallel launches: one workgroup per RowBlock, which owns `rows`
+// consecutive rows of one box starting at row0.  For the colour passes
+// (k_smooth, k_smooth32) a row is an interior row (j, k) and a thread owns
+// one colour point of it; for the whole-row kernels (residual, defect,
+// update) a row is one colour sub-row of an interior row and a thread owns
+// one slot of it, so a workgroup covers one contiguous run of memory.
+struct RowBlock {
   const DevBox *box;
-  unsigned long long *norm;
+  unsigned long long *norm;  // null: no norm
   int row0;
   int rows;
 };
+constexpr int kRowThreads = 256;  // threads per row workgroup
 
 // point-parallel launches (transfers): item owns target points p0..p0+np
 // (all points of the box, or its ghost-shell points for the ghost fill)
@@ -139,10 +149,20 @@ constexpr int kCoarseOwn = 8;     // interior points per thread in the coarse so
 
 void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa,
                              double robin_a);
-void launch_smooth(hipStream_t s, const RowItem *items, int n, int total_rows, int max_row_len,
-                   int order, int eq, int colour);
-void launch_residual(hipStream_t s, const RowItem *items, int n, int total_rows, int max_row_len,
-                     int order, int eq, int write_defect);
+// per_row: threads per row (colour points per interior row for the colour
+// passes, slots per sub-row for the whole-row kernels); the host sizes
+// RowBlock::rows to at most kRowThreads / per_row
+void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                   int eq, int colour);
+void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                     int eq, int write_defect);
+// mixed precision: FP64 defect -> FP32 (r, q, e = 0), FP32 colour pass of
+// the linearised correction equation, FP64 update u += e
+void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                     int eq);
+void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
+                     int colour);
+void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order);
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
 void launch_restrict_u(hipStream_t s, const PtItem *items, int n, long long total);
 void launch_restrict_defect(hipStream_t s, const PtItem *items, int n, long long total, int order,

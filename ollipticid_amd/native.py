@@ -83,9 +83,15 @@ SIGNATURES = {
     "oll_engine_create": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(_vp)]),
     "oll_engine_destroy": (None, [_vp]),
     "oll_engine_admit": (C.c_int, [_vp, _P, C.c_longlong]),
+    "oll_engine_admit_hier": (C.c_int, [_vp, _vp, C.c_longlong]),
     "oll_engine_step": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "oll_engine_pop_done": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(Result),
                                       C.POINTER(C.c_double)]),
+    "oll_engine_pop_hier": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(Result),
+                                      C.POINTER(_vp)]),
+    "oll_engine_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "oll_engine_kernel_timing": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]),
     "oll_engine_stats": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]),
     "oll_last_norms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
@@ -172,6 +178,10 @@ def problem_from_argv(argv):
             p.robin_a = float(v)
         elif tok == "-Bn":
             p.robin_n = int(float(v))
+        elif tok in ("-prec", "--precision"):
+            if v not in ("fp64", "mixed"):
+                raise ValueError("precision must be fp64 or mixed")
+            p.precision = PREC_MIXED if v == "mixed" else PREC_FP64
         elif tok in ("-psc", "-ps_case"):
             p.ps_case = int(float(v))
         elif tok in ("-r0", "--radius_sf_r0"):
@@ -225,11 +235,14 @@ def problem_footprint(p):
 class Hierarchy:
     """One box hierarchy resident in HBM (oll_hier)."""
 
-    def __init__(self, problem):
+    def __init__(self, problem, _handle=None):
         self.L = load()
         self.problem = problem
-        h = _vp()
-        _check(self.L.oll_hier_create(C.byref(problem), C.byref(h)))
+        if _handle is None:
+            h = _vp()
+            _check(self.L.oll_hier_create(C.byref(problem), C.byref(h)))
+        else:
+            h = _handle
         self.h = h
         self.nbox = self.L.oll_num_boxes(self.h)
         self.levels = self.L.oll_num_levels(self.h)
@@ -374,6 +387,17 @@ class Engine:
         _check(rc)
         return True
 
+    def admit_hier(self, hier, tag):
+        """Moves a resident Hierarchy into the engine as request `tag` (reset
+        to u = 0); False when the HBM budget is exhausted.  `hier` is left
+        empty and closed."""
+        rc = self.L.oll_engine_admit_hier(self.h, hier.h, tag)
+        if rc == EBUSY:
+            return False
+        _check(rc)
+        hier.close()
+        return True
+
     def step(self):
         n = C.c_int()
         _check(self.L.oll_engine_step(self.h, C.byref(n)))
@@ -393,6 +417,30 @@ class Engine:
                 out.append((tag.value, res.cycles, res.residual, rc))
             else:
                 out.append((tag.value, res.cycles, res.residual, list(adm)))
+
+    def pop_hier(self, problems=None):
+        """List of (tag, cycles, residual, Hierarchy) of the finished requests,
+        each hierarchy handed over with its fields (problems: optional
+        tag -> Problem map kept on the returned objects)."""
+        out = []
+        while True:
+            tag = C.c_longlong()
+            res = Result()
+            h = _vp()
+            rc = self.L.oll_engine_pop_hier(self.h, C.byref(tag), C.byref(res), C.byref(h))
+            if rc == 0:
+                return out
+            hier = Hierarchy((problems or {}).get(tag.value), _handle=h) if h.value else None
+            out.append((tag.value, res.cycles, res.residual, hier if rc > 0 else rc))
+
+    def set_kernel_timing(self, on):
+        _check(self.L.oll_engine_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        """(smoother launches, their summed device ms, algorithmic bytes)"""
+        n, ms, b = C.c_int(), C.c_double(), C.c_double()
+        _check(self.L.oll_engine_kernel_timing(self.h, C.byref(n), C.byref(ms), C.byref(b)))
+        return n.value, ms.value, b.value
 
     def stats(self):
         b, a, d = C.c_size_t(), C.c_int(), C.c_int()

@@ -1,6 +1,7 @@
-"""The batched engine: requests of different N, level count, order and
-equation solved together give, request by request, exactly what each gives
-alone; admission respects the HBM budget."""
+"""The batched engine: requests of different N, level count, order, equation
+and precision solved together give, request by request, exactly what each
+gives alone (every u value of every box, bit for bit), also when they arrive
+while others are mid-solve; admission respects the HBM budget."""
 import numpy as np
 import pytest
 
@@ -21,34 +22,91 @@ MIX = [
     PUNC + ["-NumBH", "3", "-bhmp1", "0.3", "-bhx1", "3", "-bhmp2", "0.4", "-bhy2", "-2.5",
             "-bhmp3", "0.35", "-bhz3", "2.2", "-bhpx3", "0.1", "-L", "96", "-N", "49", "-l", "5",
             "-O", "4", "-nu2", "3", "-s", "1e-10"],
+    # mixed precision member
+    PUNC + ["-NumBH", "2", "-bhmp1", "0.5", "-bhx1", "1.5", "-bhpy1", "0.1", "-bhmp2", "0.5",
+            "-bhx2", "-1.5", "-bhpy2", "-0.1", "-L", "40", "-N", "25", "-l", "3", "-O", "4",
+            "-nu2", "3", "-s", "1e-10", "-prec", "mixed"],
 ]
+
+
+def fields(h):
+    return [h.get_field(ib, oa.F_U) for ib in range(h.nbox)]
 
 
 def alone(argv):
     with oa.Hierarchy(oa.problem_from_argv(argv)) as h:
         cycles, hist, res = h.solve()
         adm, _ = h.adm_mass()
-    return cycles, res, adm
+        return cycles, res, adm, fields(h)
+
+
+def check_same(tag, ref, got):
+    c, r, a, u = ref
+    gc, gr, h = got
+    assert gc == c, (tag, gc, c)
+    assert gr == r, (tag, gr, r)
+    np.testing.assert_array_equal(h.adm_mass()[0], a)
+    for ib, (x, y) in enumerate(zip(fields(h), u)):
+        np.testing.assert_array_equal(x, y, err_msg="request %d box %d" % (tag, ib))
+
+
+def run_to_end(e, problems, got, arrivals=None):
+    """Steps the engine; arrivals maps step index -> list of tags admitted
+    just before that step."""
+    step = 0
+    while True:
+        for tag in (arrivals or {}).get(step, []):
+            assert e.admit(problems[tag], tag)
+        active = e.step()
+        for tag, cycles, res, h in e.pop_hier(problems):
+            got[tag] = (cycles, res, h)
+        step += 1
+        pending = [s for s in (arrivals or {}) if s >= step]
+        if active == 0 and not pending:
+            return
+        assert step < 400
 
 
 def test_mix_equals_alone():
     ref = [alone(a) for a in MIX]
-    print("alone:", ref, flush=True)
+    print("alone:", [(r[0], r[1]) for r in ref], flush=True)
+    problems = {t: oa.problem_from_argv(a) for t, a in enumerate(MIX)}
     e = oa.Engine()
-    for tag, a in enumerate(MIX):
-        assert e.admit(oa.problem_from_argv(a), tag)
+    for tag in problems:
+        assert e.admit(problems[tag], tag)
     got = {}
-    for _ in range(200):
-        if e.step() == 0:
-            break
-    for tag, cycles, res, adm in e.pop_done():
-        got[tag] = (cycles, res, adm)
+    run_to_end(e, problems, got)
     assert sorted(got) == list(range(len(MIX)))
-    for tag, (c, r, a) in enumerate(ref):
-        gc, gr, ga = got[tag]
-        assert gc == c, (tag, gc, c)
-        assert gr == r, (tag, gr, r)
-        np.testing.assert_array_equal(ga[:len(a)], a)
+    for tag in got:
+        check_same(tag, ref[tag], got[tag])
+
+
+def test_staggered_arrivals_equal_alone():
+    """Continuous batching: requests join while others are mid-solve."""
+    ref = [alone(a) for a in MIX]
+    problems = {t: oa.problem_from_argv(a) for t, a in enumerate(MIX)}
+    e = oa.Engine()
+    got = {}
+    run_to_end(e, problems, got, arrivals={0: [0, 3], 2: [1], 3: [4], 5: [2]})
+    assert sorted(got) == list(range(len(MIX)))
+    for tag in got:
+        check_same(tag, ref[tag], got[tag])
+
+
+def test_resident_readmission():
+    """A solved hierarchy moved back into the engine solves again from u = 0
+    to the same result, without being rebuilt."""
+    ref = alone(MIX[1])
+    p = oa.problem_from_argv(MIX[1])
+    e = oa.Engine()
+    h = oa.Hierarchy(p)
+    for rnd in range(2):
+        assert e.admit_hier(h, 7)
+        got = {}
+        run_to_end(e, {7: p}, got)
+        check_same(7, ref, got[7])
+        h = got[7][2]
+    h.close()
 
 
 def test_budget_admission():
