@@ -155,6 +155,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
         b.n[2] >= size_t(kMaxAxisPoints))
       return err = "a box exceeds 4095 points per axis", OLL_EINVAL;
     const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
+    if (layout_size(n, b.R) * (long long)sizeof(double) >= kMaxFieldBytes)
+      return err = "a box field exceeds 4 GiB", OLL_EINVAL;
     off.push_back(total);
     total += (layout_size(n, b.R) + 31) & ~31LL;  // 256-byte aligned boxes
   }
@@ -259,11 +261,11 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   if ((rc = upload(&H.dxfer, xf, H.bytes, err))) return rc;
 
   H.nslots = 2 * L + 2;
-  ENG_CHECK(hipMalloc(&H.norms, size_t(H.nslots) * kNormSubSlots * sizeof(unsigned long long)));
+  ENG_CHECK(hipMalloc(&H.norms, size_t(H.nslots) * kNormWords * sizeof(unsigned long long)));
   ENG_CHECK(hipMalloc(&H.iters, sizeof(int)));
   ENG_CHECK(hipMemsetAsync(H.iters, 0, sizeof(int), s));
-  H.bytes += size_t(H.nslots) * kNormSubSlots * sizeof(unsigned long long) + sizeof(int);
-  launch_zero_u64(s, H.norms, (long long)H.nslots * kNormSubSlots);
+  H.bytes += size_t(H.nslots) * kNormWords * sizeof(unsigned long long) + sizeof(int);
+  launch_zero_u64(s, H.norms, (long long)H.nslots * kNormWords);
   H.eps_coarse = 2 * DBL_EPSILON;
   H.eps_solv = p.tol < H.eps_coarse ? H.eps_coarse : p.tol;
   // over-relaxation of the coarsest-level loop, tuned on the 13^3-21^3
@@ -380,7 +382,7 @@ struct Builder {
     int count = 0;
     for (Hier *H : M) {
       ptrs.push_back(H->norms);
-      count = std::max(count, H->nslots * kNormSubSlots);
+      count = std::max(count, H->nslots * kNormWords);
     }
     Launch L{Launch::ZERO};
     L.off = S.add(ptrs);
@@ -765,7 +767,7 @@ int run_schedule(Schedule &S, hipStream_t s, Timing *t, bool graphs, std::string
 }
 
 int read_norms(Hier &H, hipStream_t s, std::string &err) {
-  H.raw.resize(size_t(H.nslots) * kNormSubSlots);
+  H.raw.resize(size_t(H.nslots) * kNormWords);
   ENG_CHECK(hipMemcpyAsync(H.raw.data(), H.norms, H.raw.size() * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, s));
   ENG_CHECK(hipMemcpyAsync(&H.raw_iters, H.iters, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -776,7 +778,8 @@ void decode_norms(Hier &H) {
   H.last_slots.assign(size_t(H.nslots), 0.0);
   for (int sl = 0; sl < H.nslots; ++sl) {
     unsigned long long m = 0;
-    for (int q = 0; q < kNormSubSlots; ++q) m = std::max(m, H.raw[size_t(sl) * kNormSubSlots + q]);
+    for (int q = 0; q < kNormSubSlots; ++q)
+      m = std::max(m, H.raw[size_t(sl) * kNormWords + size_t(q) * kNormStride]);
     double d;
     std::memcpy(&d, &m, sizeof d);
     H.last_slots[size_t(sl)] = d;

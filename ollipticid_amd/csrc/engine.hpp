@@ -39,7 +39,7 @@ struct Hier {
   };
   // [fine level k][target box within its level]
   std::vector<std::vector<XRef>> ru, rw, pv, gf;
-  unsigned long long *norms = nullptr;  // nslots * kNormSubSlots
+  unsigned long long *norms = nullptr;  // nslots * kNormWords
   int nslots = 0;
   int *iters = nullptr;
   PunctureArgs pa{};
@@ -59,7 +59,7 @@ struct Hier {
   int top() const { return geo.levels - 1; }
   int level_order(int l) const { return hbox[size_t(geo.level_first[size_t(l)])].order; }
   bool mixed() const { return prob.precision == OLL_PREC_MIXED; }
-  unsigned long long *slot(int s) const { return norms + size_t(s) * kNormSubSlots; }
+  unsigned long long *slot(int s) const { return norms + size_t(s) * kNormWords; }
   static int slot_pre(int k) { return 2 * k; }
   static int slot_post(int k) { return 2 * k + 1; }
   int slot_scratch() const { return nslots - 1; }

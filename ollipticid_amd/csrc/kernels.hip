@@ -125,14 +125,12 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return r;
 }
 
-// Norm slots: a logical norm is kNormSubSlots u64 words; blocks spread
-// their atomics over them (the host takes the max) and only issue an atomic
-// when it raises the value read.  Non-negative doubles order like their
-// bit patterns.
+// Norm slots: non-negative doubles order like their bit patterns, so a max
+// is an unsigned atomic max.  The atomic returns nothing, so the issuing
+// wave does not wait for it.
 __device__ __forceinline__ void norm_update(unsigned long long *slot, double v) {
-  unsigned long long *s = slot + (blockIdx.x & (kNormSubSlots - 1));
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-  if (bits > __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(s, bits);
+  unsigned long long *s = slot + (blockIdx.x & (kNormSubSlots - 1)) * kNormStride;
+  atomicMax(s, (unsigned long long)__double_as_longlong(v));
 }
 
 // exact g / d for g < 2^24 via a float reciprocal and one correction step
@@ -172,53 +170,144 @@ __device__ __forceinline__ void point_ijk(const DevBox &B, long long q, int &i, 
   i = int(r - unsigned(j) * n0);
 }
 
-// block-wide max of v (every thread must call), one norm update per block
-__device__ __forceinline__ void block_norm(unsigned long long *slot, double v) {
-  __shared__ double sh[kRowThreads / 64];
-  const double m = block_max(v, sh);
-  if (threadIdx.x == 0 && slot && m > 0.0) norm_update(slot, m);
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remapping so that XCD x runs one contiguous run of logical blocks
+// keeps the rows and planes a stencil shares inside one L2.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb / kXcds, rem = nb % kXcds;
+  const unsigned x = b % kXcds, idx = b / kXcds;
+  return x * q + (x < rem ? x : rem) + idx;
 }
 
-// interior row r of box B -> (j, k)
+// Row kernels read their box descriptor through the constant address space
+// (scalar loads: kernels never write descriptors) and address fields with
+// 32-bit byte offsets from a uniform global base.
+typedef const __attribute__((address_space(4))) DevBox ConstBox;
+typedef __attribute__((address_space(1))) char gchar;
+
+__device__ __forceinline__ ConstBox &cbox(const DevBox *b) { return *(ConstBox *)b; }
+
+template <typename T>
+__device__ __forceinline__ T ld(const void *base, unsigned off) {
+  return *(const __attribute__((address_space(1))) T *)((const gchar *)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void st(void *base, unsigned off, T v) {
+  *(__attribute__((address_space(1))) T *)((gchar *)base + off) = v;
+}
+
+// axis-star Laplacian at byte offset o: i-neighbours at o + xp[m-1] and
+// o + xm[m-1] (bytes), j- and k-neighbours m * sy and m * sz bytes away
+template <int ORD, typename T>
+__device__ __forceinline__ T lapb(const void *u, unsigned o, const int *xp, const int *xm,
+                                  unsigned sy, unsigned sz, T ix2, T iy2, T iz2) {
+  const T u0 = ld<T>(u, o);
+#define UX(m) (ld<T>(u, o + xp[(m) - 1]) + ld<T>(u, o + xm[(m) - 1]))
+#define UY(m) (ld<T>(u, o + (m) * sy) + ld<T>(u, o - (m) * sy))
+#define UZ(m) (ld<T>(u, o + (m) * sz) + ld<T>(u, o - (m) * sz))
+  if constexpr (ORD == 2) {
+    return (UX(1) - T(2) * u0) * ix2 + (UY(1) - T(2) * u0) * iy2 + (UZ(1) - T(2) * u0) * iz2;
+  } else if constexpr (ORD == 4) {
+    const T ax = -UX(2) + T(16) * UX(1) - T(30) * u0;
+    const T ay = -UY(2) + T(16) * UY(1) - T(30) * u0;
+    const T az = -UZ(2) + T(16) * UZ(1) - T(30) * u0;
+    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(12));
+  } else if constexpr (ORD == 6) {
+    const T ax = T(2) * UX(3) - T(27) * UX(2) + T(270) * UX(1) - T(490) * u0;
+    const T ay = T(2) * UY(3) - T(27) * UY(2) + T(270) * UY(1) - T(490) * u0;
+    const T az = T(2) * UZ(3) - T(27) * UZ(2) + T(270) * UZ(1) - T(490) * u0;
+    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(180));
+  } else {
+    const T ax = T(-9) * UX(4) + T(128) * UX(3) - T(1008) * UX(2) + T(8064) * UX(1) - T(14350) * u0;
+    const T ay = T(-9) * UY(4) + T(128) * UY(3) - T(1008) * UY(2) + T(8064) * UY(1) - T(14350) * u0;
+    const T az = T(-9) * UZ(4) + T(128) * UZ(3) - T(1008) * UZ(2) + T(8064) * UZ(1) - T(14350) * u0;
+    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(5040));
+  }
+#undef UX
+#undef UY
+#undef UZ
+}
+
+// one DPP step on a double: lanes of rows in ROW_MASK read lane CTRL's value
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = int(b), hi = int(b >> 32);
+  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2));
+}
+
+// max over the 64 lanes of a wave (all lanes active); uniform result
+__device__ __forceinline__ double wave_max_dpp(double v) {
+  v = fmax(v, dpp_f64<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_f64<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_f64<0x141, 0xF>(v));  // row_half_mirror
+  v = fmax(v, dpp_f64<0x140, 0xF>(v));  // row_mirror
+  v = fmax(v, dpp_f64<0x142, 0xA>(v));  // row_bcast15
+  v = fmax(v, dpp_f64<0x143, 0xC>(v));  // row_bcast31: lane 63 holds the max
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), 63);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// block max of v (every thread must call), one norm update per block
+__device__ __forceinline__ void wave_norm(unsigned long long *slot, double v) {
+  __shared__ double sh[kRowThreads / 64];
+  const double m = wave_max_dpp(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0 && slot) {
+    double b = sh[0];
+#pragma unroll
+    for (int w = 1; w < kRowThreads / 64; ++w) b = fmax(b, sh[w]);
+    if (b > 0.0) norm_update(slot, b);
+  }
+}
+
+// the block's interior row r -> (j, k)
 template <int R>
-__device__ __forceinline__ void row_jk(const DevBox &B, unsigned r, int &j, int &k) {
+__device__ __forceinline__ void row_jk(ConstBox &B, unsigned r, int &j, int &k) {
   const unsigned nyi = unsigned(B.n[1] - 2 * R);
-  const unsigned kq = fast_div(r, nyi, 1.0f / float(nyi));
+  const unsigned kq = fast_div(r, nyi, B.inyi);
   k = R + int(kq);
   j = R + int(r - kq * nyi);
 }
 
-// colour passes: the thread's point of colour `colour` in the block's rows:
+// colour passes: the thread's point of colour `colour` in the block's rows,
 // i = i_first(row) + t (R+1), which in the colour-split row is slot
 // i_first / (R+1) + t of sub-row i_first mod (R+1).  Returns false when the
-// thread has no point.
+// thread has no point; p is the element index.
 template <int R>
-__device__ __forceinline__ bool colour_point(const RowBlock &Bk, unsigned per_row, int colour,
-                                             long long &p, int &sub) {
+__device__ __forceinline__ bool colour_point(const RowBlock &Bk, ConstBox &B, unsigned per_row,
+                                             float inv_per_row, int colour, unsigned &p,
+                                             int &sub) {
   constexpr int NC = R + 1;
-  const unsigned lr = fast_div(threadIdx.x, per_row, 1.0f / float(per_row));
+  const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   if (lr >= unsigned(Bk.rows)) return false;
-  const DevBox &B = *Bk.box;
   int j, k;
   row_jk<R>(B, unsigned(Bk.row0) + lr, j, k);
-  const int first = R + ((colour - j - k - R) % NC + 2 * NC) % NC;
+  // j + k + R < 3 kMaxAxisPoints keeps the dividend positive
+  const int first = R + int(unsigned(colour + NC * 3 * kMaxAxisPoints - j - k - R) % NC);
   if (first + t * NC >= B.n[0] - R) return false;
   sub = first % NC;
-  p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + first / NC + t;
+  p = unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) + unsigned(sub * B.rl + first / NC + t);
   return true;
 }
 
-// whole-row kernels: the thread's slot t of sub-row `sub` of an interior
-// row; false when the slot is a ghost or padding point
+// whole-row kernels: the thread's slot t of colour sub-row `sub` of an
+// interior row; false for ghost and padding slots
 template <int R>
-__device__ __forceinline__ bool row_slot(const RowBlock &Bk, unsigned per_row, long long &p,
-                                         int &sub) {
+__device__ __forceinline__ bool row_slot(const RowBlock &Bk, ConstBox &B, unsigned per_row,
+                                         float inv_per_row, unsigned &p, int &sub) {
   constexpr int NC = R + 1;
-  const unsigned lr = fast_div(threadIdx.x, per_row, 1.0f / float(per_row));
+  const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   if (lr >= unsigned(Bk.rows)) return false;
-  const DevBox &B = *Bk.box;
   const unsigned sr = unsigned(Bk.row0) + lr;
   const unsigned row = sr / NC;
   sub = int(sr - row * NC);
@@ -226,8 +315,19 @@ __device__ __forceinline__ bool row_slot(const RowBlock &Bk, unsigned per_row, l
   if (i < R || i >= B.n[0] - R || t >= B.rl) return false;
   int j, k;
   row_jk<R>(B, row, j, k);
-  p = (long long)k * B.ps + (long long)j * B.rs + sub * B.rl + t;
+  p = unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) + unsigned(sub * B.rl + t);
   return true;
+}
+
+// byte offsets of the i-neighbours of a point in sub-row `sub`
+template <int NC, int ES>
+__device__ __forceinline__ void xoffs_bytes(int sub, int rl, int *xp, int *xm) {
+  xoffs<NC>(sub, rl, xp, xm);
+#pragma unroll
+  for (int m = 0; m < NC - 1; ++m) {
+    xp[m] *= ES;
+    xm[m] *= ES;
+  }
 }
 
 // ------------------------------------------------------------------
@@ -236,58 +336,65 @@ __device__ __forceinline__ bool row_slot(const RowBlock &Bk, unsigned per_row, l
 // ------------------------------------------------------------------
 template <int ORD, int EQ>
 __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
-                                                        unsigned per_row, int colour) {
+                                                        unsigned per_row, float inv_per_row,
+                                                        int colour) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
-  const RowBlock Bk = blocks[blockIdx.x];
+  const RowBlock Bk = blocks[xcd_block()];
+  ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
-  long long p;
+  unsigned p;
   int sub;
-  if (colour_point<R>(Bk, per_row, colour, p, sub)) {
-    const DevBox &B = *Bk.box;
+  if (colour_point<R>(Bk, B, per_row, inv_per_row, colour, p, sub)) {
     int xp[NC - 1], xm[NC - 1];
-    xoffs<NC>(sub, B.rl, xp, xm);
-    double *__restrict__ u = B.f[F_U];
-    const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
-    const double cu = B.f[F_C][p] + u[p];
+    xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
+    const unsigned o = p * 8u;
+    double *u = B.f[F_U];
+    const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
+                                        B.idx2[0], B.idx2[1], B.idx2[2]);
+    const double u0 = ld<double>(u, o);
+    const double cu = ld<double>(B.f[F_C], o) + u0;
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], cu, EQ ? B.f[F_D][p] : 0.0,
-                  EQ ? B.f[F_E][p] : 0.0, N, dN);
-    const double res = N - B.f[F_RHO][p];
-    const double un = u[p] - res / dN;
-    u[p] = un;
+    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), cu, EQ ? ld<double>(B.f[F_D], o) : 0.0,
+                  EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+    const double res = N - ld<double>(B.f[F_RHO], o);
+    const double un = u0 - res / dN;
+    st<double>(u, o, un);
     rmax = fabs(res);
     if (!(un == un)) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
   }
-  block_norm(Bk.norm, rmax);
+  wave_norm(Bk.norm, rmax);
 }
 
 // residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u)
 template <int ORD, int EQ>
 __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__restrict__ blocks,
-                                                          unsigned per_row, int write_defect) {
+                                                          unsigned per_row, float inv_per_row,
+                                                          int write_defect) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
-  const RowBlock Bk = blocks[blockIdx.x];
+  const RowBlock Bk = blocks[xcd_block()];
+  ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
-  long long p;
+  unsigned p;
   int sub;
-  if (row_slot<R>(Bk, per_row, p, sub)) {
-    const DevBox &B = *Bk.box;
+  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, sub)) {
     int xp[NC - 1], xm[NC - 1];
-    xoffs<NC>(sub, B.rl, xp, xm);
-    const double *__restrict__ u = B.f[F_U];
-    const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
+    xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
+    const unsigned o = p * 8u;
+    const double *u = B.f[F_U];
+    const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
+                                        B.idx2[0], B.idx2[1], B.idx2[2]);
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
-                  EQ ? B.f[F_E][p] : 0.0, N, dN);
+    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + ld<double>(u, o),
+                  EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
     if (write_defect)
-      B.f[F_W][p] = B.f[F_RHO][p] - N;
+      st<double>(B.f[F_W], o, ld<double>(B.f[F_RHO], o) - N);
     else
-      B.f[F_R][p] = N;
+      st<double>(B.f[F_R], o, N);
     rmax = fabs(N);
   }
-  block_norm(Bk.norm, rmax);
+  wave_norm(Bk.norm, rmax);
 }
 
 // ------------------------------------------------------------------
@@ -298,68 +405,71 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
 //   Lap e + q e = r,   e = 0 on the ghost shell          (k_smooth32)
 // and the FP64 update u += e (k_update32).  At a converged u the defect is
 // zero, e stays zero and u is unchanged: the fixed point is the FP64 one.
-// All three use the residual's launch shape (64 consecutive slots of one
-// sub-row per wave) or the smoother's (one colour point per thread).
 // ------------------------------------------------------------------
 template <int ORD, int EQ>
 __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__restrict__ blocks,
-                                                          unsigned per_row) {
+                                                          unsigned per_row, float inv_per_row) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
-  const RowBlock Bk = blocks[blockIdx.x];
+  const RowBlock Bk = blocks[xcd_block()];
+  ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
-  long long p;
+  unsigned p;
   int sub;
-  if (row_slot<R>(Bk, per_row, p, sub)) {
-    const DevBox &B = *Bk.box;
+  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, sub)) {
     int xp[NC - 1], xm[NC - 1];
-    xoffs<NC>(sub, B.rl, xp, xm);
-    const double *__restrict__ u = B.f[F_U];
-    const double lu = lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
+    xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
+    const unsigned o = p * 8u, o4 = p * 4u;
+    const double *u = B.f[F_U];
+    const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
+                                        B.idx2[0], B.idx2[1], B.idx2[2]);
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, B.f[F_B][p], B.f[F_C][p] + u[p], EQ ? B.f[F_D][p] : 0.0,
-                  EQ ? B.f[F_E][p] : 0.0, N, dN);
-    const double res = B.f[F_RHO][p] - N;
-    B.g[G_R][p] = float(res);
-    B.g[G_Q][p] = float(dN - B.dulap);
-    B.g[G_E][p] = 0.0f;
+    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + ld<double>(u, o),
+                  EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+    const double res = ld<double>(B.f[F_RHO], o) - N;
+    st<float>(B.g[G_R], o4, float(res));
+    st<float>(B.g[G_Q], o4, float(dN - B.dulap));
+    st<float>(B.g[G_E], o4, 0.0f);
     rmax = fabs(res);
     if (!(res == res)) rmax = __longlong_as_double(0x7ff0000000000000LL);
   }
-  block_norm(Bk.norm, rmax);
+  wave_norm(Bk.norm, rmax);
 }
 
 template <int ORD>
 __global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__restrict__ blocks,
-                                                          unsigned per_row, int colour) {
+                                                          unsigned per_row, float inv_per_row,
+                                                          int colour) {
   constexpr int R = ORD / 2;
   constexpr int NC = R + 1;
-  const RowBlock Bk = blocks[blockIdx.x];
-  long long p;
+  const RowBlock Bk = blocks[xcd_block()];
+  ConstBox &B = cbox(Bk.box);
+  unsigned p;
   int sub;
-  if (!colour_point<R>(Bk, per_row, colour, p, sub)) return;
-  const DevBox &B = *Bk.box;
+  if (!colour_point<R>(Bk, B, per_row, inv_per_row, colour, p, sub)) return;
   int xp[NC - 1], xm[NC - 1];
-  xoffs<NC>(sub, B.rl, xp, xm);
-  float *__restrict__ e = B.g[G_E];
-  const float le = lap<ORD, float>(e, p, xp, xm, B.rs, B.ps, float(B.idx2[0]), float(B.idx2[1]),
-                                   float(B.idx2[2]));
-  const float q = B.g[G_Q][p];
-  const float ep = e[p];
-  const float res = le + q * ep - B.g[G_R][p];
-  e[p] = ep - res / (float(B.dulap) + q);
+  xoffs_bytes<NC, 4>(sub, B.rl, xp, xm);
+  const unsigned o = p * 4u;
+  float *e = B.g[G_E];
+  const float le = lapb<ORD, float>(e, o, xp, xm, unsigned(B.rs) * 4u, unsigned(B.ps) * 4u,
+                                    float(B.idx2[0]), float(B.idx2[1]), float(B.idx2[2]));
+  const float q = ld<float>(B.g[G_Q], o);
+  const float ep = ld<float>(e, o);
+  const float res = le + q * ep - ld<float>(B.g[G_R], o);
+  st<float>(e, o, ep - res / (float(B.dulap) + q));
 }
 
 template <int ORD>
 __global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__restrict__ blocks,
-                                                          unsigned per_row) {
+                                                          unsigned per_row, float inv_per_row) {
   constexpr int R = ORD / 2;
-  const RowBlock Bk = blocks[blockIdx.x];
-  long long p;
+  const RowBlock Bk = blocks[xcd_block()];
+  ConstBox &B = cbox(Bk.box);
+  unsigned p;
   int sub;
-  if (!row_slot<R>(Bk, per_row, p, sub)) return;
-  const DevBox &B = *Bk.box;
-  B.f[F_U][p] += double(B.g[G_E][p]);
+  if (!row_slot<R>(Bk, B, per_row, inv_per_row, p, sub)) return;
+  double *u = B.f[F_U];
+  st<double>(u, p * 8u, ld<double>(u, p * 8u) + double(ld<float>(B.g[G_E], p * 4u)));
 }
 
 // ------------------------------------------------------------------
@@ -1177,35 +1287,35 @@ template <int ORD, int EQ>
 void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour) {
   if (nb == 0) return;
   hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
-                     unsigned(per_row), colour);
+                     unsigned(per_row), 1.0f / float(per_row), colour);
 }
 
 template <int ORD, int EQ>
 void residual_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int write_defect) {
   if (nb == 0) return;
   hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
-                     unsigned(per_row), write_defect);
+                     unsigned(per_row), 1.0f / float(per_row), write_defect);
 }
 
 template <int ORD, int EQ>
 void defect32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
   if (nb == 0) return;
   hipLaunchKernelGGL((k_defect32<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
-                     unsigned(per_row));
+                     unsigned(per_row), 1.0f / float(per_row));
 }
 
 template <int ORD, int EQ>
 void smooth32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour) {
   if (nb == 0) return;
   hipLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
-                     unsigned(per_row), colour);
+                     unsigned(per_row), 1.0f / float(per_row), colour);
 }
 
 template <int ORD, int EQ>
 void update32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
   if (nb == 0) return;
   hipLaunchKernelGGL((k_update32<ORD>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
-                     unsigned(per_row));
+                     unsigned(per_row), 1.0f / float(per_row));
 }
 
 template <int ORD, int EQ>

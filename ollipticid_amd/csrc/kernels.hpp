@@ -40,6 +40,7 @@ struct DevBox {
   unsigned ncm;      // ceil(65536 / nc): i / nc == (i * ncm) >> 16 for i < 4096
   long long ps;      // plane stride, rs * n1
   long long size;    // elements per field, ps * n2
+  float inyi;        // 1 / (n1 - 2R): interior row -> (j, k) by float division
   double *f[NFIELDS];
   float *g[NFIELDS32];  // null unless the hierarchy runs mixed precision
 };
@@ -65,6 +66,7 @@ inline void set_layout(DevBox &B) {
   B.ncm = (65536u + unsigned(B.nc) - 1u) / unsigned(B.nc);
   B.ps = (long long)B.rs * B.n[1];
   B.size = B.ps * B.n[2];
+  B.inyi = 1.0f / float(B.n[1] - 2 * B.R);
 }
 inline long long layout_size(const int n[3], int R) {
   const int nc = R + 1;
@@ -144,7 +146,12 @@ struct PunctureArgs {
 enum Region { REG_ALL = 0, REG_INSIDE = 1, REG_BOUNDARY = 2 };
 enum RobinMode { ROBIN_UPDATE = 0, ROBIN_OPERATOR_R = 1, ROBIN_DEFECT_W = 2 };
 
-constexpr int kNormSubSlots = 8;  // u64 words per logical norm slot
+// A logical norm slot holds kNormSubSlots partial maxima, each in its own
+// 64-byte line (kNormStride u64 words apart); blocks spread their atomics
+// over them and the host takes the max.
+constexpr int kNormSubSlots = 16;
+constexpr int kNormStride = 8;
+constexpr int kNormWords = kNormSubSlots * kNormStride;  // u64 words per logical slot
 constexpr int kCoarseOwn = 8;     // interior points per thread in the coarse solve
 
 void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa,
@@ -152,6 +159,9 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs 
 // per_row: threads per row (colour points per interior row for the colour
 // passes, slots per sub-row for the whole-row kernels); the host sizes
 // RowBlock::rows to at most kRowThreads / per_row
+// Row kernels address fields with 32-bit byte offsets: a box field must stay
+// below 4 GiB (checked when the hierarchy is built).
+constexpr long long kMaxFieldBytes = 1LL << 32;
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                    int eq, int colour);
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
