@@ -157,6 +157,9 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
     if (layout_size(n, b.R) * (long long)sizeof(double) >= kMaxFieldBytes)
       return err = "a box field exceeds 4 GiB", OLL_EINVAL;
+    // point kernels split point indices with 24-bit exact float division
+    if ((long long)n[0] * n[1] * n[2] >= (1LL << 24))
+      return err = "a box exceeds 2^24 points", OLL_EINVAL;
     off.push_back(total);
     total += (layout_size(n, b.R) + 31) & ~31LL;  // 256-byte aligned boxes
   }
@@ -500,61 +503,46 @@ struct Builder {
     S.launches.push_back(L);
   }
 
-  // point items of the target boxes of level tl[h] with transfer refs
+  // point blocks of the target boxes of level tl with transfer refs
   // X[k][t] (k = the member's fine level of the pair)
   static void points(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
-                     bool shell, int mg, std::vector<PtItem> &items, long long &total) {
+                     bool shell, int mg, std::vector<PtBlock> &out) {
     const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
     for (int t = 0; t < n; ++t) {
       const DevBox &B = H.hbox[size_t(f + t)];
       const Hier::XRef r = X[size_t(k)][size_t(t)];
       const long long np = shell ? shell_pts(B) : npts(B);
-      items.push_back(PtItem{H.dbox + f + t, H.dxfer + r.off, r.cnt, mg, total, np});
-      total += np;
+      for (long long q0 = 0; q0 < np; q0 += kPtThreads)
+        out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, mg, unsigned(q0),
+                              unsigned(std::min<long long>(kPtThreads, np - q0))});
     }
   }
 
   void transfer(Launch::Kind kind, const std::vector<Hier *> &M, const std::vector<int> &lv) {
-    // lv[h] = the member's fine level k of the pair (k-1, k)
-    if (kind == Launch::RESTRICT_DEFECT) {
-      std::map<std::pair<int, int>, std::vector<int>> groups;  // coarse (order, eq)
-      for (size_t h = 0; h < M.size(); ++h)
-        if (lv[h] >= 1) groups[{M[h]->level_order(lv[h] - 1), M[h]->prob.eq}].push_back(int(h));
-      for (auto &g : groups) {
-        std::vector<PtItem> items;
-        long long total = 0;
-        for (int h : g.second) {
-          const Hier &H = *M[size_t(h)];
-          const int k = lv[size_t(h)];
-          points(H, k - 1, k, H.rw, false, k <= H.geo.mgrid, items, total);
-        }
-        Launch L{kind};
-        L.order = g.first.first;
-        L.eq = g.first.second;
-        L.off = S.add(items);
-        L.n = int(items.size());
-        L.total = total;
-        S.launches.push_back(L);
-      }
-      return;
-    }
-    std::vector<PtItem> items;
-    long long total = 0;
+    // lv[h] = the member's fine level k of the pair (k-1, k); restrictions
+    // are grouped by the coarse level's (order, eq), ghost fills by the
+    // fine level's order (their Lagrange stencil)
+    std::map<std::pair<int, int>, std::vector<PtBlock>> groups;
     for (size_t h = 0; h < M.size(); ++h) {
       const int k = lv[h];
       if (k < 1) continue;
       const Hier &H = *M[h];
       const int mg = k <= H.geo.mgrid;
-      if (kind == Launch::RESTRICT_U) points(H, k - 1, k, H.ru, false, mg, items, total);
-      if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, items, total);
-      if (kind == Launch::GHOST && !mg) points(H, k, k, H.gf, true, 0, items, total);
+      if (kind == Launch::RESTRICT_U) points(H, k - 1, k, H.ru, false, mg, groups[{0, 0}]);
+      if (kind == Launch::RESTRICT_DEFECT)
+        points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}]);
+      if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, groups[{0, 0}]);
+      if (kind == Launch::GHOST && !mg) points(H, k, k, H.gf, true, 0, groups[{H.level_order(k), 0}]);
     }
-    if (items.empty()) return;
-    Launch L{kind};
-    L.off = S.add(items);
-    L.n = int(items.size());
-    L.total = total;
-    S.launches.push_back(L);
+    for (auto &g : groups) {
+      if (g.second.empty()) continue;
+      Launch L{kind};
+      L.order = g.first.first;
+      L.eq = g.first.second;
+      L.off = S.add(g.second);
+      L.n = int(g.second.size());
+      S.launches.push_back(L);
+    }
   }
 
   void coarse(const std::vector<Hier *> &M) {
@@ -716,10 +704,10 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
                         L.mode);
         break;
       case Launch::RESTRICT_U:
-        launch_restrict_u(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);
+        launch_restrict_u(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
       case Launch::RESTRICT_DEFECT:
-        launch_restrict_defect(s, reinterpret_cast<const PtItem *>(items), L.n, L.total, L.order, L.eq);
+        launch_restrict_defect(s, reinterpret_cast<const PtBlock *>(items), L.n, L.order, L.eq);
         break;
       case Launch::COARSE:
         if (!launch_coarse_solve(s, reinterpret_cast<const CoarseItem *>(items), L.n, L.max_pts,
@@ -727,10 +715,10 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
           return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;
         break;
       case Launch::PROLONG:
-        launch_prolong(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);
+        launch_prolong(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
       case Launch::GHOST:
-        launch_ghost_fill(s, reinterpret_cast<const PtItem *>(items), L.n, L.total);
+        launch_ghost_fill(s, reinterpret_cast<const PtBlock *>(items), L.n, L.order);
         break;
     }
     if (e1) {

@@ -141,15 +141,6 @@ __device__ __forceinline__ unsigned fast_div(unsigned g, unsigned d, float inv_d
   return q;
 }
 
-__device__ __forceinline__ int find_pt_item(const PtItem *items, int n, long long pt) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].p0 <= pt) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
 // coordinate of index i, without contraction so it equals the host value
 __device__ __forceinline__ double coord(const DevBox &B, int a, int i) {
   return __dadd_rn(B.lo[a], __dmul_rn((double)i, B.d[a]));
@@ -188,6 +179,13 @@ typedef const __attribute__((address_space(4))) DevBox ConstBox;
 typedef __attribute__((address_space(1))) char gchar;
 
 __device__ __forceinline__ ConstBox &cbox(const DevBox *b) { return *(ConstBox *)b; }
+// the generic helpers (robin_op_at) take a generic reference
+__device__ __forceinline__ const DevBox &cbox_plain(ConstBox &b) { return *(const DevBox *)&b; }
+// offset of point i within its row (kernels.hpp ioff) from a constant-space box
+__device__ __forceinline__ int ioffc(ConstBox &B, int i) {
+  const int q = int((unsigned(i) * B.ncm) >> 16);
+  return (i - q * B.nc) * B.rl + q;
+}
 
 template <typename T>
 __device__ __forceinline__ T ld(const void *base, unsigned off) {
@@ -693,36 +691,76 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int
 }
 
 // ------------------------------------------------------------------
-// transfers
+// transfers.  Descriptors (target box, transfer, source box) are read
+// through the constant address space; fields through 32-bit byte offsets.
 // ------------------------------------------------------------------
+typedef const __attribute__((address_space(4))) DevXfer ConstXfer;
+typedef const __attribute__((address_space(4))) PtBlock ConstPtBlock;
 
-// tensor Lagrange interpolation of f over box S (reference accumulation
-// order, interpolation.cc:257-296); the nodes of coincident axes are
-// skipped, their weights being exactly one unit vector
-__device__ double tensor_interp(const DevBox &S, const double *__restrict__ f, const DevAxis *ax,
-                                const int t[3], const unsigned fl[3]) {
+__device__ __forceinline__ unsigned boff(ConstBox &S, int i, int j, int k) {
+  return (unsigned(k) * unsigned(S.ps) + unsigned(j) * unsigned(S.rs) + unsigned(ioffc(S, i))) * 8u;
+}
+
+// the block's target point in lexicographic order; false past its end
+__device__ __forceinline__ bool pt_lex(const PtBlock &Bk, ConstBox &T, int &i, int &j, int &k) {
+  if (threadIdx.x >= Bk.nq) return false;
+  const unsigned q = Bk.q0 + threadIdx.x;
+  const unsigned n0 = unsigned(T.n[0]), plane = n0 * unsigned(T.n[1]);
+  const unsigned kq = fast_div(q, plane, T.in01);
+  const unsigned r = q - kq * plane;
+  const unsigned jq = fast_div(r, n0, T.in0);
+  k = int(kq);
+  j = int(jq);
+  i = int(r - jq * n0);
+  return true;
+}
+
+__device__ __forceinline__ bool is_interior(ConstBox &B, int i, int j, int k) {
+  const int R = B.R;
+  return i >= R && i < B.n[0] - R && j >= R && j < B.n[1] - R && k >= R && k < B.n[2] - R;
+}
+
+__device__ __forceinline__ unsigned axis_flags(ConstXfer &X, const int t[3], unsigned fl[3]) {
+  fl[0] = X.ax[0].flags[t[0]];
+  fl[1] = X.ax[1].flags[t[1]];
+  fl[2] = X.ax[2].flags[t[2]];
+  return fl[0] & fl[1] & fl[2];
+}
+
+// tensor Lagrange interpolation of field f of box S with KP nodes per axis
+// (reference accumulation order, interpolation.cc:257-296); the nodes of
+// coincident axes are skipped, their weights being exactly one unit vector
+template <int KP>
+__device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer &X, const int t[3],
+                                          const unsigned fl[3]) {
   int s0[3], kk[3];
   const double *w[3];
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (fl[a] & 4) {
-      s0[a] = ax[a].coinc[t[a]];
+      s0[a] = X.ax[a].coinc[t[a]];
       kk[a] = 1;
       w[a] = nullptr;
     } else {
-      s0[a] = ax[a].start[t[a]];
-      kk[a] = ax[a].k;
-      w[a] = ax[a].w + (long long)t[a] * ax[a].k;
+      s0[a] = X.ax[a].start[t[a]];
+      kk[a] = KP;
+      w[a] = X.ax[a].w + t[a] * KP;
     }
   }
-  const long long sy = S.rs, sz = S.ps;
+  double wz[KP];
+#pragma unroll
+  for (int K = 0; K < KP; ++K) wz[K] = w[2] ? w[2][K] : 1.0;
+  const unsigned sy = unsigned(S.rs) * 8u, sz = unsigned(S.ps) * 8u;
   double acc = 0.0;
   for (int I = 0; I < kk[0]; ++I) {
-    const int xo = ioff(S, s0[0] + I);
+    const unsigned xo = unsigned(ioffc(S, s0[0] + I)) * 8u;
     double uj = 0.0;
     for (int J = 0; J < kk[1]; ++J) {
+      const unsigned base = xo + unsigned(s0[1] + J) * sy + unsigned(s0[2]) * sz;
       double uk = 0.0;
-      const long long base = xo + sy * (s0[1] + J) + sz * (long long)s0[2];
-      for (int K = 0; K < kk[2]; ++K) uk += f[base + sz * K] * (w[2] ? w[2][K] : 1.0);
+#pragma unroll
+      for (int K = 0; K < KP; ++K)
+        if (K < kk[2]) uk += ld<double>(f, base + unsigned(K) * sz) * wz[K];
       uj += uk * (w[1] ? w[1][J] : 1.0);
     }
     acc += uj * (w[0] ? w[0][I] : 1.0);
@@ -731,47 +769,39 @@ __device__ double tensor_interp(const DevBox &S, const double *__restrict__ f, c
 }
 
 // trilinear interpolation of (u - ww) of box S (interpolation.cc:125-182)
-__device__ __forceinline__ double trilinear_corr(const DevBox &S, const DevAxis *ax, const int t[3]) {
-  const double *__restrict__ u = S.f[F_U];
-  const double *__restrict__ ww = S.f[F_WW];
-  const long long sy = S.rs, sz = S.ps;
-  const int x0 = ax[0].start[t[0]];
-  const long long b = pidx(S, x0, ax[1].start[t[1]], ax[2].start[t[2]]);
-  const int x1 = ioff(S, x0 + 1) - ioff(S, x0);
-  const double dxi = ax[0].w[2 * t[0]], dxf = ax[0].w[2 * t[0] + 1];
-  const double dyi = ax[1].w[2 * t[1]], dyf = ax[1].w[2 * t[1] + 1];
-  const double dzi = ax[2].w[2 * t[2]], dzf = ax[2].w[2 * t[2] + 1];
-#define V(o) (u[b + (o)] - ww[b + (o)])
+__device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, const int t[3]) {
+  const void *u = S.f[F_U];
+  const void *ww = S.f[F_WW];
+  const unsigned sy = unsigned(S.rs) * 8u, sz = unsigned(S.ps) * 8u;
+  const int x0 = X.ax[0].start[t[0]];
+  const unsigned b = boff(S, x0, X.ax[1].start[t[1]], X.ax[2].start[t[2]]);
+  const unsigned x1 = unsigned(ioffc(S, x0 + 1) - ioffc(S, x0)) * 8u;
+  const double *wx = X.ax[0].w + 2 * t[0];
+  const double *wy = X.ax[1].w + 2 * t[1];
+  const double *wzp = X.ax[2].w + 2 * t[2];
+  const double dxi = wx[0], dxf = wx[1], dyi = wy[0], dyf = wy[1], dzi = wzp[0], dzf = wzp[1];
+#define V(o) (ld<double>(u, b + (o)) - ld<double>(ww, b + (o)))
   return ((V(0) * dxi + V(x1) * dxf) * dyi + (V(sy) * dxi + V(sy + x1) * dxf) * dyf) * dzi +
          ((V(sz) * dxi + V(sz + x1) * dxf) * dyi + (V(sz + sy) * dxi + V(sz + sy + x1) * dxf) * dyf) *
              dzf;
 #undef V
 }
 
-__device__ __forceinline__ unsigned axis_flags(const DevXfer &X, const int t[3], unsigned fl[3]) {
-  fl[0] = X.ax[0].flags[t[0]];
-  fl[1] = X.ax[1].flags[t[1]];
-  fl[2] = X.ax[2].flags[t[2]];
-  return fl[0] & fl[1] & fl[2];
-}
-
 // u restriction: coarse INSIDE points inside a fine box's non-ghost part
-// (all = false), injection at coincident points
-__global__ __launch_bounds__(256) void k_restrict_u(const PtItem *__restrict__ items, int n,
-                                                     long long total) {
-  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= total) return;
-  const PtItem &I = items[find_pt_item(items, n, pt)];
-  const DevBox &T = *I.box;
+// (all = false), injection at coincident points (Lagrange 3 elsewhere)
+__global__ __launch_bounds__(kPtThreads) void k_restrict_u(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
   int i, j, k;
-  point_ijk(T, pt - I.p0, i, j, k);
-  if (!interior(T, i, j, k)) return;
+  if (!pt_lex(Bk, T, i, j, k) || !is_interior(T, i, j, k)) return;
   const int t[3] = {i, j, k};
-  for (int x = I.nsrc - 1; x >= 0; --x) {
-    const DevXfer &X = I.src[x];
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  for (int x = Bk.nsrc - 1; x >= 0; --x) {
+    ConstXfer &X = xs[x];
     unsigned fl[3];
     if (!(axis_flags(X, t, fl) & 1)) continue;
-    T.f[F_U][pidx(T, i, j, k)] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
+    ConstBox &S = cbox(X.src);
+    st<double>(T.f[F_U], boff(T, i, j, k), tinterp<4>(S, S.f[F_U], X, t, fl));
     return;
   }
 }
@@ -784,104 +814,106 @@ __global__ __launch_bounds__(256) void k_restrict_u(const PtItem *__restrict__ i
 // the point is inside both boxes' non-ghost parts (ffunction.cc:2808-2849).
 // Also saves ww_c = u_c on every coarse point.
 template <int ORD, int EQ>
-__global__ __launch_bounds__(256) void k_restrict_defect(const PtItem *__restrict__ items, int n,
-                                                          long long total) {
-  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= total) return;
-  const PtItem &I = items[find_pt_item(items, n, pt)];
-  const DevBox &T = *I.box;
+__global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
   int i, j, k;
-  point_ijk(T, pt - I.p0, i, j, k);
-  const long long p = pidx(T, i, j, k);
-  T.f[F_WW][p] = T.f[F_U][p];
-  const bool inside = interior(T, i, j, k);
-  if (!I.mg && !inside) return;
+  if (!pt_lex(Bk, T, i, j, k)) return;
+  const unsigned o = boff(T, i, j, k);
+  const double uc = ld<double>(T.f[F_U], o);
+  st<double>(T.f[F_WW], o, uc);
+  const bool inside = is_interior(T, i, j, k);
+  if (!Bk.mg && !inside) return;
   const int t[3] = {i, j, k};
   bool covered = false;
   double wc = 0.0;
-  for (int x = I.nsrc - 1; x >= 0; --x) {
-    const DevXfer &X = I.src[x];
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  for (int x = Bk.nsrc - 1; x >= 0; --x) {
+    ConstXfer &X = xs[x];
     unsigned fl[3];
     const unsigned f = axis_flags(X, t, fl);
-    if (!((f & 1) || (I.mg && (f & 2)))) continue;
-    const DevBox &S = *X.src;
-    const double *__restrict__ w = S.f[F_W];
+    if (!((f & 1) || (Bk.mg && (f & 2)))) continue;
+    ConstBox &S = cbox(X.src);
+    const void *w = S.f[F_W];
     if (f & 8) {
-      const long long sy = S.rs, sz = S.ps;
+      const unsigned sy = unsigned(S.rs) * 8u, sz = unsigned(S.ps) * 8u;
       const int ci = X.ax[0].coinc[i];
-      const long long c = pidx(S, ci, X.ax[1].coinc[j], X.ax[2].coinc[k]);
-      const int xo = ioff(S, ci);
-      const int xm = ioff(S, ci - 1) - xo, xp = ioff(S, ci + 1) - xo;
+      const unsigned c = boff(S, ci, X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      const int xo = ioffc(S, ci);
+      const int xm = (ioffc(S, ci - 1) - xo) * 8, xp = (ioffc(S, ci + 1) - xo) * 8;
       double acc = 0.0;
       for (int dz = -1; dz <= 1; ++dz) {
         const double wz = dz == 0 ? 0.5 : 0.25;
         for (int dy = -1; dy <= 1; ++dy) {
           const double wy = dy == 0 ? 0.5 : 0.25;
-          const long long r = c + sy * dy + sz * dz;
-          acc += wz * wy * (0.25 * w[r + xm] + 0.5 * w[r] + 0.25 * w[r + xp]);
+          const unsigned r = c + unsigned(dy) * sy + unsigned(dz) * sz;
+          acc += wz * wy * (0.25 * ld<double>(w, r + xm) + 0.5 * ld<double>(w, r) + 0.25 * ld<double>(w, r + xp));
         }
       }
       wc = acc;
     } else {
-      wc = tensor_interp(S, w, X.ax, t, fl);
+      wc = tinterp<3>(S, w, X, t, fl);
     }
     covered = true;
     break;
   }
-  if (!covered && !I.mg) return;
-  if (covered) T.f[F_W][p] = wc; else wc = T.f[F_W][p];
+  if (!covered && !Bk.mg) return;
+  if (covered) st<double>(T.f[F_W], o, wc); else wc = ld<double>(T.f[F_W], o);
   double Rc;
   if (inside) {
-    const double *__restrict__ u = T.f[F_U];
+    constexpr int NC = ORD / 2 + 1;
+    int xp[NC - 1], xm[NC - 1];
+    xoffs_bytes<NC, 8>(i % NC, T.rl, xp, xm);
     double dN;
-    nonlinear<EQ>(lap_at<ORD>(T, u, p, i), T.dulap, T.f[F_B][p],
-                  T.f[F_C][p] + u[p], EQ ? T.f[F_D][p] : 0.0, EQ ? T.f[F_E][p] : 0.0, Rc, dN);
+    nonlinear<EQ>(lapb<ORD, double>(T.f[F_U], o, xp, xm, unsigned(T.rs) * 8u, unsigned(T.ps) * 8u,
+                                    T.idx2[0], T.idx2[1], T.idx2[2]),
+                  T.dulap, ld<double>(T.f[F_B], o), ld<double>(T.f[F_C], o) + uc,
+                  EQ ? ld<double>(T.f[F_D], o) : 0.0, EQ ? ld<double>(T.f[F_E], o) : 0.0, Rc, dN);
   } else {
-    Rc = robin_op_at(T, i, j, k);
+    Rc = robin_op_at(cbox_plain(T), i, j, k);
   }
-  T.f[F_RHO][p] = wc + Rc;
+  st<double>(T.f[F_RHO], o, wc + Rc);
 }
 
 // prolongation of the coarse correction and u += v (all points on
 // multigrid levels, interior points on refinement levels)
-__global__ __launch_bounds__(256) void k_prolong(const PtItem *__restrict__ items, int n,
-                                                  long long total) {
-  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= total) return;
-  const PtItem &I = items[find_pt_item(items, n, pt)];
-  const DevBox &T = *I.box;
+__global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
   int i, j, k;
-  point_ijk(T, pt - I.p0, i, j, k);
-  if (!I.mg && !interior(T, i, j, k)) return;
+  if (!pt_lex(Bk, T, i, j, k)) return;
+  if (!Bk.mg && !is_interior(T, i, j, k)) return;
   const int t[3] = {i, j, k};
-  for (int x = I.nsrc - 1; x >= 0; --x) {
-    const DevXfer &X = I.src[x];
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  for (int x = Bk.nsrc - 1; x >= 0; --x) {
+    ConstXfer &X = xs[x];
     unsigned fl[3];
     const unsigned f = axis_flags(X, t, fl);
     if (!(f & 3)) continue;  // all = true
-    const DevBox &S = *X.src;
+    ConstBox &S = cbox(X.src);
     double v;
     if (f & 4) {
-      const long long c = pidx(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
-      v = S.f[F_U][c] - S.f[F_WW][c];
+      const unsigned c = boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      v = ld<double>(S.f[F_U], c) - ld<double>(S.f[F_WW], c);
     } else {
-      v = trilinear_corr(S, X.ax, t);
+      v = trilinear_corr(S, X, t);
     }
-    T.f[F_U][pidx(T, i, j, k)] += v;
+    const unsigned o = boff(T, i, j, k);
+    st<double>(T.f[F_U], o, ld<double>(T.f[F_U], o) + v);
     return;
   }
 }
 
 // shell point q of a box: the R-thick z slabs first, then y slabs, then x slabs
-__device__ __forceinline__ void shell_point(const DevBox &B, unsigned q, int &i, int &j, int &k) {
+__device__ __forceinline__ void shell_point(ConstBox &B, unsigned q, int &i, int &j, int &k) {
   const unsigned R = unsigned(B.R);
   const unsigned n0 = unsigned(B.n[0]), n1 = unsigned(B.n[1]), n2 = unsigned(B.n[2]);
   const unsigned zs = n0 * n1 * R;  // one z slab
   if (q < 2 * zs) {
     const unsigned s = q >= zs, r = q - s * zs;
     const unsigned plane = n0 * n1;
-    const unsigned l = r / plane, rr = r - l * plane;
-    j = int(rr / n0);
+    const unsigned l = fast_div(r, plane, B.in01), rr = r - l * plane;
+    j = int(fast_div(rr, n0, B.in0));
     i = int(rr - unsigned(j) * n0);
     k = s ? int(n2 - R + l) : int(l);
     return;
@@ -890,7 +922,7 @@ __device__ __forceinline__ void shell_point(const DevBox &B, unsigned q, int &i,
   const unsigned ys = n0 * R * (n2 - 2 * R);
   if (q < 2 * ys) {
     const unsigned s = q >= ys, r = q - s * ys;
-    const unsigned row = r / n0;
+    const unsigned row = fast_div(r, n0, B.in0);
     i = int(r - row * n0);
     const unsigned kk = row / R, l = row - kk * R;
     k = int(R + kk);
@@ -901,29 +933,30 @@ __device__ __forceinline__ void shell_point(const DevBox &B, unsigned q, int &i,
   const unsigned xs = R * (n1 - 2 * R) * (n2 - 2 * R);
   const unsigned s = q >= xs, r = q - s * xs;
   const unsigned col = r / R, l = r - col * R;
-  const unsigned kk = col / (n1 - 2 * R);
+  const unsigned kk = fast_div(col, n1 - 2 * R, B.inyi);
   j = int(R + (col - kk * (n1 - 2 * R)));
   k = int(R + kk);
   i = s ? int(n0 - R + l) : int(l);
 }
 
 // ghost shell of refinement boxes from the coarser level (Lagrange
-// order+1); one thread per shell point
-__global__ __launch_bounds__(256) void k_ghost_fill(const PtItem *__restrict__ items, int n,
-                                                     long long total) {
-  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pt >= total) return;
-  const PtItem &I = items[find_pt_item(items, n, pt)];
-  const DevBox &T = *I.box;
+// order+1, KP = order+2 nodes per axis); one thread per shell point
+template <int KP>
+__global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
+  if (threadIdx.x >= Bk.nq) return;
   int i, j, k;
-  shell_point(T, unsigned(pt - I.p0), i, j, k);
+  shell_point(T, Bk.q0 + threadIdx.x, i, j, k);
   const int t[3] = {i, j, k};
-  for (int x = I.nsrc - 1; x >= 0; --x) {
-    const DevXfer &X = I.src[x];
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  for (int x = Bk.nsrc - 1; x >= 0; --x) {
+    ConstXfer &X = xs[x];
     unsigned fl[3];
     const unsigned f = axis_flags(X, t, fl);
     if (!(f & 3)) continue;
-    T.f[F_U][pidx(T, i, j, k)] = tensor_interp(*X.src, X.src->f[F_U], X.ax, t, fl);
+    ConstBox &S = cbox(X.src);
+    st<double>(T.f[F_U], boff(T, i, j, k), tinterp<KP>(S, S.f[F_U], X, t, fl));
     return;
   }
 }
@@ -1319,10 +1352,9 @@ void update32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
 }
 
 template <int ORD, int EQ>
-void restrict_defect_impl(hipStream_t s, const PtItem *items, int n, long long total) {
-  if (total == 0) return;
-  hipLaunchKernelGGL((k_restrict_defect<ORD, EQ>), dim3(ceil_div(total, 256)), dim3(256), 0, s,
-                     items, n, total);
+void restrict_defect_impl(hipStream_t s, const PtBlock *blocks, int nb) {
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_restrict_defect<ORD, EQ>), dim3(nb), dim3(kPtThreads), 0, s, blocks);
 }
 
 constexpr int kLdsLimit = 160 * 1024;
@@ -1389,24 +1421,28 @@ void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, 
   hipLaunchKernelGGL(k_robin_edges_corners, dim3(n), dim3(1024), 0, s, items, mode);
 }
 
-void launch_restrict_u(hipStream_t s, const PtItem *items, int n, long long total) {
-  if (total == 0) return;
-  hipLaunchKernelGGL(k_restrict_u, dim3(ceil_div(total, 256)), dim3(256), 0, s, items, n, total);
+void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks) {
+  if (nblocks == 0) return;
+  hipLaunchKernelGGL(k_restrict_u, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
 }
 
-void launch_restrict_defect(hipStream_t s, const PtItem *items, int n, long long total, int order,
-                            int eq) {
-  OLL_DISPATCH_ORDER(order, eq, restrict_defect_impl, s, items, n, total);
+void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq) {
+  OLL_DISPATCH_ORDER(order, eq, restrict_defect_impl, s, blocks, nblocks);
 }
 
-void launch_prolong(hipStream_t s, const PtItem *items, int n, long long total) {
-  if (total == 0) return;
-  hipLaunchKernelGGL(k_prolong, dim3(ceil_div(total, 256)), dim3(256), 0, s, items, n, total);
+void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks) {
+  if (nblocks == 0) return;
+  hipLaunchKernelGGL(k_prolong, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
 }
 
-void launch_ghost_fill(hipStream_t s, const PtItem *items, int n, long long total) {
-  if (total == 0) return;
-  hipLaunchKernelGGL(k_ghost_fill, dim3(ceil_div(total, 256)), dim3(256), 0, s, items, n, total);
+void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order) {
+  if (nblocks == 0) return;
+  switch (order) {
+    case 2: hipLaunchKernelGGL(k_ghost_fill<4>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
+    case 4: hipLaunchKernelGGL(k_ghost_fill<6>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
+    case 6: hipLaunchKernelGGL(k_ghost_fill<8>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
+    default: hipLaunchKernelGGL(k_ghost_fill<10>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
+  }
 }
 
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,

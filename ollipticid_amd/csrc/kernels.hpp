@@ -41,6 +41,7 @@ struct DevBox {
   long long ps;      // plane stride, rs * n1
   long long size;    // elements per field, ps * n2
   float inyi;        // 1 / (n1 - 2R): interior row -> (j, k) by float division
+  float in0, in01;   // 1 / n0, 1 / (n0 n1): point index -> (i, j, k)
   double *f[NFIELDS];
   float *g[NFIELDS32];  // null unless the hierarchy runs mixed precision
 };
@@ -67,6 +68,8 @@ inline void set_layout(DevBox &B) {
   B.ps = (long long)B.rs * B.n[1];
   B.size = B.ps * B.n[2];
   B.inyi = 1.0f / float(B.n[1] - 2 * B.R);
+  B.in0 = 1.0f / float(B.n[0]);
+  B.in01 = 1.0f / float(B.n[0] * B.n[1]);
 }
 inline long long layout_size(const int n[3], int R) {
   const int nc = R + 1;
@@ -103,16 +106,19 @@ struct RowBlock {
 };
 constexpr int kRowThreads = 256;  // threads per row workgroup
 
-// point-parallel launches (transfers): item owns target points p0..p0+np
-// (all points of the box, or its ghost-shell points for the ghost fill)
-struct PtItem {
+// point-parallel launches (transfers): one workgroup per PtBlock, which
+// owns target points q0 .. q0+nq-1 (nq <= kPtThreads) of one box, either
+// its lexicographic points (i fastest) or its ghost-shell points in shell
+// order (the ghost fill)
+struct PtBlock {
   const DevBox *box;
   const DevXfer *src;  // source tables in source order
   int nsrc;
   int mg;              // multigrid level (Robin shell, "all" coverage)
-  long long p0;
-  long long np;
+  unsigned q0;
+  unsigned nq;
 };
+constexpr int kPtThreads = 256;
 
 // Robin stages: face columns col0..col0+ncol of a box
 struct RobinItem {
@@ -174,11 +180,11 @@ void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
                      int colour);
 void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order);
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
-void launch_restrict_u(hipStream_t s, const PtItem *items, int n, long long total);
-void launch_restrict_defect(hipStream_t s, const PtItem *items, int n, long long total, int order,
-                            int eq);
-void launch_prolong(hipStream_t s, const PtItem *items, int n, long long total);
-void launch_ghost_fill(hipStream_t s, const PtItem *items, int n, long long total);
+void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
+void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq);
+void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks);
+// order: stencil order of the target (refinement) boxes; Lagrange order+1
+void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order);
 // false when the largest box does not fit the LDS-resident kernel
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
                          int eq, double eps, int max_iter);
