@@ -555,7 +555,7 @@ struct Builder {
                                               H->coarse_omega});
       auto &sz = sizes[H->prob.eq];
       sz.first = std::max<int>(sz.first, int(B.size));
-      sz.second = std::max<int>(sz.second, (B.n[0] - 2) * (B.n[1] - 2) * (B.n[2] - 2));
+      sz.second = std::max<int>(sz.second, int(coarse_colour_slots(B.n)));
     }
     for (auto &g : groups) {
       Launch L{Launch::COARSE};
@@ -580,9 +580,7 @@ int build_vcycle_from(const std::vector<Hier *> &M, const std::vector<int> &star
       return err = "members of one schedule must share nu1 and nu2", OLL_EINVAL;
   for (Hier *H : M) {
     const DevBox &B = H->hbox[size_t(H->geo.level_first[0])];
-    const size_t lds = 2 * sizeof(double) * size_t(B.size);
-    if (lds + 2048 > size_t(kCoarseLdsLimit) ||
-        (B.n[0] - 2) * (B.n[1] - 2) * (B.n[2] - 2) > 1024 * kCoarseOwn)
+    if (!coarse_box_fits(B.n, B.size))
       return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;
   }
   Builder b{S};

@@ -964,97 +964,228 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
 // ------------------------------------------------------------------
 // coarsest level: {Robin, one multi-colour sweep} until the sweep norm is
 // <= eps or stops changing (elliptic.cc:1782-1816); one block per
-// hierarchy, u and rho in LDS for the whole loop, b, c (d, e) of the owned
-// interior points in registers.  Interior updates are over-relaxed by omega
-// (omega = 1 is the reference's Gauss-Seidel-Newton); the loop's fixed
-// point, the coarse solution, does not depend on omega.
+// hierarchy, u and rho in LDS (colour-split layout) for the whole loop, b,
+// c (d, e) of the owned interior points in registers.  Interior updates are
+// over-relaxed by omega (omega = 1 is the reference's Gauss-Seidel-Newton);
+// the loop's fixed point, the coarse solution, does not depend on omega.
+// The coarsest level always has order 2 (R = 1): its Robin shell is one
+// layer of faces, then edges, then corners (boundary.cc:447-1101), each
+// phase reading only the previous ones; every thread decodes its shell
+// points once.
 // ------------------------------------------------------------------
-template <int EQ>
-__global__ __launch_bounds__(1024) void k_coarse_solve(const CoarseItem *__restrict__ items,
-                                                        double eps, int max_iter) {
+struct CoarseGeo {
+  int n[3];
+  int rl, rs, ps, nc;
+  unsigned ncm;
+  double lo[3], d[3], idx[3], idx2[3];
+  double ipn, dulap;
+};
+
+__device__ __forceinline__ int coarse_ioff(const CoarseGeo &G, int i) {
+  const int q = int((unsigned(i) * G.ncm) >> 16);
+  return (i - q * G.nc) * G.rl + q;
+}
+__device__ __forceinline__ int coarse_p(const CoarseGeo &G, int i, int j, int k) {
+  return k * G.ps + j * G.rs + coarse_ioff(G, i);
+}
+
+// Robin update at shell point (i, j, k) of the LDS field; mask has bit a
+// for each axis whose index is 0 or n-1 (dx2 one-sided derivative,
+// diff_2nd.cc:237-335)
+__device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, const double *srho,
+                                             int i, int j, int k, int mask) {
+  const int p = coarse_p(G, i, j, k);
+  const int ijk[3] = {i, j, k};
+  double X[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) X[a] = __dadd_rn(G.lo[a], __dmul_rn((double)ijk[a], G.d[a]));
+  const double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+  double sF = 0.0, sdF = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!(mask & (1 << a))) continue;
+    const bool lo = ijk[a] == 0;
+    const int dir = lo ? 1 : -1;
+    int o1, o2;
+    if (a == 0) {
+      const int x0 = coarse_ioff(G, i);
+      o1 = coarse_ioff(G, i + dir) - x0;
+      o2 = coarse_ioff(G, i + 2 * dir) - x0;
+    } else {
+      const int st = a == 1 ? G.rs : G.ps;
+      o1 = dir * st;
+      o2 = 2 * dir * st;
+    }
+    const double id = G.idx[a];
+    double cf, dv;
+    if (lo) {
+      cf = -3.0 * 0.5 * id;
+      dv = (-3.0 * su[p] + 4.0 * su[p + o1] - su[p + o2]) * 0.5 * id;
+    } else {
+      cf = 3.0 * 0.5 * id;
+      dv = (3.0 * su[p] - 4.0 * su[p + o1] + su[p + o2]) * 0.5 * id;
+    }
+    sF += dv / X[a];
+    sdF += cf / X[a];
+  }
+  const double op = r2 * G.ipn * sF + su[p];
+  const double Fv = op - srho[p];
+  const double dF = r2 * G.ipn * sdF + 1.0;
+  su[p] = su[p] - Fv / dF;
+}
+
+constexpr int kCoarseThreads = 512;
+
+// OWN: interior points per colour per thread (colour c's points are
+// numbered row by row, ceil(ni / 2) slots per interior row)
+template <int EQ, int OWN>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseItem *__restrict__ items,
+                                                                 double eps, int max_iter) {
   extern __shared__ double lds[];
-  __shared__ double sh[16];
-  __shared__ double s_norm;
-  const CoarseItem &It = items[blockIdx.x];
-  const DevBox &B = *It.box;
+  __shared__ double sh[kCoarseThreads / 64];
+  const CoarseItem It = items[blockIdx.x];
+  ConstBox &B = cbox(It.box);
+  CoarseGeo G;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    G.n[a] = B.n[a];
+    G.lo[a] = B.lo[a];
+    G.d[a] = B.d[a];
+    G.idx[a] = B.idx[a];
+    G.idx2[a] = B.idx2[a];
+  }
+  G.rl = B.rl;
+  G.rs = B.rs;
+  G.ps = int(B.ps);
+  G.nc = B.nc;
+  G.ncm = B.ncm;
+  G.ipn = B.ipn;
+  G.dulap = B.dulap;
   const double omega = It.omega;
-  const int R = B.R;  // 1: the coarsest level always uses order 2
-  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-  const int np = int(B.size);  // the LDS copy keeps the colour-split layout
+  const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+  const int np = int(B.size);
   double *su = lds;
   double *srho = lds + np;
-  const int ni = n0 - 2 * R, nj = n1 - 2 * R, nk = n2 - 2 * R;
-  double *__restrict__ ug = B.f[F_U];
-  for (int q = threadIdx.x; q < np; q += blockDim.x) {
-    su[q] = ug[q];
-    srho[q] = B.f[F_RHO][q];
+  double *ug = B.f[F_U];
+  const double *rg = B.f[F_RHO];
+  for (int q = threadIdx.x; q < np; q += kCoarseThreads) {
+    su[q] = ld<double>(ug, unsigned(q) * 8u);
+    srho[q] = ld<double>(rg, unsigned(q) * 8u);
   }
-  const int nint = ni * nj * nk;
-  int own_p[kCoarseOwn], own_c[kCoarseOwn], own_s[kCoarseOwn];
-  double own_b[kCoarseOwn], own_cc[kCoarseOwn], own_d[kCoarseOwn], own_e[kCoarseOwn];
+  // owned interior points, colour by colour
+  const int ni = n0 - 2, nj = n1 - 2, nk = n2 - 2;
+  const int per_row = (ni + 1) / 2;
+  const int slots = nj * nk * per_row;
+  int own_p[2 * OWN];
+  double own_b[2 * OWN], own_cc[2 * OWN], own_d[2 * OWN], own_e[2 * OWN];
 #pragma unroll
-  for (int m = 0; m < kCoarseOwn; ++m) {
-    const int t = threadIdx.x + m * blockDim.x;
-    own_p[m] = -1;
-    own_c[m] = -1;
-    own_s[m] = 0;
-    own_b[m] = own_cc[m] = own_d[m] = own_e[m] = 0.0;
-    if (t < nint) {
-      const int i = R + t % ni, j = R + (t / ni) % nj, k = R + t / (ni * nj);
-      const int p = int(pidx(B, i, j, k));
-      own_p[m] = p;
-      own_c[m] = (i + j + k) % (R + 1);
-      own_s[m] = i & 1;
-      own_b[m] = B.f[F_B][p];
-      own_cc[m] = B.f[F_C][p];
+  for (int col = 0; col < 2; ++col)
+#pragma unroll
+    for (int m = 0; m < OWN; ++m) {
+      const int q = col * OWN + m;
+      const int t = threadIdx.x + m * kCoarseThreads;
+      own_p[q] = -1;
+      own_b[q] = own_cc[q] = own_d[q] = own_e[q] = 0.0;
+      if (t >= slots) continue;
+      const int row = t / per_row, tt = t - row * per_row;
+      const int j = 1 + row % nj, k = 1 + row / nj;
+      const int i = 1 + ((col - j - k - 1) & 1) + 2 * tt;
+      if (i >= n0 - 1) continue;
+      const int p = coarse_p(G, i, j, k) | ((i & 1) << 30);
+      own_p[q] = p;
+      const unsigned o = unsigned(p & 0x3fffffff) * 8u;
+      own_b[q] = ld<double>(B.f[F_B], o);
+      own_cc[q] = ld<double>(B.f[F_C], o);
       if (EQ) {
-        own_d[m] = B.f[F_D][p];
-        own_e[m] = B.f[F_E][p];
+        own_d[q] = ld<double>(B.f[F_D], o);
+        own_e[q] = ld<double>(B.f[F_E], o);
       }
     }
-  }
+  // shell points: faces (x, y, z pairs), edges (XY, XZ, YZ), corners;
+  // decoded each sweep (one Robin body keeps the register budget)
+  const int fx = nj * nk, fy = ni * nk, fz = ni * nj;
+  const int nface = 2 * (fx + fy + fz), nedge = 4 * (ni + nj + nk);
+  const float ini = 1.0f / float(ni), inj = 1.0f / float(nj), ink = 1.0f / float(nk);
   __syncthreads();
-  BoxData D;
-  D.u = su;
-  D.rho = srho;
-  D.r = nullptr;
-  D.w = nullptr;
-  const int nface = face_columns(B);
   double prev = 1.0, cur = 1.0;
   int it = 0;
   do {
     prev = cur;
-    for (int t = threadIdx.x; t < nface; t += blockDim.x) robin_face_column(B, D, t, ROBIN_UPDATE);
+    for (int t = threadIdx.x; t < nface; t += kCoarseThreads) {
+      int i, j, k, mask, r = t;
+      if (r < 2 * fx) {
+        const int hi = r >= fx;
+        r -= hi * fx;
+        const int q = int(fast_div(unsigned(r), unsigned(nj), inj));
+        i = hi ? n0 - 1 : 0; j = 1 + r - q * nj; k = 1 + q; mask = 1;
+      } else if ((r -= 2 * fx) < 2 * fy) {
+        const int hi = r >= fy;
+        r -= hi * fy;
+        const int q = int(fast_div(unsigned(r), unsigned(ni), ini));
+        j = hi ? n1 - 1 : 0; i = 1 + r - q * ni; k = 1 + q; mask = 2;
+      } else {
+        r -= 2 * fy;
+        const int hi = r >= fz;
+        r -= hi * fz;
+        const int q = int(fast_div(unsigned(r), unsigned(ni), ini));
+        k = hi ? n2 - 1 : 0; i = 1 + r - q * ni; j = 1 + q; mask = 4;
+      }
+      coarse_robin(G, su, srho, i, j, k, mask);
+    }
     __syncthreads();
-    robin_edges_corners(B, D, ROBIN_UPDATE);
+    if (threadIdx.x < nedge) {
+      int t = threadIdx.x, i, j, k, mask;
+      if (t < 4 * nk) {
+        const int e = int(fast_div(unsigned(t), unsigned(nk), ink));
+        i = (e >> 1) ? n0 - 1 : 0; j = (e & 1) ? n1 - 1 : 0; k = 1 + t - e * nk; mask = 3;
+      } else if ((t -= 4 * nk) < 4 * nj) {
+        const int e = int(fast_div(unsigned(t), unsigned(nj), inj));
+        i = (e >> 1) ? n0 - 1 : 0; k = (e & 1) ? n2 - 1 : 0; j = 1 + t - e * nj; mask = 5;
+      } else {
+        t -= 4 * nj;
+        const int e = int(fast_div(unsigned(t), unsigned(ni), ini));
+        k = (e >> 1) ? n2 - 1 : 0; j = (e & 1) ? n1 - 1 : 0; i = 1 + t - e * ni; mask = 6;
+      }
+      coarse_robin(G, su, srho, i, j, k, mask);
+    }
     __syncthreads();
-    double m = 0.0;
-    for (int col = 0; col <= R; ++col) {
+    if (threadIdx.x < 8)
+      coarse_robin(G, su, srho, (threadIdx.x >> 1) & 1 ? n0 - 1 : 0, (threadIdx.x >> 2) & 1 ? n1 - 1 : 0,
+                   threadIdx.x & 1 ? n2 - 1 : 0, 7);
+    __syncthreads();
+    double mx = 0.0;
 #pragma unroll
-      for (int q = 0; q < kCoarseOwn; ++q) {
-        if (own_c[q] != col) continue;
-        const long long p = own_p[q];
+    for (int col = 0; col < 2; ++col) {
+#pragma unroll
+      for (int m = 0; m < OWN; ++m) {
+        const int q = col * OWN + m;
+        if (own_p[q] < 0) continue;
+        const int p = own_p[q] & 0x3fffffff;
         int xp[1], xm[1];
-        xoffs<2>(own_s[q], B.rl, xp, xm);
-        const double lu = lap<2>(su, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
+        xoffs<2>(own_p[q] >> 30, G.rl, xp, xm);
+        const double lu = lap<2>(su, p, xp, xm, G.rs, G.ps, G.idx2[0], G.idx2[1], G.idx2[2]);
         double N, dN;
-        nonlinear<EQ>(lu, B.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
+        nonlinear<EQ>(lu, G.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
         const double res = N - srho[p];
         const double un = su[p] - omega * (res / dN);
         su[p] = un;
-        m = fmax(m, fabs(res));
-        if (!(un == un)) m = __longlong_as_double(0x7ff0000000000000LL);
+        mx = fmax(mx, fabs(res));
+        if (!(un == un)) mx = __longlong_as_double(0x7ff0000000000000LL);
+        // keep each point's live range local (the unrolled bodies would
+        // otherwise be interleaved past the register budget)
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();
+      if (col == 0) __syncthreads();
     }
-    const double bm = block_max(m, sh);
-    if (threadIdx.x == 0) s_norm = bm;
+    const double wm = wave_max_dpp(mx);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = wm;
     __syncthreads();
-    cur = s_norm;
+    cur = sh[0];
+#pragma unroll
+    for (int w = 1; w < kCoarseThreads / 64; ++w) cur = fmax(cur, sh[w]);
     ++it;
-    __syncthreads();
   } while (cur > eps && prev != cur && it < max_iter && !isinf(cur));
-  for (int q = threadIdx.x; q < np; q += blockDim.x) ug[q] = su[q];
+  for (int q = threadIdx.x; q < np; q += kCoarseThreads) st<double>(ug, unsigned(q) * 8u, su[q]);
   if (threadIdx.x == 0) {
     *It.norm = (unsigned long long)__double_as_longlong(cur);
     *It.iters = it;
@@ -1448,12 +1579,21 @@ void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int or
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
                          int eq, double eps, int max_iter) {
   const size_t lds = 2 * sizeof(double) * size_t(max_pts);  // max_pts: largest B.size
-  if (lds + 2048 > size_t(kLdsLimit) || max_int > 1024 * kCoarseOwn) return false;
+  // max_int: largest number of points of one colour slot list, ceil(ni/2) nj nk
+  if (lds + 2048 > size_t(kLdsLimit) || max_int > kCoarseThreads * 8) return false;
   if (n == 0) return true;
-  if (eq)
-    hipLaunchKernelGGL(k_coarse_solve<1>, dim3(n), dim3(1024), lds, s, items, eps, max_iter);
-  else
-    hipLaunchKernelGGL(k_coarse_solve<0>, dim3(n), dim3(1024), lds, s, items, eps, max_iter);
+  const bool small = max_int <= kCoarseThreads * 4;
+  if (eq) {
+    if (small)
+      hipLaunchKernelGGL((k_coarse_solve<1, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+    else
+      hipLaunchKernelGGL((k_coarse_solve<1, 8>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+  } else {
+    if (small)
+      hipLaunchKernelGGL((k_coarse_solve<0, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+    else
+      hipLaunchKernelGGL((k_coarse_solve<0, 8>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+  }
   return true;
 }
 
@@ -1465,10 +1605,12 @@ void launch_zero_lists(hipStream_t s, unsigned long long *const *ptrs, int n, in
 void kernels_init() {
   static bool done = false;
   if (done) return;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<0>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_coarse_solve<1>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
+  const void *fns[] = {reinterpret_cast<const void *>(&k_coarse_solve<0, 4>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<0, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<1, 4>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<1, 8>)};
+  for (const void *f : fns)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
   done = true;
 }
 

@@ -158,7 +158,17 @@ enum RobinMode { ROBIN_UPDATE = 0, ROBIN_OPERATOR_R = 1, ROBIN_DEFECT_W = 2 };
 constexpr int kNormSubSlots = 16;
 constexpr int kNormStride = 8;
 constexpr int kNormWords = kNormSubSlots * kNormStride;  // u64 words per logical slot
-constexpr int kCoarseOwn = 8;     // interior points per thread in the coarse solve
+// colour slot list of the coarse solve: ceil(ni / 2) * nj * nk points
+inline long long coarse_colour_slots(const int n[3]) {
+  return (long long)((n[0] - 1) / 2) * (n[1] - 2) * (n[2] - 2);
+}
+// largest coarsest box the LDS-resident solve (512 threads) takes: <= 8
+// points per colour per thread, edge points <= 512, u and rho within the LDS
+inline bool coarse_box_fits(const int n[3], long long size) {
+  const long long ni = n[0] - 2, nj = n[1] - 2, nk = n[2] - 2;
+  return coarse_colour_slots(n) <= 8 * 512 && 4 * (ni + nj + nk) <= 512 &&
+         2 * 8 * size + 2048 <= 160 * 1024;
+}
 
 void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa,
                              double robin_a);
