@@ -133,7 +133,20 @@ def timed_batches(batch, warmup, steps):
     return wall, lat, results
 
 
-def roofline(batch, kernel):
+def measured_traffic(name):
+    """HBM bytes per launch of a smoother from its committed hardware-counter
+    summary (rocprofv3 FETCH_SIZE and WRITE_SIZE of the same workload and
+    batch, scaled by a device-to-device copy of known size); None if absent."""
+    path = os.path.join(REPO, "profiles", name)
+    if not name or not os.path.isfile(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def roofline(batch, kernel, traffic_profile):
     """One more batch with HIP events around every smoother launch."""
     batch.engine.set_kernel_timing(True)
     batch.step()
@@ -142,7 +155,8 @@ def roofline(batch, kernel):
     avg_ms = kms / launches
     achieved = (kbytes / launches) / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(traffic_profile), "traffic_source": "profiles/" + traffic_profile,
             "launches": launches, "avg_launch_us": 1e3 * avg_ms,
             "bytes_per_launch": kbytes / launches}
 
@@ -202,7 +216,8 @@ def main():
     wall, lat, results = timed_batches(batch, args.warmup, args.steps)
     for c, r, a in results:
         assert c == cycles and r == res and list(a) == list(adm), (c, r, a)
-    roof = roofline(batch, "k_smooth<4,0> (FP64 multi-colour GS-Newton pass)")
+    roof = roofline(batch, "k_smooth<4,0> (FP64 multi-colour GS-Newton pass)",
+                    "pmc_k_smooth_fp64_binary_batch16.md" if args.batch == 16 else "")
     batch.close()
 
     value = args.batch * args.steps / wall
@@ -245,7 +260,8 @@ def main():
         for ib in range(len(u64)):
             d = np.abs(mb.hiers[0].get_field(ib, oa.F_U) - u64[ib])[masks[ib]]
             du = max(du, float(d.max()))
-        roof_m = roofline(mb, "k_smooth32<4> (FP32 correction colour pass)")
+        roof_m = roofline(mb, "k_smooth32<4> (FP32 correction colour pass)",
+                          "pmc_k_smooth32_mixed_binary_batch16.md" if args.batch == 16 else "")
         mb.close()
         mixed_value = args.batch * args.steps / wall_m
         out["mixed"] = {

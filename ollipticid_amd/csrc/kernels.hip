@@ -24,6 +24,11 @@
 
 #include "kernels.hpp"
 
+// No implicit contraction: every kernel's rounding is fixed by its source,
+// whatever the launch shape or unrolling (a request solves to the same bits
+// alone and in a batch).  Hot expressions use explicit fma.
+#pragma clang fp contract(off)
+
 namespace oll {
 
 namespace {
@@ -45,36 +50,45 @@ __device__ __forceinline__ void xoffs(int s, int rl, int *xp, int *xm) {
   }
 }
 
+// axis-star second differences (explicit fma; shared by all stencil kernels)
+template <int ORD, typename T>
+__device__ __forceinline__ T star(T u0, T s1, T s2, T s3, T s4) {
+  if constexpr (ORD == 2) {
+    return fma(T(-2), u0, s1);
+  } else if constexpr (ORD == 4) {
+    return fma(T(-30), u0, fma(T(16), s1, -s2));
+  } else if constexpr (ORD == 6) {
+    return fma(T(-490), u0, fma(T(270), s1, fma(T(-27), s2, T(2) * s3)));
+  } else {
+    return fma(T(-14350), u0, fma(T(8064), s1, fma(T(-1008), s2, fma(T(128), s3, T(-9) * s4))));
+  }
+}
+template <int ORD, typename T>
+__device__ __forceinline__ T star_sum(T ax, T ay, T az, T ix2, T iy2, T iz2) {
+  const T s = fma(ax, ix2, fma(ay, iy2, az * iz2));
+  if constexpr (ORD == 2) return s;
+  else if constexpr (ORD == 4) return s * (T(1) / T(12));
+  else if constexpr (ORD == 6) return s * (T(1) / T(180));
+  else return s * (T(1) / T(5040));
+}
+
 // axis-star Laplacian at storage index p; i-neighbours at p + xp[m-1] and
 // p + xm[m-1], j- and k-neighbours m * sy and m * sz away
 template <int ORD, typename T = double>
 __device__ __forceinline__ T lap(const T *__restrict__ u, long long p, const int *xp,
                                  const int *xm, long long sy, long long sz, T ix2, T iy2, T iz2) {
-  const T u0 = u[p];
-#define UX(m) (u[p + xp[(m) - 1]] + u[p + xm[(m) - 1]])
-#define UY(m) (u[p + (m) * sy] + u[p - (m) * sy])
-#define UZ(m) (u[p + (m) * sz] + u[p - (m) * sz])
-  if constexpr (ORD == 2) {
-    return (UX(1) - T(2) * u0) * ix2 + (UY(1) - T(2) * u0) * iy2 + (UZ(1) - T(2) * u0) * iz2;
-  } else if constexpr (ORD == 4) {
-    const T ax = -UX(2) + T(16) * UX(1) - T(30) * u0;
-    const T ay = -UY(2) + T(16) * UY(1) - T(30) * u0;
-    const T az = -UZ(2) + T(16) * UZ(1) - T(30) * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(12));
-  } else if constexpr (ORD == 6) {
-    const T ax = T(2) * UX(3) - T(27) * UX(2) + T(270) * UX(1) - T(490) * u0;
-    const T ay = T(2) * UY(3) - T(27) * UY(2) + T(270) * UY(1) - T(490) * u0;
-    const T az = T(2) * UZ(3) - T(27) * UZ(2) + T(270) * UZ(1) - T(490) * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(180));
-  } else {
-    const T ax = T(-9) * UX(4) + T(128) * UX(3) - T(1008) * UX(2) + T(8064) * UX(1) - T(14350) * u0;
-    const T ay = T(-9) * UY(4) + T(128) * UY(3) - T(1008) * UY(2) + T(8064) * UY(1) - T(14350) * u0;
-    const T az = T(-9) * UZ(4) + T(128) * UZ(3) - T(1008) * UZ(2) + T(8064) * UZ(1) - T(14350) * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(5040));
+  constexpr int R = ORD / 2;
+  T sx[4] = {0, 0, 0, 0}, syv[4] = {0, 0, 0, 0}, szv[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int m = 1; m <= R; ++m) {
+    sx[m - 1] = u[p + xp[m - 1]] + u[p + xm[m - 1]];
+    syv[m - 1] = u[p + m * sy] + u[p - m * sy];
+    szv[m - 1] = u[p + m * sz] + u[p - m * sz];
   }
-#undef UX
-#undef UY
-#undef UZ
+  const T u0 = u[p];
+  return star_sum<ORD, T>(star<ORD, T>(u0, sx[0], sx[1], sx[2], sx[3]),
+                          star<ORD, T>(u0, syv[0], syv[1], syv[2], syv[3]),
+                          star<ORD, T>(u0, szv[0], szv[1], szv[2], szv[3]), ix2, iy2, iz2);
 }
 
 // Laplacian of a box at (i, j, k), storage index p
@@ -96,13 +110,13 @@ __device__ __forceinline__ void nonlinear(double lapu, double dulap, double b, d
   const double inv4 = inv2 * inv2;
   const double inv7 = inv4 * inv2 * inv;
   const double inv8 = inv4 * inv4;
-  N = lapu + b * inv7;
-  dN = dulap - 7.0 * b * inv8;
+  N = fma(b, inv7, lapu);
+  dN = fma(-7.0 * b, inv8, dulap);
   if constexpr (EQ == 1) {
     const double c2 = cu * cu;
     const double c4 = c2 * c2;
-    N += dd * cu + ee * (c4 * cu);
-    dN += dd + 5.0 * ee * c4;
+    N = fma(ee, c4 * cu, fma(dd, cu, N));
+    dN = fma(5.0 * ee, c4, dN + dd);
   }
 }
 
@@ -201,31 +215,18 @@ __device__ __forceinline__ void st(void *base, unsigned off, T v) {
 template <int ORD, typename T>
 __device__ __forceinline__ T lapb(const void *u, unsigned o, const int *xp, const int *xm,
                                   unsigned sy, unsigned sz, T ix2, T iy2, T iz2) {
-  const T u0 = ld<T>(u, o);
-#define UX(m) (ld<T>(u, o + xp[(m) - 1]) + ld<T>(u, o + xm[(m) - 1]))
-#define UY(m) (ld<T>(u, o + (m) * sy) + ld<T>(u, o - (m) * sy))
-#define UZ(m) (ld<T>(u, o + (m) * sz) + ld<T>(u, o - (m) * sz))
-  if constexpr (ORD == 2) {
-    return (UX(1) - T(2) * u0) * ix2 + (UY(1) - T(2) * u0) * iy2 + (UZ(1) - T(2) * u0) * iz2;
-  } else if constexpr (ORD == 4) {
-    const T ax = -UX(2) + T(16) * UX(1) - T(30) * u0;
-    const T ay = -UY(2) + T(16) * UY(1) - T(30) * u0;
-    const T az = -UZ(2) + T(16) * UZ(1) - T(30) * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(12));
-  } else if constexpr (ORD == 6) {
-    const T ax = T(2) * UX(3) - T(27) * UX(2) + T(270) * UX(1) - T(490) * u0;
-    const T ay = T(2) * UY(3) - T(27) * UY(2) + T(270) * UY(1) - T(490) * u0;
-    const T az = T(2) * UZ(3) - T(27) * UZ(2) + T(270) * UZ(1) - T(490) * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(180));
-  } else {
-    const T ax = T(-9) * UX(4) + T(128) * UX(3) - T(1008) * UX(2) + T(8064) * UX(1) - T(14350) * u0;
-    const T ay = T(-9) * UY(4) + T(128) * UY(3) - T(1008) * UY(2) + T(8064) * UY(1) - T(14350) * u0;
-    const T az = T(-9) * UZ(4) + T(128) * UZ(3) - T(1008) * UZ(2) + T(8064) * UZ(1) - T(14350) * u0;
-    return (ax * ix2 + ay * iy2 + az * iz2) * (T(1) / T(5040));
+  constexpr int R = ORD / 2;
+  T sx[4] = {0, 0, 0, 0}, syv[4] = {0, 0, 0, 0}, szv[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int m = 1; m <= R; ++m) {
+    sx[m - 1] = ld<T>(u, o + xp[m - 1]) + ld<T>(u, o + xm[m - 1]);
+    syv[m - 1] = ld<T>(u, o + m * sy) + ld<T>(u, o - m * sy);
+    szv[m - 1] = ld<T>(u, o + m * sz) + ld<T>(u, o - m * sz);
   }
-#undef UX
-#undef UY
-#undef UZ
+  const T u0 = ld<T>(u, o);
+  return star_sum<ORD, T>(star<ORD, T>(u0, sx[0], sx[1], sx[2], sx[3]),
+                          star<ORD, T>(u0, syv[0], syv[1], syv[2], syv[3]),
+                          star<ORD, T>(u0, szv[0], szv[1], szv[2], szv[3]), ix2, iy2, iz2);
 }
 
 // one DPP step on a double: lanes of rows in ROW_MASK read lane CTRL's value
@@ -275,21 +276,16 @@ __device__ __forceinline__ void row_jk(ConstBox &B, unsigned r, int &j, int &k) 
   j = R + int(r - kq * nyi);
 }
 
-// colour passes: the thread's point of colour `colour` in the block's rows,
-// i = i_first(row) + t (R+1), which in the colour-split row is slot
-// i_first / (R+1) + t of sub-row i_first mod (R+1).  Returns false when the
-// thread has no point; p is the element index.
+// colour passes: the thread's slot t of local row lr:
+// element index p of its colour point and the sub-row; false when the row
+// is past the block or the slot past the row's colour points
 template <int R>
-__device__ __forceinline__ bool colour_point(const RowBlock &Bk, ConstBox &B, unsigned per_row,
-                                             float inv_per_row, int colour, unsigned &p,
-                                             int &sub) {
+__device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, unsigned lr, int t,
+                                            int colour, unsigned &p, int &sub) {
   constexpr int NC = R + 1;
-  const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
-  const int t = int(threadIdx.x - lr * per_row);
   if (lr >= unsigned(Bk.rows)) return false;
   int j, k;
   row_jk<R>(B, unsigned(Bk.row0) + lr, j, k);
-  // j + k + R < 3 kMaxAxisPoints keeps the dividend positive
   const int first = R + int(unsigned(colour + NC * 3 * kMaxAxisPoints - j - k - R) % NC);
   if (first + t * NC >= B.n[0] - R) return false;
   sub = first % NC;
@@ -332,6 +328,8 @@ __device__ __forceinline__ void xoffs_bytes(int sub, int rl, int *xp, int *xm) {
 // smoother: one colour of one Newton-Gauss-Seidel sweep on the rows of
 // every listed block
 // ------------------------------------------------------------------
+// smoother: one colour of one Newton-Gauss-Seidel sweep on the rows of
+// every listed block; thread (lr, t) owns colour point t of local row lr
 template <int ORD, int EQ>
 __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
                                                         unsigned per_row, float inv_per_row,
@@ -340,10 +338,12 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restri
   constexpr int NC = R + 1;
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
+  const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
+  const int t = int(threadIdx.x - lr * per_row);
   double rmax = 0.0;
   unsigned p;
   int sub;
-  if (colour_point<R>(Bk, B, per_row, inv_per_row, colour, p, sub)) {
+  if (colour_slot<R>(Bk, B, lr, t, colour, p, sub)) {
     int xp[NC - 1], xm[NC - 1];
     xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
     const unsigned o = p * 8u;
@@ -351,10 +351,9 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restri
     const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
                                         B.idx2[0], B.idx2[1], B.idx2[2]);
     const double u0 = ld<double>(u, o);
-    const double cu = ld<double>(B.f[F_C], o) + u0;
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), cu, EQ ? ld<double>(B.f[F_D], o) : 0.0,
-                  EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + u0,
+                  EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
     const double res = N - ld<double>(B.f[F_RHO], o);
     const double un = u0 - res / dN;
     st<double>(u, o, un);
@@ -442,9 +441,11 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__rest
   constexpr int NC = R + 1;
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
+  const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
+  const int t = int(threadIdx.x - lr * per_row);
   unsigned p;
   int sub;
-  if (!colour_point<R>(Bk, B, per_row, inv_per_row, colour, p, sub)) return;
+  if (!colour_slot<R>(Bk, B, lr, t, colour, p, sub)) return;
   int xp[NC - 1], xm[NC - 1];
   xoffs_bytes<NC, 4>(sub, B.rl, xp, xm);
   const unsigned o = p * 4u;
@@ -453,8 +454,9 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__rest
                                     float(B.idx2[0]), float(B.idx2[1]), float(B.idx2[2]));
   const float q = ld<float>(B.g[G_Q], o);
   const float ep = ld<float>(e, o);
-  const float res = le + q * ep - ld<float>(B.g[G_R], o);
-  st<float>(e, o, ep - res / (float(B.dulap) + q));
+  const float res = fmaf(q, ep, le) - ld<float>(B.g[G_R], o);
+  // float32 relaxation of a correction: the hardware reciprocal suffices
+  st<float>(e, o, fmaf(-res, __builtin_amdgcn_rcpf(float(B.dulap) + q), ep));
 }
 
 template <int ORD>
