@@ -48,6 +48,14 @@ extern "C" {
 #define OLL_PREC_FP64 0
 #define OLL_PREC_MIXED 1
 
+/* stopping rule of the V-cycle loop.  REFERENCE (elliptic.cc:1330): go on
+ * while res > tol and res < the previous cycle's res, i.e. stop at the first
+ * cycle that does not lower the residual.  TOLERANCE: go on while res > tol,
+ * stopping only at max_cycles or after `patience` cycles without a new
+ * lowest residual (for streams whose solves must reach tol). */
+#define OLL_STOP_REFERENCE 0
+#define OLL_STOP_TOLERANCE 1
+
 /* field ids (reference field names, operator.cc:103-124,509-567) */
 #define OLL_F_U 0
 #define OLL_F_B 1
@@ -83,8 +91,10 @@ typedef struct oll_problem {
   int robin_n;          /* -Bn fall-off power (reference: 1) */
   int ps_case;          /* -psc */
   double ps_r0, ps_sigma, ps_phi0, ps_a2;  /* -r0 -sigma -phi0 -a2 */
-  int max_cycles;       /* 0: reference stopping rule only */
+  int max_cycles;       /* > 0: also stop after this many V-cycles */
   int precision;        /* OLL_PREC_* */
+  int stop_rule;        /* OLL_STOP_* */
+  int patience;         /* OLL_STOP_TOLERANCE: stop after this many cycles without a new best */
 } oll_problem;
 
 typedef struct oll_hier oll_hier;
@@ -201,6 +211,11 @@ int oll_engine_pop_done(oll_engine *e, long long *tag, oll_result *res, double *
  * destroyed with oll_hier_destroy.  Its HBM leaves the engine's budget. */
 int oll_engine_pop_hier(oll_engine *e, long long *tag, oll_result *res, oll_hier **out);
 int oll_engine_stats(const oll_engine *e, size_t *bytes_in_use, int *active, int *done_pending);
+/* host wall time since creation (ms) spent building V-cycle schedules,
+ * running cycles (launches to norms back on the host) and admitting
+ * requests; schedule rebuilds; engine steps */
+int oll_engine_profile(const oll_engine *e, double *build_ms, double *run_ms, double *admit_ms,
+                       long long *rebuilds, long long *cycles);
 /* smoother launch timing over the engine's V-cycles, as oll_set_kernel_timing
  * (while on, V-cycles are issued launch by launch instead of graph replays);
  * bytes are algorithmic: 40 B (FP64) or 16 B (FP32: e, q, r read, e written)

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -288,6 +289,8 @@ int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
   Hier &H = h->H;
   H.cycle = 0;
   H.res = H.res_prev = 1e12;
+  H.best = 1e300;
+  H.best_cycle = 0;
   H.done = false;
   H.status = 0;
   H.hist.clear();
@@ -401,7 +404,18 @@ struct oll_engine {
   bool dirty = true;
   long long cycles_run = 0;
   Timing timing;
+  // host wall time (ms) in schedule builds, cycle runs (launch to norms on
+  // the host), admissions; rebuild count
+  double build_ms = 0, run_ms = 0, admit_ms = 0;
+  long long rebuilds = 0;
 };
+
+namespace {
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+}  // namespace
 
 extern "C" {
 
@@ -434,6 +448,12 @@ void oll_engine_destroy(oll_engine *e) {
 
 int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag) {
   if (!e || !p) return fail(OLL_EINVAL, "null argument");
+  const double t0 = now_ms();
+  struct AdmitClock {
+    oll_engine *e;
+    double t0;
+    ~AdmitClock() { e->admit_ms += now_ms() - t0; }
+  } clock{e, t0};
   std::string err;
   size_t need = 0;
   int rc = problem_footprint(*p, need, err);
@@ -479,19 +499,32 @@ int oll_engine_step(oll_engine *e, int *active_after) {
   std::string err;
   int rc;
   if (e->dirty) {
-    for (auto &kv : e->sched) release_schedule(kv.second);
-    e->sched.clear();
+    const double t0 = now_ms();
     std::map<std::pair<int, int>, std::vector<Hier *>> cls;
     for (auto &m : e->active) cls[{m.H->prob.nu1, m.H->prob.nu2}].push_back(m.H.get());
+    // schedules keep their device buffers across rebuilds
+    for (auto it = e->sched.begin(); it != e->sched.end();) {
+      if (cls.count(it->first)) {
+        it->second.clear();
+        ++it;
+      } else {
+        release_schedule(it->second);
+        it = e->sched.erase(it);
+      }
+    }
     for (auto &kv : cls)
       if ((rc = build_vcycle(kv.second, e->sched[kv.first], err))) return fail(rc, err);
     e->dirty = false;
+    ++e->rebuilds;
+    e->build_ms += now_ms() - t0;
   }
+  const double t1 = now_ms();
   for (auto &kv : e->sched)
     if ((rc = run_schedule(kv.second, e->stream, &e->timing, e->graphs, err))) return fail(rc, err);
   for (auto &m : e->active)
     if ((rc = read_norms(*m.H, e->stream, err))) return fail(rc, err);
   CAPI_CHECK(hipStreamSynchronize(e->stream));
+  e->run_ms += now_ms() - t1;
   ++e->cycles_run;
   std::vector<oll_engine::Member> still;
   for (auto &m : e->active) {
@@ -583,6 +616,17 @@ int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, dou
   if (launches) *launches = int(e->timing.used / 2);
   if (total_ms) *total_ms = tot;
   if (bytes) *bytes = e->timing.bytes;
+  return OLL_OK;
+}
+
+int oll_engine_profile(const oll_engine *e, double *build_ms, double *run_ms, double *admit_ms,
+                       long long *rebuilds, long long *cycles) {
+  if (!e) return fail(OLL_EINVAL, "null handle");
+  if (build_ms) *build_ms = e->build_ms;
+  if (run_ms) *run_ms = e->run_ms;
+  if (admit_ms) *admit_ms = e->admit_ms;
+  if (rebuilds) *rebuilds = e->rebuilds;
+  if (cycles) *cycles = e->cycles_run;
   return OLL_OK;
 }
 

@@ -54,6 +54,10 @@ int validate(const oll_problem &p, std::string &err) {
   if (p.eq == OLL_EQ_PUNCTURES)
     for (int i = 0; i < p.num_bh; ++i)
       if (p.bh[i].m <= 0) return err = "Wrong punture mass parameters", OLL_EINVAL;
+  if (p.stop_rule != OLL_STOP_REFERENCE && p.stop_rule != OLL_STOP_TOLERANCE)
+    return err = "unknown stopping rule", OLL_EINVAL;
+  if (p.stop_rule == OLL_STOP_TOLERANCE && p.max_cycles <= 0 && p.patience <= 0)
+    return err = "OLL_STOP_TOLERANCE needs max_cycles or patience", OLL_EINVAL;
   if (p.precision != OLL_PREC_FP64 && p.precision != OLL_PREC_MIXED)
     return err = "precision must be OLL_PREC_FP64 or OLL_PREC_MIXED", OLL_EINVAL;
   return OLL_OK;
@@ -157,9 +161,10 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
     if (layout_size(n, b.R) * (long long)sizeof(double) >= kMaxFieldBytes)
       return err = "a box field exceeds 4 GiB", OLL_EINVAL;
-    // point kernels split point indices with 24-bit exact float division
-    if ((long long)n[0] * n[1] * n[2] >= (1LL << 24))
-      return err = "a box exceeds 2^24 points", OLL_EINVAL;
+    // the ghost fill splits shell point indices with 24-bit float division
+    const long long inner = (long long)(n[0] - 2 * b.R) * (n[1] - 2 * b.R) * (n[2] - 2 * b.R);
+    if ((long long)n[0] * n[1] * n[2] - inner >= (1LL << 24))
+      return err = "a box exceeds 2^24 ghost-shell points", OLL_EINVAL;
     off.push_back(total);
     total += (layout_size(n, b.R) + 31) & ~31LL;  // 256-byte aligned boxes
   }
@@ -299,6 +304,8 @@ int hier_reset(Hier &H, hipStream_t s, std::string &err) {
   ENG_CHECK(hipStreamSynchronize(s));
   H.cycle = 0;
   H.res = H.res_prev = 1e12;
+  H.best = 1e300;
+  H.best_cycle = 0;
   H.done = false;
   H.status = 0;
   H.hist.clear();
@@ -336,6 +343,7 @@ void Schedule::clear() {
   launches.clear();
   blob.clear();
   uploaded = false;
+  runs = 0;
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   exec = nullptr;
@@ -380,17 +388,13 @@ struct Builder {
     }
   }
 
+  // clears every member's norm slots, each with its own slot count
   void zero(const std::vector<Hier *> &M) {
-    std::vector<unsigned long long *> ptrs;
-    int count = 0;
-    for (Hier *H : M) {
-      ptrs.push_back(H->norms);
-      count = std::max(count, H->nslots * kNormWords);
-    }
+    std::vector<ZeroItem> items;
+    for (Hier *H : M) items.push_back(ZeroItem{H->norms, (long long)H->nslots * kNormWords});
     Launch L{Launch::ZERO};
-    L.off = S.add(ptrs);
-    L.n = int(ptrs.size());
-    L.total = count;
+    L.off = S.add(items);
+    L.n = int(items.size());
     S.launches.push_back(L);
   }
 
@@ -512,9 +516,11 @@ struct Builder {
       const DevBox &B = H.hbox[size_t(f + t)];
       const Hier::XRef r = X[size_t(k)][size_t(t)];
       const long long np = shell ? shell_pts(B) : npts(B);
+      const long long plane = (long long)B.n[0] * B.n[1];
       for (long long q0 = 0; q0 < np; q0 += kPtThreads)
         out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, mg, unsigned(q0),
-                              unsigned(std::min<long long>(kPtThreads, np - q0))});
+                              unsigned(std::min<long long>(kPtThreads, np - q0)),
+                              int(q0 % B.n[0]), int((q0 % plane) / B.n[0]), int(q0 / plane)});
     }
   }
 
@@ -678,7 +684,7 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
     }
     switch (L.kind) {
       case Launch::ZERO:
-        launch_zero_lists(s, reinterpret_cast<unsigned long long *const *>(items), L.n, int(L.total));
+        launch_zero_lists(s, reinterpret_cast<const ZeroItem *>(items), L.n);
         break;
       case Launch::ROBIN:
         launch_robin(s, reinterpret_cast<const RobinItem *>(items), L.n, int(L.total), L.mode);
@@ -734,7 +740,7 @@ int run_schedule(Schedule &S, hipStream_t s, Timing *t, bool graphs, std::string
   int rc;
   if (!S.uploaded && (rc = upload_schedule(S, s, err))) return rc;
   const bool timed = t && t->on;
-  if (!graphs || timed) return issue(S, s, t, err);
+  if (!graphs || timed || S.runs++ == 0) return issue(S, s, t, err);
   if (!S.exec) {
     ENG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     rc = issue(S, s, nullptr, err);
@@ -798,7 +804,16 @@ int apply_cycle(Hier &H, std::string &err) {
       err = "NAN check error in the smoother";
       return OLL_ENAN;
     }
-  if (!(H.res > H.eps_solv && H.res < H.res_prev)) H.done = true;
+  if (H.res < H.best) {
+    H.best = H.res;
+    H.best_cycle = H.cycle;
+  }
+  if (H.prob.stop_rule == OLL_STOP_TOLERANCE) {
+    if (!(H.res > H.eps_solv)) H.done = true;
+    if (H.prob.patience > 0 && H.cycle - H.best_cycle >= H.prob.patience) H.done = true;
+  } else if (!(H.res > H.eps_solv && H.res < H.res_prev)) {
+    H.done = true;
+  }
   if (H.prob.max_cycles > 0 && H.cycle >= H.prob.max_cycles) H.done = true;
   return OLL_OK;
 }

@@ -48,6 +48,8 @@ struct Hier {
   // V-cycle state
   int cycle = 0;
   double res = 1e12, res_prev = 1e12;
+  double best = 1e300;           // lowest top-level residual so far
+  int best_cycle = 0;
   bool done = false;
   int status = 0;                // 0 ok, <0 error code
   std::vector<double> hist;      // (cycle, pre-smoothing norm) pairs
@@ -96,6 +98,7 @@ struct Schedule {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool uploaded = false;
+  int runs = 0;  // times run since built: graphs are captured on the second run
 
   template <typename T>
   size_t add(const std::vector<T> &items) {
@@ -122,7 +125,9 @@ int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string 
 enum Stage { STAGE_SMOOTH = 0, STAGE_RESIDUAL, STAGE_ROBIN, STAGE_BOUNDARY_OP };
 
 int upload_schedule(Schedule &S, hipStream_t s, std::string &err);
-// runs the schedule; with graphs = true it captures once and replays
+// runs the schedule; with graphs = true it is captured as a graph on its
+// second run (a schedule run once is cheaper to issue directly) and replayed
+// from then on
 int run_schedule(Schedule &S, hipStream_t s, Timing *t, bool graphs, std::string &err);
 void release_schedule(Schedule &S);
 

@@ -703,17 +703,22 @@ __device__ __forceinline__ unsigned boff(ConstBox &S, int i, int j, int k) {
   return (unsigned(k) * unsigned(S.ps) + unsigned(j) * unsigned(S.rs) + unsigned(ioffc(S, i))) * 8u;
 }
 
-// the block's target point in lexicographic order; false past its end
+// the block's target point in lexicographic order (point q0 + thread,
+// stepped from the block's first point so that only small offsets are
+// divided); false past the block's end
 __device__ __forceinline__ bool pt_lex(const PtBlock &Bk, ConstBox &T, int &i, int &j, int &k) {
   if (threadIdx.x >= Bk.nq) return false;
-  const unsigned q = Bk.q0 + threadIdx.x;
-  const unsigned n0 = unsigned(T.n[0]), plane = n0 * unsigned(T.n[1]);
-  const unsigned kq = fast_div(q, plane, T.in01);
-  const unsigned r = q - kq * plane;
-  const unsigned jq = fast_div(r, n0, T.in0);
-  k = int(kq);
-  j = int(jq);
-  i = int(r - jq * n0);
+  const unsigned n0 = unsigned(T.n[0]), n1 = unsigned(T.n[1]);
+  const unsigned x = unsigned(Bk.i0) + threadIdx.x;
+  const unsigned dj = fast_div(x, n0, T.in0);
+  unsigned jj = unsigned(Bk.j0) + dj, kk = unsigned(Bk.k0);
+  while (jj >= n1) {  // a block spans at most a few rows: n1 >= 3
+    jj -= n1;
+    ++kk;
+  }
+  i = int(x - dj * n0);
+  j = int(jj);
+  k = int(kk);
   return true;
 }
 
@@ -1441,10 +1446,10 @@ __global__ void k_zero_u64(unsigned long long *p, long long n) {
   if (i < n) p[i] = 0ull;
 }
 
-// zero `count` words at each of n pointers (one block per pointer)
-__global__ void k_zero_lists(unsigned long long *const *ptrs, int count) {
-  unsigned long long *p = ptrs[blockIdx.x];
-  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0ull;
+// clear item blockIdx.x: its own word count, never another member's
+__global__ void k_zero_lists(const ZeroItem *items) {
+  const ZeroItem it = items[blockIdx.x];
+  for (long long i = threadIdx.x; i < it.count; i += blockDim.x) it.p[i] = 0ull;
 }
 
 inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1) / b); }
@@ -1599,9 +1604,9 @@ bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_
   return true;
 }
 
-void launch_zero_lists(hipStream_t s, unsigned long long *const *ptrs, int n, int count) {
+void launch_zero_lists(hipStream_t s, const ZeroItem *items, int n) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(256), 0, s, ptrs, count);
+  hipLaunchKernelGGL(k_zero_lists, dim3(n), dim3(256), 0, s, items);
 }
 
 void kernels_init() {

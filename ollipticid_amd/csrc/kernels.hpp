@@ -117,6 +117,7 @@ struct PtBlock {
   int mg;              // multigrid level (Robin shell, "all" coverage)
   unsigned q0;
   unsigned nq;
+  int i0, j0, k0;      // lexicographic blocks: coordinates of point q0
 };
 constexpr int kPtThreads = 256;
 
@@ -199,7 +200,12 @@ void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int or
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
                          int eq, double eps, int max_iter);
 void launch_zero_u64(hipStream_t s, unsigned long long *p, long long n);
-void launch_zero_lists(hipStream_t s, unsigned long long *const *ptrs, int n, int count);
+// words to clear at one address (each member's own count: the arrays differ)
+struct ZeroItem {
+  unsigned long long *p;
+  long long count;
+};
+void launch_zero_lists(hipStream_t s, const ZeroItem *items, int n);
 // one-time kernel attributes (dynamic LDS of the coarse solve); call before
 // any graph capture
 void kernels_init();

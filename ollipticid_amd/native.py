@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libolliptic_amd.so")
 MAX_BH = 16
 EQ_PUNCTURES, EQ_PUNCTURES_SCALAR = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+STOP_REFERENCE, STOP_TOLERANCE = 0, 1
 F_U, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW = range(10)
 NFIELDS = 10
 ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EDEVICE", -4: "ENAN", -5: "ENOMEM", -6: "EBUSY"}
@@ -33,6 +34,7 @@ class Problem(C.Structure):
         ("ps_case", C.c_int), ("ps_r0", C.c_double), ("ps_sigma", C.c_double),
         ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
         ("max_cycles", C.c_int), ("precision", C.c_int),
+        ("stop_rule", C.c_int), ("patience", C.c_int),
     ]
 
 
@@ -92,6 +94,9 @@ SIGNATURES = {
     "oll_engine_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "oll_engine_kernel_timing": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                            C.POINTER(C.c_double)]),
+    "oll_engine_profile": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                     C.POINTER(C.c_longlong)]),
     "oll_engine_stats": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]),
     "oll_last_norms": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
@@ -441,6 +446,15 @@ class Engine:
         n, ms, b = C.c_int(), C.c_double(), C.c_double()
         _check(self.L.oll_engine_kernel_timing(self.h, C.byref(n), C.byref(ms), C.byref(b)))
         return n.value, ms.value, b.value
+
+    def profile(self):
+        """host ms in schedule builds, cycle runs, admissions; rebuilds; steps"""
+        b, r, a = C.c_double(), C.c_double(), C.c_double()
+        nb, nc = C.c_longlong(), C.c_longlong()
+        _check(self.L.oll_engine_profile(self.h, C.byref(b), C.byref(r), C.byref(a), C.byref(nb),
+                                         C.byref(nc)))
+        return {"build_ms": b.value, "run_ms": r.value, "admit_ms": a.value,
+                "rebuilds": nb.value, "steps": nc.value}
 
     def stats(self):
         b, a, d = C.c_size_t(), C.c_int(), C.c_int()
