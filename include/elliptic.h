@@ -33,9 +33,9 @@ class domain {
   double Get_Xi() const { return b.origin[0]; }
   double Get_Yi() const { return b.origin[1]; }
   double Get_Zi() const { return b.origin[2]; }
-  double Get_Xf() const { return Get_x(b.n[0] - 1); }
-  double Get_Yf() const { return Get_y(b.n[1] - 1); }
-  double Get_Zf() const { return Get_z(b.n[2] - 1); }
+  double Get_Xf() const { return b.end[0]; }
+  double Get_Yf() const { return b.end[1]; }
+  double Get_Zf() const { return b.end[2]; }
   double Get_x(int i) const { return b.origin[0] + double(i) * b.d[0]; }
   double Get_y(int j) const { return b.origin[1] + double(j) * b.d[1]; }
   double Get_z(int k) const { return b.origin[2] + double(k) * b.d[2]; }
@@ -45,8 +45,26 @@ class domain {
   size_t Get_order() const { return size_t(b.order); }
   size_t Get_level() const { return size_t(b.level); }
   size_t Get_buffer() const { return size_t(b.ghost); }
+  // domain.cc:1493-1675 Get_i/Get_j/Get_k: the first grid index whose
+  // coordinate is at or above x (within 3 DBL_EPSILON); 1 at or below the
+  // box, n - 1 at or above it
+  size_t Get_i(double x) const { return index_of(0, x); }
+  size_t Get_j(double y) const { return index_of(1, y); }
+  size_t Get_k(double z) const { return index_of(2, z); }
+  // domain.h:767-851
+  bool x_is_in(double x) const { return Get_Xi() <= x && x <= Get_Xf(); }
+  bool y_is_in(double y) const { return Get_Yi() <= y && y <= Get_Yf(); }
+  bool z_is_in(double z) const { return Get_Zi() <= z && z <= Get_Zf(); }
+  bool is_in(double x, double y, double z) const;
+  // strictly inside the box less its buffer of order/2 points
+  bool is_in_nb(double x, double y, double z) const;
+  // domain.cc:1947-1954: this box lies strictly inside D on every axis
+  bool operator<(const domain &D) const;
 
  private:
+  size_t index_of(int a, double x) const;
+  double lo(int a) const { return b.origin[a]; }
+  double hi(int a) const { return b.end[a]; }
   oll_box_desc b{};
 };
 
@@ -54,6 +72,7 @@ class ffunction {
  public:
   ffunction(oll_hier *h, int box, int field, const domain &d) : h_(h), box_(box), field_(field), d_(d) {}
   const domain *Get_domain() const { return &d_; }
+  int Field() const { return field_; }
   size_t Get_nx() const { return d_.Get_Nx(); }
   size_t Get_ny() const { return d_.Get_Ny(); }
   size_t Get_nz() const { return d_.Get_Nz(); }
@@ -82,13 +101,40 @@ class multigrid {
   size_t Get_dom_index(size_t level, size_t box) const;
   const domain &Get_domain(size_t level, size_t box) const;
   ffunction Get_ffunction(size_t level, size_t box, const std::string &name) const;
+  // multigrid.cc:35-48: the box that holds every other one
+  size_t Get_l0() const { return l0_; }
+  // multigrid.cc:223-241: the last box, over levels and their boxes in
+  // order, whose interior less its buffer holds the point; box l0 otherwise
+  size_t Box_Of(double x, double y, double z) const;
+  // Lagrange interpolation (order + 1 points per axis) of field `name` at
+  // xyz[3q..3q+2] in box[q], one device launch for all points
+  std::vector<double> Interpol(const std::string &name, const std::vector<int> &box,
+                               const std::vector<double> &xyz, size_t order) const;
+  // multigrid.h:312-337 + ffunction.cc:819-1160: per level box a 1-D cut
+  // through its middle (or through x1, x2 where the box holds them) to
+  // <dir><name>-l<l>b<b>-0.x/.y/.z
+  void Print_1D(const std::string &dir, const std::string &name, size_t l, int cut, double x1,
+                double x2, bool interpolar, size_t order) const;
+  // multigrid.cc:151-548: a cut over [-LD/2, LD/2] with step _dx (LD/1000
+  // when _dx <= 0) through the composite grid to <dir><name>.X/.Y/.Z
+  void Print_Interpol_1D(const std::string &dir, const std::string &name, double LD, double _dx,
+                         int cut, double x1, double x2, size_t order) const;
+  // multigrid.cc:554-958: a plane over [-LD1/4, LD1/4] x [-LD2/4, LD2/4]
+  // (steps LD/200 when _dx <= 0) to <dir><name>.YZ/.XZ/.XY
+  void Print_Interpol_2D(const std::string &dir, const std::string &name, double LD1,
+                         double LD2, double _dx1, double _dx2, int cut, double x1,
+                         size_t order) const;
 
  private:
   oll_hier *h_;
   int levels_ = 0;
+  size_t l0_ = 0;
   std::vector<int> first_, count_;
   std::vector<domain> dom_;
 };
+
+// cut directions (ffunction.h `cut`)
+enum { CUT_X = 0, CUT_Y = 1, CUT_Z = 2 };
 
 class elliptic {
  public:
@@ -122,7 +168,8 @@ class elliptic {
   oll_hier *h_ = nullptr;
   multigrid mg_;
   std::string dir_;
-  bool print_1d_ = true, print_3d_ = false, quiet_ = false;
+  bool print_1d_ = true, print_3d_ = false, print_interpol_ = true, interpol_output_ = true;
+  bool quiet_ = false;
   std::vector<double> adm_;
   int cycles_ = 0;
   double res_ = 0, tol_ = 0;

@@ -107,6 +107,9 @@ typedef struct oll_box_desc {
   double origin[3];     /* coordinate of index 0 per axis */
   double d[3];          /* spacing */
   double centre[3];
+  double end[3];        /* coordinate of the last point as the reference's domain keeps
+                           it (domain.cc:326-351); may differ from origin + (n-1) d in
+                           the last bit */
 } oll_box_desc;
 
 typedef struct oll_result {
@@ -163,6 +166,15 @@ int oll_last_norms(const oll_hier *h, double *pre, double *post, int nlevels);
 int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap);
 /* elliptic.cc:2447-2611 Compute_ADM_Mass: per puncture and per box. */
 int oll_adm_mass(oll_hier *h, double *puncture, double *surface);
+/* ffunction::Interpol_In with LAGRANGE of order ord (interpolation.cc:189-303,
+ * set by ffunction::Set_Interpol, interpolation.cc:32-60): field at npts
+ * points, point q = (xyz[3q], xyz[3q+1], xyz[3q+2]) taken from box box[q]
+ * (ord + 1 stencil points per axis, shifted to stay inside the box; a point
+ * within 3 DBL_EPSILON of a grid point on every axis returns that grid
+ * value).  Host arrays; ord 1..9.
+ * The multigrid writers (multigrid.cc:151-958) choose the box per point. */
+int oll_interp(oll_hier *h, int field, int npts, const int *box, const double *xyz, int ord,
+               double *out);
 
 /* Device time of the last oll_solve (HIP events on the handle's stream). */
 int oll_last_solve_ms(const oll_hier *h, double *ms);

@@ -41,7 +41,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     cli = [os.path.join(CSRC, "cli", f) for f in ("interface.cpp", "elliptic.cpp", "main.cpp")]
     if all(os.path.exists(f) for f in cli):
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(REPO, "include")] + cli +
+        _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + os.path.join(REPO, "include")] + cli +
              ["-o", EXE, "-L" + HERE, "-lolliptic_amd", "-Wl,-rpath,$ORIGIN"])
     return LIB
 

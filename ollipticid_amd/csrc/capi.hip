@@ -163,6 +163,7 @@ static void fill_desc(const Box &b, oll_box_desc *out) {
     out->origin[a] = b.lo[a];
     out->d[a] = b.d[a];
     out->centre[a] = b.c[a];
+    out->end[a] = b.hi[a];
   }
 }
 
@@ -381,6 +382,20 @@ int oll_adm_mass(oll_hier *h, double *puncture, double *surface) {
   if (!h) return fail(OLL_EINVAL, "null handle");
   std::string err;
   const int rc = adm_mass(h->H, h->stream, puncture, surface, err);
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
+int oll_interp(oll_hier *h, int field, int npts, const int *box, const double *xyz, int ord,
+               double *out) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  if (npts < 0 || (npts > 0 && (!box || !xyz || !out))) return fail(OLL_EINVAL, "null argument");
+  if (field < 0 || field >= NFIELDS) return fail(OLL_EINVAL, "field out of range");
+  if (ord < 1 || ord > 9) return fail(OLL_EINVAL, "interpolation order must be 1..9");
+  const int nb = int(h->H.hbox.size());
+  for (int q = 0; q < npts; ++q)
+    if (box[q] < 0 || box[q] >= nb) return fail(OLL_EINVAL, "box index out of range");
+  std::string err;
+  const int rc = interp_points(h->H, h->stream, field, npts, box, xyz, ord, out, err);
   return rc ? fail(rc, err) : OLL_OK;
 }
 

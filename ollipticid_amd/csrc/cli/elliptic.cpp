@@ -7,6 +7,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <limits>
 #include <sstream>
 
 namespace olliptic {
@@ -31,6 +32,43 @@ void ffunction::Store() {
     throw std::runtime_error(oll_last_error());
 }
 
+size_t domain::index_of(int a, double x) const {
+  const int np = b.n[a];
+  if (!(lo(a) < x && x < hi(a))) return lo(a) >= x ? 1 : size_t(np - 1);
+  int inf = 0, sup = np;
+  int i = (np % 2 == 0) ? np / 2 : (np - 1) / 2;
+  do {
+    const double xi = lo(a) + double(i) * b.d[a];
+    if (xi > x || std::fabs(xi - x) <= 3 * std::numeric_limits<double>::epsilon())
+      sup = i;
+    else
+      inf = i;
+    i = ((sup + inf) % 2 == 0) ? (sup + inf) / 2 : (sup + inf - 1) / 2;
+  } while (inf + 1 < sup);
+  return size_t(i + 1);
+}
+
+bool domain::is_in(double x, double y, double z) const {
+  const double p[3] = {x, y, z};
+  for (int a = 0; a < 3; ++a)
+    if (!(lo(a) - b.d[a] * 0.1 <= p[a] && p[a] <= hi(a) + b.d[a] * 0.1)) return false;
+  return true;
+}
+
+bool domain::is_in_nb(double x, double y, double z) const {
+  const double p[3] = {x, y, z};
+  const double w = double(b.ghost) - 0.1;
+  for (int a = 0; a < 3; ++a)
+    if (!(lo(a) + b.d[a] * w < p[a] && p[a] < hi(a) - b.d[a] * w)) return false;
+  return true;
+}
+
+bool domain::operator<(const domain &D) const {
+  for (int a = 0; a < 3; ++a)
+    if (!(D.lo(a) < lo(a) && hi(a) < D.hi(a))) return false;
+  return true;
+}
+
 multigrid::multigrid(oll_hier *h) : h_(h) {
   if (!h) return;
   levels_ = oll_num_levels(h);
@@ -45,6 +83,8 @@ multigrid::multigrid(oll_hier *h) : h_(h) {
     oll_box_info(h, b, &d);
     dom_.emplace_back(d);
   }
+  for (size_t d = 1; d < dom_.size(); ++d)
+    if (dom_[l0_] < dom_[d]) l0_ = d;
 }
 
 size_t multigrid::Get_num_dom(size_t level) const {
@@ -69,6 +109,167 @@ ffunction multigrid::Get_ffunction(size_t level, size_t box, const std::string &
   if (f < 0) throw std::runtime_error("unknown field " + name);
   const size_t ib = Get_dom_index(level, box);
   return ffunction(h_, int(ib), f, dom_[ib]);
+}
+
+size_t multigrid::Box_Of(double x, double y, double z) const {
+  size_t in = Get_dom_index(l0_, 0);
+  for (size_t l = 0; l < Get_levels(); ++l)
+    for (size_t i = 0; i < Get_num_dom(l); ++i) {
+      const size_t ib = Get_dom_index(l, i);
+      if (dom_[ib].is_in_nb(x, y, z)) in = ib;
+    }
+  return in;
+}
+
+std::vector<double> multigrid::Interpol(const std::string &name, const std::vector<int> &box,
+                                        const std::vector<double> &xyz, size_t order) const {
+  const ffunction f = Get_ffunction(0, 0, name);
+  std::vector<double> out(box.size());
+  if (box.empty()) return out;
+  if (oll_interp(h_, f.Field(), int(box.size()), box.data(), xyz.data(), int(order),
+                 out.data()) != OLL_OK)
+    throw std::runtime_error(oll_last_error());
+  return out;
+}
+
+namespace {
+
+// point list with the box each point is read from
+struct Points {
+  std::vector<int> box;
+  std::vector<double> xyz;
+  void add(size_t b, double x, double y, double z) {
+    box.push_back(int(b));
+    xyz.insert(xyz.end(), {x, y, z});
+  }
+};
+
+void open_sci(std::ofstream &f, const std::string &path, int digits) {
+  f.open(path, std::ios::app);
+  f.setf(std::ios_base::scientific, std::ios_base::floatfield);
+  f.precision(digits);
+}
+
+}  // namespace
+
+void multigrid::Print_1D(const std::string &dir, const std::string &name, size_t l, int c,
+                         double x1, double x2, bool interpolar, size_t order) const {
+  static const char *ext[3] = {".x", ".y", ".z"};
+  l = l < Get_levels() ? l : Get_levels() - 1;
+  for (size_t ib = 0; ib < Get_num_dom(l); ++ib) {
+    const size_t b = Get_dom_index(l, ib);
+    const domain &D = dom_[b];
+    const size_t n[3] = {D.Get_Nx(), D.Get_Ny(), D.Get_Nz()};
+    // ffunction.cc:838-844: the middle point, then the two fixed axes
+    size_t idx[3];
+    for (int a = 0; a < 3; ++a) idx[a] = (n[a] % 2 == 0) ? n[a] / 2 : (n[a] - 1) / 2;
+    const int a1 = c == CUT_X ? 1 : 0, a2 = c == CUT_Z ? 1 : 2;
+    double p[3] = {D.Get_x(int(idx[0])), D.Get_y(int(idx[1])), D.Get_z(int(idx[2]))};
+    const double want[2] = {x1, x2};
+    const int fixed[2] = {a1, a2};
+    for (int t = 0; t < 2; ++t) {
+      const int a = fixed[t];
+      const bool in = a == 0 ? D.x_is_in(want[t]) : a == 1 ? D.y_is_in(want[t]) : D.z_is_in(want[t]);
+      if (in && interpolar) {
+        p[a] = want[t];
+      } else if (in) {
+        idx[a] = a == 0 ? D.Get_i(want[t]) : a == 1 ? D.Get_j(want[t]) : D.Get_k(want[t]);
+        p[a] = a == 0 ? D.Get_x(int(idx[a])) : a == 1 ? D.Get_y(int(idx[a])) : D.Get_z(int(idx[a]));
+      }
+    }
+    // values: Interpol_In at (node, p[a1], p[a2]); without interpolation
+    // every point is a grid point and the same call returns its value
+    Points pts;
+    for (size_t i = 0; i < n[c]; ++i) {
+      double q[3] = {p[0], p[1], p[2]};
+      q[c] = c == CUT_X ? D.Get_x(int(i)) : c == CUT_Y ? D.Get_y(int(i)) : D.Get_z(int(i));
+      pts.add(b, q[0], q[1], q[2]);
+    }
+    const std::vector<double> v = Interpol(name, pts.box, pts.xyz, order);
+    std::ostringstream fn;
+    fn << dir << name << "-l" << l << "b" << ib << "-" << 0 << ext[c];
+    std::ofstream f;
+    open_sci(f, fn.str(), 12);
+    for (size_t i = 0; i < v.size(); ++i) f << pts.xyz[3 * i + size_t(c)] << "\t" << v[i] << "\t\n";
+    f << "\n";
+  }
+}
+
+void multigrid::Print_Interpol_1D(const std::string &dir, const std::string &name, double LD,
+                                  double _dx, int c, double x1, double x2, size_t order) const {
+  static const char *ext[3] = {".X", ".Y", ".Z"};
+  const domain &D0 = dom_[l0_];
+  double p[3] = {0, 0, 0};
+  const double xi = -0.5 * LD, xf = 0.5 * LD;
+  const double Dx = _dx > 0.0 ? _dx : LD / 1000;
+  // multigrid.cc:204-211, 317-324, 430-437: the fixed coordinates
+  if (c == CUT_X) {
+    if (D0.y_is_in(x1)) p[1] = x1;
+    if (D0.z_is_in(x2)) p[2] = x2;
+  } else if (c == CUT_Y) {
+    if (D0.x_is_in(x1)) p[0] = x1;
+    if (D0.z_is_in(x2)) p[2] = x2;
+  } else {
+    if (D0.x_is_in(x1)) p[0] = x1;
+    if (D0.y_is_in(x2)) p[1] = x2;
+  }
+  Points pts;
+  std::vector<double> t;
+  for (double s = xi; s <= xf; s += Dx) {
+    p[c] = s;
+    // only the x cut is limited to the outer box (multigrid.cc:225)
+    if (c == CUT_X && !D0.is_in(p[0], p[1], p[2])) continue;
+    pts.add(Box_Of(p[0], p[1], p[2]), p[0], p[1], p[2]);
+    t.push_back(s);
+  }
+  const std::vector<double> v = Interpol(name, pts.box, pts.xyz, order);
+  std::ofstream f;
+  open_sci(f, dir + name + ext[c], 18);
+  for (size_t i = 0; i < v.size(); ++i) f << t[i] << "\t" << v[i] << "\n";
+  f << "\n";
+}
+
+void multigrid::Print_Interpol_2D(const std::string &dir, const std::string &name, double LD1,
+                                  double LD2, double _dx1, double _dx2, int c, double x1,
+                                  size_t order) const {
+  static const char *ext[3] = {".YZ", ".XZ", ".XY"};
+  std::cout << "Print 2D interpol dir: " << dir << " name: " << name
+            << " dir+name: " << dir + name + ".XY" << std::endl;
+  const domain &D0 = dom_[l0_];
+  double p[3] = {0, 0, 0};
+  const double s1i = -0.25 * LD1, s2i = -0.25 * LD2, s1f = 0.25 * LD1, s2f = 0.25 * LD2;
+  const double D1 = _dx1 > 0.0 ? _dx1 : LD1 / 200;
+  const double D2 = _dx2 > 0.0 ? _dx2 : LD2 / 200;
+  // the plane's two axes and the fixed one
+  const int a1 = c == CUT_X ? 1 : 0, a2 = c == CUT_Z ? 1 : 2;
+  if (c == CUT_X && D0.x_is_in(x1)) p[0] = x1;
+  if (c == CUT_Y && D0.y_is_in(x1)) p[1] = x1;
+  if (c == CUT_Z && D0.z_is_in(x1)) p[2] = x1;
+  Points pts;
+  std::vector<size_t> row_end;
+  for (double s = s1i; s <= s1f; s += D1) {
+    for (double r = s2i; r <= s2f; r += D2) {
+      p[a1] = s;
+      p[a2] = r;
+      pts.add(Box_Of(p[0], p[1], p[2]), p[0], p[1], p[2]);
+    }
+    row_end.push_back(pts.box.size());
+  }
+  const std::vector<double> v = Interpol(name, pts.box, pts.xyz, order);
+  const std::string out = dir + name + ext[c];
+  std::ofstream f;
+  open_sci(f, out, 18);
+  // multigrid.cc:880-900: the XY plane ends each row with a blank line
+  size_t row = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    f << pts.xyz[3 * i + size_t(a1)] << "\t" << pts.xyz[3 * i + size_t(a2)] << "\t" << v[i] << "\n";
+    if (c == CUT_Z && i + 1 == row_end[row]) {
+      f << "\n";
+      ++row;
+    }
+  }
+  f << "\n";
+  std::cout << "Out file: " << out << std::endl;
 }
 
 // ---------------------------------------------------------------- elliptic
@@ -107,6 +308,8 @@ void elliptic::Make(interface *Oll) {
   tol_ = prob_.tol < 2 * 2.220446049250313e-16 ? 2 * 2.220446049250313e-16 : prob_.tol;
   print_1d_ = Oll->Get_logic_par("print_1d");
   print_3d_ = Oll->Get_logic_par("print_3d");
+  print_interpol_ = Oll->Get_logic_par("print_interpol");
+  interpol_output_ = Oll->Get_logic_par("interpol_output");
   quiet_ = Oll->Get_TOutput() == NO;
 
   const std::string name = Oll->Get_string_par("dir_name");
@@ -238,11 +441,16 @@ void elliptic::Solve() {
   }
 }
 
-// output.cc:624-729: -p3d writes every box (ffunction.cc:411-467 format);
-// the interpolated 1-D/2-D cuts are not produced by this build
-void elliptic::Print_Solution(double, double, double, double) {
+// output.cc:624-729: 1-D cuts of every box (-p1d), every box in 3-D
+// (-p3d, ffunction.cc:411-467 format), composite cuts and the z = 0 plane
+// (-pint), all interpolated with Lagrange order + 1
+void elliptic::Print_Solution(double Dxyz, double Lx, double Ly, double Lz) {
   if (!ok_) return;
   if (!quiet_) std::cout << "Printing..." << std::endl;
+  const size_t ord = size_t(prob_.order) + 1;
+  if (print_1d_)
+    for (size_t l = 0; l < mg_.Get_levels(); ++l)
+      for (int c : {CUT_X, CUT_Y, CUT_Z}) mg_.Print_1D(dir_, "u", l, c, 0, 0, interpol_output_, ord);
   if (print_3d_) {
     for (size_t l = 0; l < mg_.Get_levels(); ++l)
       for (size_t b = 0; b < mg_.Get_num_dom(l); ++b) {
@@ -258,6 +466,12 @@ void elliptic::Print_Solution(double, double, double, double) {
               f << d.Get_x(int(i)) << "\t" << d.Get_y(int(j)) << "\t" << d.Get_z(int(k)) << "\t"
                 << u.Get_val(i, j, k) << "\t\n";
       }
+  }
+  if (print_interpol_) {
+    mg_.Print_Interpol_1D(dir_, "u", Lx, Dxyz, CUT_X, 0, 0, ord);
+    mg_.Print_Interpol_1D(dir_, "u", Ly, Dxyz, CUT_Y, 0, 0, ord);
+    mg_.Print_Interpol_1D(dir_, "u", Lz, Dxyz, CUT_Z, 0, 0, ord);
+    mg_.Print_Interpol_2D(dir_, "u", Lx, Ly, -0.1, -0.1, CUT_Z, 0, ord);
   }
   if (!quiet_) std::cout << "done" << std::endl;
 }

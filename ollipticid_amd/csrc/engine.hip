@@ -895,4 +895,30 @@ int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::str
   return OLL_OK;
 }
 
+int interp_points(Hier &H, hipStream_t s, int field, int npts, const int *box, const double *xyz,
+                  int ord, double *out, std::string &err) {
+  if (npts <= 0) return OLL_OK;
+  int *d_box = nullptr;
+  double *d_xyz = nullptr, *d_out = nullptr;
+  struct Release {  // frees whatever was allocated, on every return path
+    int *&b;
+    double *&x, *&o;
+    ~Release() {
+      (void)hipFree(b);
+      (void)hipFree(x);
+      (void)hipFree(o);
+    }
+  } rel{d_box, d_xyz, d_out};
+  ENG_CHECK(hipMalloc(&d_box, size_t(npts) * sizeof(int)));
+  ENG_CHECK(hipMalloc(&d_xyz, size_t(3 * npts) * sizeof(double)));
+  ENG_CHECK(hipMalloc(&d_out, size_t(npts) * sizeof(double)));
+  ENG_CHECK(hipMemcpyAsync(d_box, box, size_t(npts) * sizeof(int), hipMemcpyHostToDevice, s));
+  ENG_CHECK(hipMemcpyAsync(d_xyz, xyz, size_t(3 * npts) * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_interp_points(s, H.dbox, field, d_box, d_xyz, npts, ord, d_out);
+  ENG_CHECK(hipGetLastError());
+  ENG_CHECK(hipMemcpyAsync(out, d_out, size_t(npts) * sizeof(double), hipMemcpyDeviceToHost, s));
+  ENG_CHECK(hipStreamSynchronize(s));
+  return OLL_OK;
+}
+
 }  // namespace oll

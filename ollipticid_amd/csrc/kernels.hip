@@ -155,9 +155,11 @@ __device__ __forceinline__ unsigned fast_div(unsigned g, unsigned d, float inv_d
   return q;
 }
 
-// coordinate of index i, without contraction so it equals the host value
+// coordinate of index i, lo + i d rounded twice as on the host (the file's
+// contract(off) keeps the two operations apart; the __dmul_rn/__dadd_rn
+// helpers are compiled under the headers' own contraction setting and fuse)
 __device__ __forceinline__ double coord(const DevBox &B, int a, int i) {
-  return __dadd_rn(B.lo[a], __dmul_rn((double)i, B.d[a]));
+  return B.lo[a] + (double)i * B.d[a];
 }
 
 __device__ __forceinline__ bool interior(const DevBox &B, int i, int j, int k) {
@@ -1005,7 +1007,7 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
   const int ijk[3] = {i, j, k};
   double X[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) X[a] = __dadd_rn(G.lo[a], __dmul_rn((double)ijk[a], G.d[a]));
+  for (int a = 0; a < 3; ++a) X[a] = G.lo[a] + (double)ijk[a] * G.d[a];
   const double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
   double sF = 0.0, sdF = 0.0;
 #pragma unroll
@@ -1413,6 +1415,9 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
     out[q] = f[pidx(B, c[0], c[1], c[2])];
     return;
   }
+  // a box with fewer than ord + 1 points per axis takes the widest stencil
+  // it holds (interpolation.cc:221-250 would read outside the box there)
+  ord = min(ord, min(B.n[0], min(B.n[1], B.n[2])) - 1);
   double w[3][10];
   for (int a = 0; a < 3; ++a) {
     int s0 = (ord % 2 == 0) ? c[a] - ord / 2 : c[a] - (ord + 1) / 2;

@@ -41,12 +41,12 @@ class Problem(C.Structure):
 class BoxDesc(C.Structure):
     _fields_ = [("level", C.c_int), ("order", C.c_int), ("ghost", C.c_int),
                 ("n", C.c_int * 3), ("origin", C.c_double * 3), ("d", C.c_double * 3),
-                ("centre", C.c_double * 3)]
+                ("centre", C.c_double * 3), ("end", C.c_double * 3)]
 
     def as_dict(self):
         return {"level": self.level, "order": self.order, "R": self.ghost,
                 "n": list(self.n), "origin": list(self.origin), "d": list(self.d),
-                "centre": list(self.centre)}
+                "centre": list(self.centre), "end": list(self.end)}
 
 
 class Result(C.Structure):
@@ -79,6 +79,8 @@ SIGNATURES = {
     "oll_fas": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
     "oll_solve": (C.c_int, [_vp, C.POINTER(Result), C.POINTER(C.c_double), C.c_int]),
     "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "oll_interp": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                             C.c_int, C.POINTER(C.c_double)]),
     "oll_last_solve_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "oll_reset": (C.c_int, [_vp]),
     "oll_set_graphs": (C.c_int, [_vp, C.c_int]),
@@ -334,6 +336,20 @@ class Hierarchy:
         s = (C.c_double * self.nbox)()
         _check(self.L.oll_adm_mass(self.h, p, s))
         return list(p), list(s)
+
+    def interp(self, field, box, xyz, order):
+        """Lagrange interpolation (order + 1 points per axis) of `field` at
+        points xyz[q] of boxes box[q] (ffunction::Interpol_In,
+        interpolation.cc:189-303)."""
+        box = np.ascontiguousarray(box, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        assert len(box) == len(xyz)
+        out = np.empty(len(box))
+        _check(self.L.oll_interp(self.h, field, len(box),
+                                 box.ctypes.data_as(C.POINTER(C.c_int)),
+                                 xyz.ctypes.data_as(C.POINTER(C.c_double)), order,
+                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def last_norms(self):
         pre = (C.c_double * self.levels)()

@@ -9,7 +9,10 @@ configurations below, in a scratch directory, and keeps only data:
   box table (level, box, n, dx, origin), cycle count;
 * ``<name>.npz``   - converged ``u`` of every box in plain lexicographic order
   (``i`` fastest), float64, parsed from the reference's ``-p3d yes`` output
-  (12 significant digits), optionally subsampled by ``stride`` per axis.
+  (12 significant digits), optionally subsampled by ``stride`` per axis;
+* ``out_<name>.json/.npz`` - the default output files of a run (1-D cuts per
+  box, interpolated composite cuts and plane) as rows of numbers, with the
+  block structure of each file and the writer's stdout lines.
 
 Usage: python tests/golden/make_golden.py [name ...]
 """
@@ -169,7 +172,64 @@ def run_case(name):
     print("%s: %d cycles, %.1f s" % (name, meta["cycles"], wall), flush=True)
 
 
+# name -> argv of runs whose default output files (-p1d yes, -pint yes:
+# per-box 1-D cuts u-l<l>b<b>-0.x/.y/.z, composite cuts u.X/.Y/.Z and the
+# z = 0 plane u.XY) are kept as data
+OUTPUT_CASES = {
+    "out_cfg1": CASES["cfg1_conv"][0],
+    "out_o2_4bh": CASES["small_o2_4bh"][0],
+}
+
+
+def parse_blocks(path):
+    """Rows of numbers of one output file and the length of each block
+    (blocks end with a blank line)."""
+    rows, blocks, cur = [], [], 0
+    with open(path) as f:
+        for ln in f:
+            if ln.strip():
+                rows.append([float(v) for v in ln.split()])
+                cur += 1
+            else:
+                blocks.append(cur)
+                cur = 0
+    if cur:
+        blocks.append(cur)
+    return np.array(rows), blocks
+
+
+def output_files(run_dir):
+    names = sorted(f for f in os.listdir(run_dir)
+                   if re.fullmatch(r"u-l\d+b\d+-0\.[xyz]|u\.(X|Y|Z|XY)", f))
+    return names
+
+
+def run_outputs(name):
+    argv = OUTPUT_CASES[name]
+    with tempfile.TemporaryDirectory() as tmp:
+        out = subprocess.run([REF_BIN] + argv + ["-o", "run"], cwd=tmp, capture_output=True,
+                             text=True)
+        run = os.path.join(tmp, "run")
+        arrays, blocks = {}, {}
+        for f in output_files(run):
+            a, b = parse_blocks(os.path.join(run, f))
+            arrays[f] = a
+            blocks[f] = b
+        stdout = [ln for ln in out.stdout.splitlines()
+                  if ln.startswith("Print 2D interpol") or ln.startswith("Out file")]
+    meta = {"name": name, "argv": argv, "files": sorted(arrays), "blocks": blocks,
+            "stdout": stdout,
+            "note": "reference output files: rows of numbers per file, blocks split by blank lines"}
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("%s: %d files" % (name, len(arrays)), flush=True)
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + list(OUTPUT_CASES)
     for n in names:
-        run_case(n)
+        if n in OUTPUT_CASES:
+            run_outputs(n)
+        else:
+            run_case(n)
