@@ -124,6 +124,12 @@ class multigrid {
   void Print_Interpol_2D(const std::string &dir, const std::string &name, double LD1,
                          double LD2, double _dx1, double _dx2, int cut, double x1,
                          size_t order) const;
+  // multigrid.cc:991-1104: each "x y z value" record of a BAM file gets u
+  // at (x, y, z) written over it in the widths 16/17/17/24; false when a
+  // point lies outside the outermost box
+  bool Print_IO_BAM(const std::string &name, const std::string &var, size_t order) const;
+  // multigrid.cc:1123-1222: "x y z u(x,y,z)" per record of name into name_m
+  bool Print_IO_Zcode(const std::string &name, const std::string &var, size_t order) const;
 
  private:
   oll_hier *h_;
@@ -149,8 +155,14 @@ class elliptic {
   // writes dir/Loo.gp and dir/stdout0.out as the reference does
   void Solve();
   void Print_Solution(double Dxyz, double Lx, double Ly, double Lz);
-  void IO_bam(const std::string &file);
-  void IO_Zcode(const std::string &file);
+  // output.cc:377-451: rewrites the BAM initial-data files <file><l>,
+  // <file><l>a .. <file><l>e (levels l = 0, 1, ... while a level has one)
+  // in place with u at their points; false (message printed) when a point
+  // lies outside the outermost box
+  bool IO_bam(const std::string &file);
+  // output.cc:455-617: writes <name>_m beside each Zcode point file
+  // <file><ll>-0<b>.mgid and <file>SH-{x,y,z}{m,p}.mgid
+  bool IO_Zcode(const std::string &file);
 
   const multigrid &Get_multigrid() const { return mg_; }
   const std::vector<double> &Get_ADM_mass() const { return adm_; }
@@ -169,7 +181,7 @@ class elliptic {
   multigrid mg_;
   std::string dir_;
   bool print_1d_ = true, print_3d_ = false, print_interpol_ = true, interpol_output_ = true;
-  bool quiet_ = false;
+  bool quiet_ = false, verbose_ = true;
   std::vector<double> adm_;
   int cycles_ = 0;
   double res_ = 0, tol_ = 0;

@@ -1,7 +1,9 @@
 // Solver object and field views of Olliptic.x; see include/elliptic.h.
 #include "elliptic.h"
 
+#include <cctype>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <filesystem>
 #include <fstream>
@@ -311,6 +313,7 @@ void elliptic::Make(interface *Oll) {
   print_interpol_ = Oll->Get_logic_par("print_interpol");
   interpol_output_ = Oll->Get_logic_par("interpol_output");
   quiet_ = Oll->Get_TOutput() == NO;
+  verbose_ = Oll->Get_TOutput() == YES;
 
   const std::string name = Oll->Get_string_par("dir_name");
   dir_ = name + "/";
@@ -476,12 +479,202 @@ void elliptic::Print_Solution(double Dxyz, double Lx, double Ly, double Lz) {
   if (!quiet_) std::cout << "done" << std::endl;
 }
 
-void elliptic::IO_bam(const std::string &) {
-  std::cerr << "IO_bam export is not available in this build\n";
+namespace {
+
+bool read_file(const std::string &path, std::string &out) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  std::ostringstream ss;
+  ss << f.rdbuf();
+  out = ss.str();
+  return true;
 }
 
-void elliptic::IO_Zcode(const std::string &) {
-  std::cerr << "IO_Zcode export is not available in this build\n";
+bool file_exists(const std::string &path) { return std::ifstream(path).is_open(); }
+
+// Numbers read one at a time as `std::istream >> double` reads them:
+// leading whitespace skipped, `eof` once a read reaches the end of the text,
+// `fail` (sticky) once a read finds no number; a failed read leaves the
+// target unchanged.  Only plain decimal notation is accepted.
+struct Reader {
+  const std::string &s;
+  size_t pos = 0;
+  bool eof = false, fail = false, malformed = false;
+  bool number(double &v) {
+    if (fail) return false;
+    while (pos < s.size() && std::isspace(static_cast<unsigned char>(s[pos]))) ++pos;
+    if (pos >= s.size()) {
+      eof = fail = true;
+      return false;
+    }
+    const char c = s[pos], c1 = pos + 1 < s.size() ? s[pos + 1] : '\0';
+    const bool digit = std::isdigit(static_cast<unsigned char>(c)) || c == '.' ||
+                       ((c == '+' || c == '-') &&
+                        (std::isdigit(static_cast<unsigned char>(c1)) || c1 == '.'));
+    char *e = nullptr;
+    const double t = digit ? std::strtod(s.c_str() + pos, &e) : 0.0;
+    if (!digit || e == s.c_str() + pos) {
+      fail = malformed = true;
+      return false;
+    }
+    v = t;
+    pos = size_t(e - s.c_str());
+    if (pos >= s.size()) eof = true;
+    return true;
+  }
+};
+
+}  // namespace
+
+bool multigrid::Print_IO_BAM(const std::string &name, const std::string &var, size_t order) const {
+  // the reference leaves cout in this format (multigrid.cc:1005-1008)
+  std::cout.setf(std::ios_base::scientific, std::ios_base::floatfield);
+  std::cout.precision(16);
+  std::cout << "reading from: " << name << std::endl;
+  std::string buf;
+  if (!read_file(name, buf)) return true;
+  const domain &D0 = dom_[Get_dom_index(l0_, 0)];
+  // A record is read from where the previous one's trailing character
+  // ended and rewritten (74 bytes) from where its read started.  Values are
+  // interpolated in batches; a batch is written out before a read starts
+  // inside the bytes one of its records will overwrite (short lines), so
+  // every read sees what the reference's sequential loop would see.
+  std::vector<size_t> at;
+  Points pts;
+  size_t write_end = 0;
+  auto land = [&]() {
+    const std::vector<double> v = Interpol(var, pts.box, pts.xyz, order);
+    for (size_t q = 0; q < v.size(); ++q) {
+      std::ostringstream o;
+      o.setf(std::ios_base::scientific, std::ios_base::floatfield);
+      o.precision(9);
+      o << std::setw(16) << pts.xyz[3 * q] << std::setw(17) << pts.xyz[3 * q + 1] << std::setw(17)
+        << pts.xyz[3 * q + 2];
+      o.precision(15);
+      o << std::setw(24) << v[q];
+      const std::string rec = o.str();
+      if (at[q] + rec.size() > buf.size()) buf.resize(at[q] + rec.size());
+      buf.replace(at[q], rec.size(), rec);
+    }
+    at.clear();
+    pts = Points();
+  };
+  bool ok = true;
+  size_t cur = 0;
+  for (;;) {
+    if (write_end > cur) land();
+    Reader r{buf, cur};
+    double x = 0, y = 0, z = 0, val = 0;
+    if (!(r.number(x) && r.number(y) && r.number(z) && r.number(val))) {
+      // the reference's loop never leaves a failed read (it seeks back to
+      // the start and spins); the records so far are what it wrote
+      if (r.malformed) std::cerr << "warning: unreadable record in " << name << " at byte " << cur << "\n";
+      break;
+    }
+    if (!D0.is_in(x, y, z)) {
+      std::cout << "Error: bam grid bigger than Olliptic grid" << std::endl;
+      ok = false;
+      break;
+    }
+    // a record whose value ends the file is read with the stream at its
+    // end, and the reference's write of it does not happen
+    if (r.eof) break;
+    at.push_back(cur);
+    pts.add(Box_Of(x, y, z), x, y, z);
+    write_end = cur + 74;
+    cur = r.pos + 1;
+  }
+  land();
+  std::ofstream out(name, std::ios::binary | std::ios::trunc);
+  out << buf;
+  return ok;
+}
+
+bool multigrid::Print_IO_Zcode(const std::string &name, const std::string &var, size_t order) const {
+  std::cout << "reading from: " << name << std::endl;
+  std::string buf;
+  if (!read_file(name, buf)) return true;
+  const domain &D0 = dom_[Get_dom_index(l0_, 0)];
+  Points pts;
+  bool ok = true, any = false;
+  Reader r{buf};
+  double x = 0, y = 0, z = 0, val = 0;
+  // multigrid.cc:1153-1209: a record per pass until the stream is at its
+  // end; a file ending in a newline therefore repeats its last record (the
+  // pass that meets the end keeps the previous values)
+  do {
+    r.number(x);
+    r.number(y);
+    r.number(z);
+    r.number(val);
+    if (r.malformed) {
+      std::cerr << "warning: unreadable record in " << name << " at byte " << r.pos << "\n";
+      break;
+    }
+    if (r.fail && !any) break;  // nothing read: the reference would use unset values
+    any = true;
+    if (!D0.is_in(x, y, z)) {
+      std::cout << "X: " << x << "\t" << "Y: " << y << "\t" << "Z: " << z << std::endl;
+      std::cout << "Error: bam grid bigger than Olliptic grid" << std::endl;
+      ok = false;
+      break;
+    }
+    pts.add(Box_Of(x, y, z), x, y, z);
+  } while (!r.eof);
+  const std::vector<double> v = Interpol(var, pts.box, pts.xyz, order);
+  std::ofstream out(name + "_m", std::ios::trunc);
+  out.setf(std::ios_base::scientific, std::ios_base::floatfield);
+  out.precision(16);
+  for (size_t q = 0; q < v.size(); ++q)
+    out << pts.xyz[3 * q] << " " << pts.xyz[3 * q + 1] << " " << pts.xyz[3 * q + 2] << " " << v[q]
+        << "\n";
+  return ok;
+}
+
+bool elliptic::IO_bam(const std::string &file) {
+  if (!ok_) return false;
+  static const char *ending[] = {"", "a", "b", "c", "d", "e"};
+  std::vector<std::string> names;
+  size_t found = 0;
+  for (size_t level = 0; level == 0 || found > 0; ++level) {
+    found = 0;
+    for (const char *e : ending) {
+      const std::string f = file + std::to_string(level) + e;
+      if (!file_exists(f)) continue;
+      ++found;
+      if (verbose_) std::cout << "File found: " << f << std::endl;
+      names.push_back(f);
+    }
+  }
+  for (const std::string &f : names)
+    if (!mg_.Print_IO_BAM(f, "u", size_t(prob_.order) + 1)) return false;
+  return true;
+}
+
+bool elliptic::IO_Zcode(const std::string &file) {
+  if (!ok_) return false;
+  std::vector<std::string> names;
+  size_t found = 0;
+  for (size_t level = 0; level == 0 || found > 0; ++level) {
+    found = 0;
+    for (int b = 0; b < 6; ++b) {
+      const std::string f = file + (level < 10 ? "0" : "") + std::to_string(level) + "-0" +
+                            std::to_string(b) + ".mgid";
+      if (!file_exists(f)) continue;
+      ++found;
+      if (verbose_) std::cout << "File found: " << f << std::endl;
+      names.push_back(f);
+    }
+  }
+  for (const char *sh : {"xm", "xp", "ym", "yp", "zm", "zp"}) {
+    const std::string f = file + "SH-" + sh + ".mgid";
+    if (!file_exists(f)) continue;
+    if (verbose_) std::cout << "File found: " << f << std::endl;
+    names.push_back(f);
+  }
+  for (const std::string &f : names)
+    if (!mg_.Print_IO_Zcode(f, "u", size_t(prob_.order) + 1)) return false;
+  return true;
 }
 
 }  // namespace olliptic

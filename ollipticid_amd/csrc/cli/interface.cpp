@@ -258,7 +258,8 @@ void interface::proc_parameters() {
   if (i_par["nx"] != i_par["ny"] || i_par["nx"] != i_par["nz"] || d_par["dx"] != d_par["dy"] ||
       d_par["dx"] != d_par["dz"])
     fail("only cubic domains (equal N and L per axis) are available");
-  if (l_par["io_bam"] || l_par["io_Zcode"]) fail("IO_bam / IO_Zcode export is not available");
+  // interface.cc:1168-1175
+  if (l_par["io_bam"] && s_par["file_bam"] == "none") fail("*** Error: Missing file.");
   if (i_par["NumBH"] < 1 || i_par["NumBH"] > OLL_MAX_BH) fail("NumBH must be in 1..16");
 }
 

@@ -226,10 +226,94 @@ def run_outputs(name):
     print("%s: %d files" % (name, len(arrays)), flush=True)
 
 
+# ---------------------------------------------------------------- exports
+def bam_text(pts, newline_end=True):
+    """BAM initial-data records in the widths the reference writes back."""
+    lines = ["%16.9e%17.9e%17.9e%24.15e" % (x, y, z, 0.0) for x, y, z in pts]
+    return "\n".join(lines) + ("\n" if newline_end else "")
+
+
+def zcode_text(pts, newline_end=True, sep=" "):
+    lines = [sep.join("%.16e" % v for v in (x, y, z, 0.0)) for x, y, z in pts]
+    return "\n".join(lines) + ("\n" if newline_end else "")
+
+
+def export_runs():
+    """Point files for the BAM (-iob) and Zcode (-ioZ) exports of the cfg1
+    solve (L = 256: the outermost box spans +-128), one reference run each
+    (the reference's BAM loop never ends after its first file)."""
+    rng = np.random.default_rng(7)
+    g = [-90.0, -30.0, 30.0, 90.0]
+    grid = [(x, y, z) for z in g for y in g for x in g]
+    inner = [tuple(v) for v in rng.uniform(-15, 15, (20, 3))]
+    edge = [(124.0, 0, 0), (-124.0, 0, 0), (0, 124.0, 0), (0, -124.0, 0), (0, 0, 124.0), (0, 0, -124.0)]
+    centre = [(0.0, 0.0, 0.0), (8.0, -8.0, 16.0)]
+    rnd = lambda n, s: [tuple(v) for v in rng.uniform(-s, s, (n, 3))]  # noqa: E731
+    return [
+        {"prefix": "bamA_", "flag": "-iob", "inputs": {"bamA_0": bam_text(grid + inner + edge + centre)}},
+        {"prefix": "bamB_", "flag": "-iob", "inputs": {"bamB_0": bam_text(rnd(5, 40), newline_end=False)}},
+        {"prefix": "bamE_", "flag": "-iob",
+         "inputs": {"bamE_0": bam_text([(1.0, 2.0, 3.0), (140.0, 0.0, 0.0), (4.0, 5.0, 6.0)])}},
+        {"prefix": "zc_", "flag": "-ioZ", "inputs": {
+            "zc_00-00.mgid": zcode_text(rnd(30, 100)),
+            "zc_00-01.mgid": zcode_text(rnd(10, 20), newline_end=False, sep="   "),
+            "zc_00-03.mgid": zcode_text(inner[:5] + edge[:2]),
+            "zc_01-00.mgid": zcode_text(rnd(10, 5)),
+            "zc_SH-xm.mgid": zcode_text([(-120.0, y, z) for y, z in rng.uniform(-100, 100, (5, 2))])}},
+        {"prefix": "zcE_", "flag": "-ioZ",
+         "inputs": {"zcE_00-00.mgid": zcode_text([(1.0, 2.0, 3.0), (0.0, -140.0, 0.0), (4.0, 5.0, 6.0)])}},
+    ]
+
+
+def run_exports(name="io_cfg1"):
+    argv = CASES["cfg1_conv"][0]
+    runs = export_runs()
+    for run in runs:
+        with tempfile.TemporaryDirectory() as tmp:
+            for f, text in run["inputs"].items():
+                with open(os.path.join(tmp, f), "w") as fh:
+                    fh.write(text)
+            log = os.path.join(tmp, "stdout.txt")
+            cmd = [REF_BIN] + argv + ["-o", "run", "-p1d", "no", "-pint", "no",
+                                      run["flag"], "yes", "-iof", run["prefix"]]
+            with open(log, "w") as lf:
+                p = subprocess.Popen(cmd, cwd=tmp, stdout=lf, stderr=subprocess.STDOUT)
+                t0 = time.time()
+                while p.poll() is None and time.time() - t0 < 300:
+                    time.sleep(0.5)
+                    if "reading from: " + run["prefix"] in open(log).read() and run["flag"] == "-iob":
+                        # the reference's BAM loop does not end: its records
+                        # are flushed one by one, so wait briefly and stop it
+                        time.sleep(3)
+                        break
+                if p.poll() is None:
+                    p.kill()
+                    p.wait()
+                    rc = None
+                else:
+                    rc = p.returncode
+            text = open(log).read().splitlines()
+            run["rc"] = rc
+            run["stdout"] = [ln for ln in text if ln.startswith(("File found", "reading from", "Error", "X: "))]
+            outs = {}
+            for f in sorted(os.listdir(tmp)):
+                if f in run["inputs"] or f.endswith("_m"):
+                    outs[f] = open(os.path.join(tmp, f)).read()
+            run["outputs"] = outs
+        print(run["prefix"], "rc", rc, [len(v) for v in run["outputs"].values()], flush=True)
+    meta = {"name": name, "argv": argv, "runs": runs,
+            "note": "reference runs with -iob/-ioZ on point files written here; rc None: "
+                    "the reference's BAM loop did not end and was stopped after its writes"}
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES) + list(OUTPUT_CASES)
+    names = sys.argv[1:] or list(CASES) + list(OUTPUT_CASES) + ["io_cfg1"]
     for n in names:
         if n in OUTPUT_CASES:
             run_outputs(n)
+        elif n == "io_cfg1":
+            run_exports(n)
         else:
             run_case(n)

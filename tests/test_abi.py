@@ -40,7 +40,7 @@ def test_binding_covers_header():
 
 
 @pytest.mark.parametrize("path", sorted(p for p in glob.glob(os.path.join(REPO, "tests", "golden", "*.json"))
-                                         if not os.path.basename(p).startswith("out_")))
+                                         if not os.path.basename(p).startswith(("out_", "io_"))))
 def test_box_tables_match_reference(path):
     meta = json.load(open(path))
     boxes, mgrid = oa.problem_boxes(oa.problem_from_argv(meta["argv"]))
