@@ -175,13 +175,15 @@ class elliptic {
   void print_pre(int k, int top, double norm, int cycle);
   void print_post(int k, int top, double norm, int cycle);
   void compute_adm(bool print);
+  void approximate_id();
+  void print_puncture_approx();
 
   oll_problem prob_{};
   oll_hier *h_ = nullptr;
   multigrid mg_;
   std::string dir_;
   bool print_1d_ = true, print_3d_ = false, print_interpol_ = true, interpol_output_ = true;
-  bool quiet_ = false, verbose_ = true;
+  bool quiet_ = false, verbose_ = true, fit_approx_ = false;
   std::vector<double> adm_;
   int cycles_ = 0;
   double res_ = 0, tol_ = 0;

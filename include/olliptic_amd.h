@@ -56,6 +56,21 @@ extern "C" {
 #define OLL_STOP_REFERENCE 0
 #define OLL_STOP_TOLERANCE 1
 
+/* solve method (-m).  VCYCLE: FAS V-cycles (elliptic.cc:1251-1342).  APP
+ * ("AppPuncture", elliptic.cc:1977-2222): u is the analytic approximation of
+ * each puncture's spin, momentum and cross terms, with no iteration; its
+ * grids have no multigrid coarsening (elliptic.cc:510, 678); Punctures only
+ * (interface.cc:1154). */
+#define OLL_METHOD_VCYCLE 0
+#define OLL_METHOD_APP 1
+
+/* initial u of the V-cycles.  ZERO: the reference's u = 0 (boundary.cc:8-84).
+ * APPROX: the analytic approximation of OLL_METHOD_APP on every point of
+ * every box (no reference counterpart; the iterates start closer, the
+ * converged solution is the same). */
+#define OLL_GUESS_ZERO 0
+#define OLL_GUESS_APPROX 1
+
 /* field ids (reference field names, operator.cc:103-124,509-567) */
 #define OLL_F_U 0
 #define OLL_F_B 1
@@ -95,6 +110,8 @@ typedef struct oll_problem {
   int precision;        /* OLL_PREC_* */
   int stop_rule;        /* OLL_STOP_* */
   int patience;         /* OLL_STOP_TOLERANCE: stop after this many cycles without a new best */
+  int method;           /* OLL_METHOD_* (-m) */
+  int initial_guess;    /* OLL_GUESS_* */
 } oll_problem;
 
 typedef struct oll_hier oll_hier;
@@ -162,8 +179,16 @@ int oll_fas(oll_hier *h, int level, double *linf);
  * the pre-smoothing norm of level k (the coarsest-loop norm for k = 0),
  * post[k] the post-smoothing norm (output.cc:1355-1594 Print_Pre/Post_Cycle). */
 int oll_last_norms(const oll_hier *h, double *pre, double *post, int nlevels);
-/* elliptic.cc:1251-1342 Vcycle loop; hist (2*cap doubles) may be NULL. */
+/* elliptic.cc:1251-1342 Vcycle loop; hist (2*cap doubles) may be NULL.
+ * On an OLL_METHOD_APP hierarchy: elliptic.cc:1977-2222 ApproximateID (u set
+ * to the analytic approximation; 0 cycles, residual 0). */
 int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap);
+/* elliptic.cc:2232-2444 Get_Puncture_App: per puncture j (up to the first
+ * with zero mass), the approximation summed over the punctures at the grid
+ * node Get_i/j/k of j's position, over every finest-level box holding it
+ * (0.1 dx tolerance); u_at_p has num_bh entries, the rest left 0.  Host
+ * arithmetic; the printed values of -m AppPuncture and -fapp. */
+int oll_puncture_approx(const oll_hier *h, double *u_at_p);
 /* elliptic.cc:2447-2611 Compute_ADM_Mass: per puncture and per box. */
 int oll_adm_mass(oll_hier *h, double *puncture, double *surface);
 /* ffunction::Interpol_In with LAGRANGE of order ord (interpolation.cc:189-303,

@@ -6,7 +6,8 @@ front end.  Build with `python -m ollipticid_amd.build`.
 """
 from .native import (  # noqa: F401
     EQ_PUNCTURES, EQ_PUNCTURES_SCALAR, F_B, F_C, F_D, F_E, F_R, F_RHO, F_U, F_V, F_W,
-    F_WW, PREC_FP64, PREC_MIXED, STOP_REFERENCE, STOP_TOLERANCE, Engine, Hierarchy, OllError,
+    F_WW, GUESS_APPROX, GUESS_ZERO, METHOD_APP, METHOD_VCYCLE, PREC_FP64, PREC_MIXED,
+    STOP_REFERENCE, STOP_TOLERANCE, Engine, Hierarchy, OllError,
     Problem, default_problem, load,
     problem_boxes, problem_footprint, problem_from_argv,
 )

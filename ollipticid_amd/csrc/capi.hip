@@ -185,7 +185,8 @@ int oll_problem_boxes(const oll_problem *p, oll_box_desc *out, int cap, int *nbo
     ps.z.push_back(p->bh[i].z);
   }
   Hierarchy g;
-  const std::string err = build_hierarchy(size_t(p->N), p->L, p->levels, p->order, ps, g);
+  const std::string err =
+      build_hierarchy(size_t(p->N), p->L, p->levels, p->order, ps, g, p->method != OLL_METHOD_APP);
   if (!err.empty()) return fail(err.find("enough") != std::string::npos ? OLL_EGRID : OLL_EINVAL, err);
   *nbox = int(g.boxes.size());
   if (mgrid) *mgrid = g.mgrid;
@@ -280,9 +281,21 @@ int oll_fas(oll_hier *h, int level, double *linf) {
 
 int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
   if (!h) return fail(OLL_EINVAL, "null handle");
-  if (h->H.top() < 1) return fail(OLL_EINVAL, "hierarchy has a single level");
   std::string err;
   int rc;
+  if (h->H.prob.method == OLL_METHOD_APP) {
+    // ApproximateID: the approximation is the answer (hier_reset writes it)
+    CAPI_CHECK(hipEventRecord(h->ev0, h->stream));
+    if ((rc = hier_reset(h->H, h->stream, err))) return fail(rc, err);
+    CAPI_CHECK(hipEventRecord(h->ev1, h->stream));
+    CAPI_CHECK(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    CAPI_CHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    if (res) *res = oll_result{0, 0.0, 0};
+    return OLL_OK;
+  }
+  if (h->H.top() < 1) return fail(OLL_EINVAL, "hierarchy has a single level");
   if (h->vcycle.launches.empty()) {
     std::vector<Hier *> M{&h->H};
     if ((rc = build_vcycle(M, h->vcycle, err))) return fail(rc, err);
@@ -385,6 +398,12 @@ int oll_adm_mass(oll_hier *h, double *puncture, double *surface) {
   return rc ? fail(rc, err) : OLL_OK;
 }
 
+int oll_puncture_approx(const oll_hier *h, double *u_at_p) {
+  if (!h || !u_at_p) return fail(OLL_EINVAL, "null argument");
+  puncture_approx(h->H, u_at_p);
+  return OLL_OK;
+}
+
 int oll_interp(oll_hier *h, int field, int npts, const int *box, const double *xyz, int ord,
                double *out) {
   if (!h) return fail(OLL_EINVAL, "null handle");
@@ -463,6 +482,7 @@ void oll_engine_destroy(oll_engine *e) {
 
 int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag) {
   if (!e || !p) return fail(OLL_EINVAL, "null argument");
+  if (p->method != OLL_METHOD_VCYCLE) return fail(OLL_EINVAL, "the engine runs V-cycle requests");
   const double t0 = now_ms();
   struct AdmitClock {
     oll_engine *e;
@@ -491,6 +511,7 @@ int oll_engine_admit(oll_engine *e, const oll_problem *p, long long tag) {
 int oll_engine_admit_hier(oll_engine *e, oll_hier *h, long long tag) {
   if (!e || !h) return fail(OLL_EINVAL, "null argument");
   if (!h->H.slab) return fail(OLL_EINVAL, "empty hierarchy handle");
+  if (h->H.prob.method != OLL_METHOD_VCYCLE) return fail(OLL_EINVAL, "the engine runs V-cycle requests");
   if (e->budget && e->in_use + h->H.bytes > e->budget) return fail(OLL_EBUSY, "HBM budget exhausted");
   CAPI_CHECK(hipStreamSynchronize(h->stream));
   release_schedule(h->vcycle);

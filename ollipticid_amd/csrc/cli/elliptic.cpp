@@ -307,6 +307,10 @@ void elliptic::Make(interface *Oll) {
   prob_.ps_phi0 = Oll->Get_double_par("ps_phi0");
   prob_.ps_a2 = Oll->Get_double_par("ps_a2");
   prob_.precision = Oll->Get_string_par("precision") == "mixed" ? OLL_PREC_MIXED : OLL_PREC_FP64;
+  prob_.method = Oll->Get_Method() == APP ? OLL_METHOD_APP : OLL_METHOD_VCYCLE;
+  prob_.initial_guess =
+      Oll->Get_string_par("initial_guess") == "approx" ? OLL_GUESS_APPROX : OLL_GUESS_ZERO;
+  fit_approx_ = Oll->Get_logic_par("fit_approx");
   tol_ = prob_.tol < 2 * 2.220446049250313e-16 ? 2 * 2.220446049250313e-16 : prob_.tol;
   print_1d_ = Oll->Get_logic_par("print_1d");
   print_3d_ = Oll->Get_logic_par("print_3d");
@@ -415,9 +419,41 @@ void elliptic::compute_adm(bool print) {
     }
 }
 
+// elliptic.cc:2232-2444 Get_Puncture_App: the approximation at each
+// puncture's grid node, printed
+void elliptic::print_puncture_approx() {
+  std::vector<double> up(size_t(prob_.num_bh), 0.0);
+  if (oll_puncture_approx(h_, up.data()) != OLL_OK) return;
+  std::cout << std::scientific << std::setprecision(7);
+  for (int i = 0; i < prob_.num_bh; ++i)
+    std::cout << " == Approximate ID: puncture " << i << "\n"
+              << " x : " << prob_.bh[i].x << "\t" << " y : " << prob_.bh[i].y << "\t"
+              << " z : " << prob_.bh[i].z << "\t" << " u : " << up[size_t(i)] << std::endl;
+}
+
+// elliptic.cc:1977-2222 ApproximateID (-m AppPuncture): u is the
+// approximation, no iteration
+void elliptic::approximate_id() {
+  oll_result r{};
+  if (oll_solve(h_, &r, nullptr, 0) != OLL_OK) {
+    ok_ = false;
+    err_ = oll_last_error();
+    std::cerr << "Error: " << err_ << "\n";
+    return;
+  }
+  for (size_t l = 0; l < mg_.Get_levels(); ++l)
+    for (size_t b = 0; b < mg_.Get_num_dom(l); ++b)
+      std::cout << " == Approximate ID: level -" << l << "- box -" << b << "-\n" << std::endl;
+  print_puncture_approx();
+  cycles_ = 0;
+  res_ = 0;
+}
+
 // elliptic.cc:1251-1342 Vcycle with the reference's per-cycle logging
 void elliptic::Solve() {
   if (!ok_) return;
+  if (prob_.method == OLL_METHOD_APP) return approximate_id();
+  if (fit_approx_) print_puncture_approx();
   const int top = int(mg_.Get_levels()) - 1;
   std::vector<double> pre(size_t(top) + 1), post(size_t(top) + 1);
   double res = 1e12, res_prev;

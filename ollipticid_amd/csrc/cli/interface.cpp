@@ -110,6 +110,8 @@ interface::interface(int argc, char *argv[]) {
   add_string("domain_sym", "full", "-ds", "--domain-sym", "full");
   add_string("boundary", "Dirichlet", "-B", "--boundary", "Robin");
   add_string("precision", "fp64", "-prec", "--precision", "fp64 or mixed");
+  // no reference counterpart: start the V-cycles from the -m AppPuncture field
+  add_string("initial_guess", "zero", "-ig", "--initial-guess", "zero or approx");
 
   bhmp.assign(1, 0);
   read_console(argc, argv);
@@ -251,7 +253,17 @@ void interface::proc_parameters() {
   boundary = b == "Robin" ? ROBIN : (b == "Asymptotic" ? ASYMPTOTIC : DIRICHLET);
   if (boundary != ROBIN) fail("boundary '" + b + "' is not solved here (Robin)");
   const std::string m = s_par["method"];
-  if (m != "Vcycle") fail("method '" + m + "' is not available here (Vcycle)");
+  if (m == "AppPuncture")
+    method = APP;
+  else if (m == "Vcycle")
+    method = VCYCLE;
+  else
+    fail("method '" + m + "' is not available here (Vcycle, AppPuncture)");
+  // interface.cc:1154-1160
+  if (method == APP && equation != PUNCTURES)
+    fail("*** Error: App puncture only works with puncture equation");
+  const std::string g = s_par["initial_guess"];
+  if (g != "zero" && g != "approx") fail("initial guess must be zero or approx");
   if (s_par["domain_sym"] != "full") fail("only the full domain is available");
   const std::string o = s_par["output"];
   output = o == "no" ? NO : (o == "partial" ? PARTIAL : YES);

@@ -8,6 +8,10 @@
 #include <map>
 #include <tuple>
 
+// host arithmetic that must equal the reference's (coordinates, the
+// approximation's puncture values) is rounded operation by operation
+#pragma clang fp contract(off)
+
 namespace oll {
 
 namespace {
@@ -60,6 +64,12 @@ int validate(const oll_problem &p, std::string &err) {
     return err = "OLL_STOP_TOLERANCE needs max_cycles or patience", OLL_EINVAL;
   if (p.precision != OLL_PREC_FP64 && p.precision != OLL_PREC_MIXED)
     return err = "precision must be OLL_PREC_FP64 or OLL_PREC_MIXED", OLL_EINVAL;
+  if (p.method != OLL_METHOD_VCYCLE && p.method != OLL_METHOD_APP)
+    return err = "unknown method", OLL_EINVAL;
+  if (p.method == OLL_METHOD_APP && p.eq != OLL_EQ_PUNCTURES)
+    return err = "App puncture only works with puncture equation", OLL_EINVAL;
+  if (p.initial_guess != OLL_GUESS_ZERO && p.initial_guess != OLL_GUESS_APPROX)
+    return err = "unknown initial guess", OLL_EINVAL;
   return OLL_OK;
 }
 
@@ -70,7 +80,7 @@ int geometry(const oll_problem &p, Hierarchy &g, std::string &err) {
     ps.y.push_back(p.bh[i].y);
     ps.z.push_back(p.bh[i].z);
   }
-  err = build_hierarchy(size_t(p.N), p.L, p.levels, p.order, ps, g);
+  err = build_hierarchy(size_t(p.N), p.L, p.levels, p.order, ps, g, p.method != OLL_METHOD_APP);
   if (!err.empty()) return err.find("enough") != std::string::npos ? OLL_EGRID : OLL_EINVAL;
   return OLL_OK;
 }
@@ -300,6 +310,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
 
 int hier_reset(Hier &H, hipStream_t s, std::string &err) {
   for (const DevBox &b : H.hbox) launch_coefficients_box(s, b, H.pa, H.prob.robin_a);
+  if (H.prob.method == OLL_METHOD_APP || H.prob.initial_guess == OLL_GUESS_APPROX)
+    for (const DevBox &b : H.hbox) launch_approx_box(s, b, H.pa);
   ENG_CHECK(hipGetLastError());
   ENG_CHECK(hipStreamSynchronize(s));
   H.cycle = 0;
@@ -893,6 +905,26 @@ int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::str
     }
   }
   return OLL_OK;
+}
+
+void puncture_approx(const Hier &H, double *u_at_p) {
+  const oll_problem &p = H.prob;
+  int n = 0;
+  while (n < p.num_bh && p.bh[n].m != 0.0) ++n;
+  for (int j = 0; j < p.num_bh; ++j) u_at_p[j] = 0.0;
+  const Hierarchy &g = H.geo;
+  const int top = g.levels - 1;
+  for (int b = g.level_first[size_t(top)]; b < g.level_first[size_t(top)] + g.level_count[size_t(top)]; ++b) {
+    const Box &B = g.boxes[size_t(b)];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double pj[3] = {p.bh[j].x, p.bh[j].y, p.bh[j].z};
+        if (!(B.in_box(0, pj[0]) && B.in_box(1, pj[1]) && B.in_box(2, pj[2]))) continue;
+        double X[3];
+        for (int a = 0; a < 3; ++a) X[a] = B.x(a, long(B.cell(a, pj[a])));
+        u_at_p[j] += approx_puncture_u(H.pa.p[i], X[0], X[1], X[2]);
+      }
+  }
 }
 
 int interp_points(Hier &H, hipStream_t s, int field, int npts, const int *box, const double *xyz,

@@ -144,6 +144,8 @@ int build_vcycle_from(const std::vector<Hier *> &members, const std::vector<int>
                       Schedule &S, std::string &err);
 // ADM masses (elliptic.cc:2447-2611); surface may be null
 int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::string &err);
+// elliptic.cc:2232-2444 Get_Puncture_App values (host arithmetic)
+void puncture_approx(const Hier &H, double *u_at_p);
 // Lagrange interpolation of `field` at npts host points (box[q], xyz[3q..3q+2])
 int interp_points(Hier &H, hipStream_t s, int field, int npts, const int *box, const double *xyz,
                   int ord, double *out, std::string &err);

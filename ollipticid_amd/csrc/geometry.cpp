@@ -132,7 +132,7 @@ bool touches(const Box &A, const Box &B) {
 }  // namespace
 
 std::string build_hierarchy(size_t Nin, double L, int levels, int order,
-                            const PunctureSet &p, Hierarchy &h) {
+                            const PunctureSet &p, Hierarchy &h, bool coarsen) {
   if (!(order == 2 || order == 4 || order == 6 || order == 8))
     return "order must be [2,4,6,8].";
   if (levels < 1) return "levels must be >= 1";
@@ -142,13 +142,13 @@ std::string build_hierarchy(size_t Nin, double L, int levels, int order,
   // multigrid coarsening of the base box, elliptic.cc:502-651
   std::vector<size_t> ns{N};
   std::vector<double> ds{Dx};
-  for (;;) {
+  while (coarsen) {
     const size_t half = (ns.back() + 1) / 2;
     if (half < 12) break;
     ns.push_back(half);
     ds.push_back(ds.back() * 2.0);
   }
-  if (ns.size() <= 1) return "There are not enough grid-points for multigrid";
+  if (coarsen && ns.size() <= 1) return "There are not enough grid-points for multigrid";
   h.mgrid = int(ns.size()) - 1;
   h.levels = levels + h.mgrid;
 

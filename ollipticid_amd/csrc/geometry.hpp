@@ -60,8 +60,10 @@ struct PunctureSet {
 // Builds the hierarchy for N points per side on a cube of side L with
 // `levels` refinement levels around the given punctures.  Returns an empty
 // string on success, otherwise the error message.
+// coarsen = false builds the grids of -m AppPuncture: no multigrid levels
+// below the base box (elliptic.cc:510, 678).
 std::string build_hierarchy(size_t N, double L, int levels, int order,
-                            const PunctureSet &p, Hierarchy &h);
+                            const PunctureSet &p, Hierarchy &h, bool coarsen = true);
 
 // ---------------------------------------------------------------------
 // Per-axis transfer tables.  Every transfer of the reference is a tensor

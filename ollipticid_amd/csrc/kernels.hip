@@ -1316,6 +1316,20 @@ __global__ __launch_bounds__(256) void k_coefficients(DevBox B, PunctureArgs pa,
   B.f[F_U][p] = 0.0;
 }
 
+// elliptic.cc:1977-2222 ApproximateID: the terms of the punctures up to the
+// first with zero mass, added in puncture order
+__global__ __launch_bounds__(256) void k_approx_u(DevBox B, PunctureArgs pa) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long np = (long long)B.n[0] * B.n[1] * B.n[2];
+  if (q >= np) return;
+  int i, j, k;
+  point_ijk(B, q, i, j, k);
+  const double x = coord(B, 0, i), y = coord(B, 1, j), z = coord(B, 2, k);
+  double u = 0.0;
+  for (int b = 0; b < pa.n && pa.p[b].m != 0.0; ++b) u += approx_puncture_u(pa.p[b], x, y, z);
+  B.f[F_U][pidx(B, i, j, k)] = u;
+}
+
 // ------------------------------------------------------------------
 // ADM mass surface integrand: dU = x_hat . grad(c + u) interpolated
 // (trilinear) onto a 101 x 101 sphere of radius 0.375 min L around the box
@@ -1531,6 +1545,11 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs 
                              double robin_a) {
   const long long np = (long long)B.n[0] * B.n[1] * B.n[2];
   hipLaunchKernelGGL(k_coefficients, dim3(ceil_div(np, 256)), dim3(256), 0, s, B, pa, robin_a);
+}
+
+void launch_approx_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa) {
+  const long long np = (long long)B.n[0] * B.n[1] * B.n[2];
+  hipLaunchKernelGGL(k_approx_u, dim3(ceil_div(np, 256)), dim3(256), 0, s, B, pa);
 }
 
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,

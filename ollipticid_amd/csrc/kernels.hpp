@@ -141,6 +141,44 @@ struct Puncture {
 
 constexpr int kMaxBH = 16;
 
+// elliptic.cc:2037-2213 (the same terms at :2286-2419): puncture b's term of
+// the analytic approximation at (x, y, z), the spin (J), momentum (P) and
+// cross (P x J) parts of the perturbative single-puncture solution.  Host
+// and device evaluate it operation for operation alike.
+__host__ __device__ inline double approx_puncture_u(const Puncture &b, double x, double y, double z) {
+#pragma clang fp contract(off)
+  const double sbl = sqrt(b.sx * b.sx + b.sy * b.sy + b.sz * b.sz);
+  const double imass = 1.0 / b.m;
+  const double imass2 = imass * imass;
+  const double Jx = 4.0 * b.sx * imass2, Jy = 4.0 * b.sy * imass2, Jz = 4.0 * b.sz * imass2;
+  const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+  const double pbl = sqrt(b.px * b.px + b.py * b.py + b.pz * b.pz);
+  const double Px = 2.0 * b.px * imass, Py = 2.0 * b.py * imass, Pz = 2.0 * b.pz * imass;
+  const double P2 = Px * Px + Py * Py + Pz * Pz;
+  const double xi = x - b.x, yi = y - b.y, zi = z - b.z;
+  double r = sqrt(xi * xi + yi * yi + zi * zi);
+  if (fabs(r) <= 0.00000000001) r = 0.0000000001;
+  const double Rx = 2.0 * xi * imass, Ry = 2.0 * yi * imass, Rz = 2.0 * zi * imass;
+  const double R = 2.0 * r * imass;
+  const double l1 = 1.0 / (1.0 + R);
+  const double l2 = l1 * l1, l3 = l2 * l1, l4 = l3 * l1, l5 = l4 * l1;
+  const double u0j = 0.025 * (l1 + l2 + l3 - 4.0 * l4 + 2.0 * l5);
+  const double u2j = -0.05 * l5;
+  const double muj = (sbl != 0.0 && r != 0.0) ? (b.sx * xi + b.sy * yi + b.sz * zi) / (r * sbl) : 0.0;
+  const double PL2j = 0.5 * (3.0 * muj * muj - 1.0);
+  const double uj = J2 * (u0j + u2j * R * R * PL2j);
+  const double u0p = 0.15625 * (l1 - 2.0 * l2 + 2.0 * l3 - l4 + 0.2 * l5);
+  const double u2p = 0.0125 *
+                     (15.0 * l1 + 132.0 * l2 + 53.0 * l3 + 96.0 * l4 + 82 * l5 + 84.0 * (l5 + log(l1) / R) / R) /
+                     R;
+  const double mup = (pbl != 0.0 && r != 0.0) ? (b.px * xi + b.py * yi + b.pz * zi) / (r * pbl) : 0.0;
+  const double PL2p = 0.5 * (3.0 * mup * mup - 1.0);
+  const double up = P2 * (u0p + u2p * PL2p);
+  const double c0 = Py * Jz - Pz * Jy, c1 = Pz * Jx - Px * Jz, c2 = Px * Jy - Py * Jx;
+  const double uc = 0.0125 * (c0 * Rx + c1 * Ry + c2 * Rz) * (1.0 + 5.0 * R + 10.0 * R * R) * l5;
+  return uj + up + uc;
+}
+
 struct PunctureArgs {
   int n;
   int eq;
@@ -213,6 +251,9 @@ void kernels_init();
 constexpr int kCoarseLdsLimit = 160 * 1024;
 void launch_adm_sphere(hipStream_t s, const DevBox *boxes, int nbox, const double *centres,
                        double *sph, int nt);
+// u on every point of box B set to the analytic approximation
+// (elliptic.cc:1977-2222)
+void launch_approx_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa);
 void launch_interp_points(hipStream_t s, const DevBox *boxes, int field, const int *pbox,
                           const double *pxyz, int npts, int ord, double *out);
 

@@ -15,6 +15,8 @@ MAX_BH = 16
 EQ_PUNCTURES, EQ_PUNCTURES_SCALAR = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 STOP_REFERENCE, STOP_TOLERANCE = 0, 1
+METHOD_VCYCLE, METHOD_APP = 0, 1
+GUESS_ZERO, GUESS_APPROX = 0, 1
 F_U, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW = range(10)
 NFIELDS = 10
 ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EDEVICE", -4: "ENAN", -5: "ENOMEM", -6: "EBUSY"}
@@ -35,6 +37,7 @@ class Problem(C.Structure):
         ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
         ("max_cycles", C.c_int), ("precision", C.c_int),
         ("stop_rule", C.c_int), ("patience", C.c_int),
+        ("method", C.c_int), ("initial_guess", C.c_int),
     ]
 
 
@@ -79,6 +82,7 @@ SIGNATURES = {
     "oll_fas": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
     "oll_solve": (C.c_int, [_vp, C.POINTER(Result), C.POINTER(C.c_double), C.c_int]),
     "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "oll_puncture_approx": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "oll_interp": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                              C.c_int, C.POINTER(C.c_double)]),
     "oll_last_solve_ms": (C.c_int, [_vp, C.POINTER(C.c_double)]),
@@ -185,6 +189,14 @@ def problem_from_argv(argv):
             p.robin_a = float(v)
         elif tok == "-Bn":
             p.robin_n = int(float(v))
+        elif tok in ("-m", "--method"):
+            if v not in ("Vcycle", "AppPuncture"):
+                raise ValueError("method %r is outside this solver (Vcycle, AppPuncture)" % v)
+            p.method = METHOD_APP if v == "AppPuncture" else METHOD_VCYCLE
+        elif tok in ("-ig", "--initial-guess"):
+            if v not in ("zero", "approx"):
+                raise ValueError("initial guess must be zero or approx")
+            p.initial_guess = GUESS_APPROX if v == "approx" else GUESS_ZERO
         elif tok in ("-prec", "--precision"):
             if v not in ("fp64", "mixed"):
                 raise ValueError("precision must be fp64 or mixed")
@@ -336,6 +348,12 @@ class Hierarchy:
         s = (C.c_double * self.nbox)()
         _check(self.L.oll_adm_mass(self.h, p, s))
         return list(p), list(s)
+
+    def puncture_approx(self):
+        """Get_Puncture_App values (elliptic.cc:2232-2444), one per puncture."""
+        out = (C.c_double * self.problem.num_bh)()
+        _check(self.L.oll_puncture_approx(self.h, out))
+        return list(out)
 
     def interp(self, field, box, xyz, order):
         """Lagrange interpolation (order + 1 points per axis) of `field` at

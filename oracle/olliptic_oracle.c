@@ -261,7 +261,7 @@ static int build_grids(oracle_solver *s) {
   dx[0] = Dx;
   const size_t base = 12;
   int limit = 0;
-  for (;;) {
+  for (; !p->method_app;) {
     size_t ntmp = (nx[nl - 1] + 1) / 2;
     if (ntmp < base) {
       limit = 1;
@@ -276,7 +276,7 @@ static int build_grids(oracle_solver *s) {
       break;
     }
   }
-  if (nl <= 1) {
+  if (nl <= 1 && !p->method_app) {
     set_err("There are not enough grid-points for multigrid");
     free(nx);
     free(dx);
@@ -817,6 +817,81 @@ static double lagrange_interp(const obox *B, const double *f, int ord, double x,
 static double interp(const obox *B, const double *f, int method, int ord, double x,
                      double y, double z) {
   return method == OI_LINEAR ? linear_interp(B, f, x, y, z) : lagrange_interp(B, f, ord, x, y, z);
+}
+
+/* elliptic.cc:2037-2213 (and 2286-2419): the term of puncture i of the
+ * approximate solution at (x, y, z) */
+static double approx_term(const oracle_params *p, int i, double x, double y, double z) {
+  const double sbl = sqrt(p->sx[i] * p->sx[i] + p->sy[i] * p->sy[i] + p->sz[i] * p->sz[i]);
+  const double imass = 1.0 / p->mp[i];
+  const double imass2 = imass * imass;
+  const double Jx = 4.0 * p->sx[i] * imass2, Jy = 4.0 * p->sy[i] * imass2,
+               Jz = 4.0 * p->sz[i] * imass2;
+  const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+  const double pbl = sqrt(p->px[i] * p->px[i] + p->py[i] * p->py[i] + p->pz[i] * p->pz[i]);
+  const double Px = 2.0 * p->px[i] * imass, Py = 2.0 * p->py[i] * imass,
+               Pz = 2.0 * p->pz[i] * imass;
+  const double P2 = Px * Px + Py * Py + Pz * Pz;
+  const double xi = x - p->x[i], yi = y - p->y[i], zi = z - p->z[i];
+  double r = sqrt(xi * xi + yi * yi + zi * zi);
+  if (fabs(r) <= 0.00000000001) r = 0.0000000001;
+  const double Rx = 2.0 * xi * imass, Ry = 2.0 * yi * imass, Rz = 2.0 * zi * imass;
+  const double R = 2.0 * r * imass;
+  const double l1 = 1.0 / (1.0 + R);
+  const double l2 = l1 * l1, l3 = l2 * l1, l4 = l3 * l1, l5 = l4 * l1;
+  const double u0j = 0.025 * (l1 + l2 + l3 - 4.0 * l4 + 2.0 * l5);
+  const double u2j = -0.05 * l5;
+  const double muj = (sbl != 0.0 && r != 0.0)
+                         ? (p->sx[i] * xi + p->sy[i] * yi + p->sz[i] * zi) / (r * sbl) : 0.0;
+  const double PL2j = 0.5 * (3.0 * muj * muj - 1.0);
+  const double uj = J2 * (u0j + u2j * R * R * PL2j);
+  const double u0p = 0.15625 * (l1 - 2.0 * l2 + 2.0 * l3 - l4 + 0.2 * l5);
+  const double u2p = 0.0125 * (15.0 * l1 + 132.0 * l2 + 53.0 * l3 + 96.0 * l4 + 82 * l5 +
+                               84.0 * (l5 + log(l1) / R) / R) / R;
+  const double mup = (pbl != 0.0 && r != 0.0)
+                         ? (p->px[i] * xi + p->py[i] * yi + p->pz[i] * zi) / (r * pbl) : 0.0;
+  const double PL2p = 0.5 * (3.0 * mup * mup - 1.0);
+  const double up = P2 * (u0p + u2p * PL2p);
+  const double c0 = Py * Jz - Pz * Jy, c1 = Pz * Jx - Px * Jz, c2 = Px * Jy - Py * Jx;
+  const double uc = 0.0125 * (c0 * Rx + c1 * Ry + c2 * Rz) * (1.0 + 5.0 * R + 10.0 * R * R) * l5;
+  return uj + up + uc;
+}
+
+void oracle_approx_id(oracle_solver *s) {
+  const oracle_params *p = &s->p;
+  for (int b = 0; b < s->nbox; b++) {
+    obox *B = &s->box[b];
+    double *u = B->f[OF_U];
+    for (size_t k = 0; k < B->D.n[2]; k++)
+      for (size_t j = 0; j < B->D.n[1]; j++)
+        for (size_t i = 0; i < B->D.n[0]; i++) {
+          const double x = dom_x(&B->D, 0, (long)i), y = dom_x(&B->D, 1, (long)j),
+                       z = dom_x(&B->D, 2, (long)k);
+          double v = 0.0;
+          for (int q = 0; q < p->nbh && p->mp[q] != 0.0; q++) v += approx_term(p, q, x, y, z);
+          u[IDX(B, i, j, k)] = v;
+        }
+  }
+}
+
+void oracle_puncture_approx(const oracle_solver *s, double *u_at_p) {
+  const oracle_params *p = &s->p;
+  int n = 0;
+  while (n < p->nbh && p->mp[n] != 0.0) n++;
+  for (int j = 0; j < n; j++) u_at_p[j] = 0.0;
+  const int top = s->levels - 1;
+  for (int b = s->lvl_first[top]; b < s->lvl_first[top] + s->lvl_count[top]; b++) {
+    const odom *D = &s->box[b].D;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        const double pj[3] = {p->x[j], p->y[j], p->z[j]};
+        if (!dom_is_in(D, pj)) continue;
+        const double x = dom_x(D, 0, (long)dom_get_i(D, 0, pj[0]));
+        const double y = dom_x(D, 1, (long)dom_get_i(D, 1, pj[1]));
+        const double z = dom_x(D, 2, (long)dom_get_i(D, 2, pj[2]));
+        u_at_p[j] += approx_term(p, i, x, y, z);
+      }
+  }
 }
 
 double oracle_interp_point(oracle_solver *s, int ibox, int field, int method, int ord,

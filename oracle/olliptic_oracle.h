@@ -59,6 +59,8 @@ typedef struct oracle_params {
   double coarse_omega;  /* over-relaxation of the coarsest-level loop:
                            0 plain Gauss-Seidel (the reference), >0 that
                            factor, <0 2 - 1.6/(n-1) (the GPU's choice) */
+  int method_app;       /* 1: -m AppPuncture, whose grids have no multigrid
+                           coarsening (elliptic.cc:510, 678) */
 } oracle_params;
 
 typedef struct oracle_solver oracle_solver;
@@ -99,6 +101,15 @@ int oracle_solve(oracle_solver *s, double *hist, int hist_cap, int *n_hist,
                  double *final_res);
 /* ADM masses: per puncture (nbh values) and per box surface integral. */
 void oracle_adm_mass(oracle_solver *s, double *puncture, double *surface);
+/* elliptic.cc:1977-2222 ApproximateID: u on every point of every box set
+ * to the analytic approximation (spin, momentum and cross terms of each
+ * puncture up to the first with zero mass). */
+void oracle_approx_id(oracle_solver *s);
+/* elliptic.cc:2232-2444 Get_Puncture_App: per puncture, the approximation
+ * summed over the punctures at the grid node Get_i/j/k of its position in
+ * every finest-level box that holds it (0.1 dx tolerance). */
+void oracle_puncture_approx(const oracle_solver *s, double *u_at_p);
+
 /* Lagrange/linear point interpolation of a field of one box (testing). */
 double oracle_interp_point(oracle_solver *s, int ibox, int field, int method,
                            int ord, double x, double y, double z);

@@ -35,6 +35,7 @@ class Params(C.Structure):
         ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
         ("sweep_order", C.c_int), ("max_cycles", C.c_int),
         ("restrict_fw", C.c_int), ("coarse_omega", C.c_double),
+        ("method_app", C.c_int),
     ]
 
 
@@ -82,6 +83,8 @@ def lib():
         L.oracle_interp_point.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.c_double, C.c_double, C.c_double]
         L.oracle_interp_point.restype = C.c_double
+        L.oracle_approx_id.argtypes = [vp]
+        L.oracle_puncture_approx.argtypes = [vp, C.POINTER(C.c_double)]
         L.oracle_coarse_iterations.argtypes = [vp]
         L.oracle_coarse_iterations.restype = C.c_long
         L.oracle_valid_n.argtypes = [C.c_size_t]
@@ -97,6 +100,7 @@ def params_from_argv(argv, sweep_order=LEX, max_cycles=0, restrict_fw=None):
     p = Params()
     lib().oracle_default_params(C.byref(p))
     eq = "none"
+    method = "Vcycle"
     bh = {}
     nbh = 1
     a = list(argv)
@@ -104,6 +108,8 @@ def params_from_argv(argv, sweep_order=LEX, max_cycles=0, restrict_fw=None):
         v = a[i + 1]
         if tok == "-eq":
             eq = v
+        elif tok == "-m":
+            method = v
         elif tok == "-NumBH":
             nbh = int(v)
         elif tok == "-L":
@@ -141,6 +147,7 @@ def params_from_argv(argv, sweep_order=LEX, max_cycles=0, restrict_fw=None):
         arr = getattr(p, key)
         for j in range(MAX_BH):
             arr[j] = bh.get((key, j), 0.0)
+    p.method_app = int(method == "AppPuncture")
     p.sweep_order = sweep_order
     p.max_cycles = max_cycles
     p.restrict_fw = (sweep_order == COLOUR) if restrict_fw is None else int(restrict_fw)
@@ -235,6 +242,14 @@ class Solver:
 
     def coarse_iterations(self):
         return self.L.oracle_coarse_iterations(self.h)
+
+    def approx_id(self):
+        self.L.oracle_approx_id(self.h)
+
+    def puncture_approx(self):
+        out = (C.c_double * self.params.nbh)()
+        self.L.oracle_puncture_approx(self.h, out)
+        return list(out)
 
     def interp(self, ib, fid, method, ord_, x, y, z):
         return self.L.oracle_interp_point(self.h, ib, fid, method, ord_, x, y, z)

@@ -226,6 +226,51 @@ def run_outputs(name):
     print("%s: %d files" % (name, len(arrays)), flush=True)
 
 
+# ------------------------------------------------ analytic approximation
+# name -> argv of -m AppPuncture runs: u of every box (-p3d, 12 digits) and
+# the puncture values the reference prints
+APP_CASES = {
+    "app_two": PUNC + ["-NumBH", "2", "-bhmp1", "0.5", "-bhx1", "1.3", "-bhpy1", "0.2",
+                       "-bhsz1", "0.1", "-bhmp2", "0.4", "-bhx2", "-1.7", "-bhy2", "0.3",
+                       "-bhpx2", "0.1", "-bhsx2", "-0.05", "-L", "40", "-N", "25", "-l", "3",
+                       "-O", "4", "-m", "AppPuncture"],
+    "app_origin": PUNC + ["-NumBH", "1", "-bhmp1", "1.0", "-bhpx1", "0.2", "-bhpz1", "-0.1",
+                          "-bhsx1", "0.3", "-bhsy1", "0.1", "-L", "64", "-N", "33", "-l", "4",
+                          "-O", "2", "-m", "AppPuncture"],
+    "app_three": PUNC + ["-NumBH", "3", "-bhmp1", "0.4", "-bhx1", "0.8", "-bhy1", "0.5",
+                         "-bhpx1", "0.1", "-bhsz1", "0.05", "-bhmp2", "0.3", "-bhx2", "-0.9",
+                         "-bhz2", "0.4", "-bhpy2", "-0.15", "-bhsx2", "0.02", "-bhmp3", "0.35",
+                         "-bhy3", "-1.1", "-bhpz3", "0.12", "-bhsy3", "-0.04",
+                         "-L", "30", "-N", "25", "-l", "3", "-O", "6", "-m", "AppPuncture"],
+}
+APP_RE = re.compile(r"x : (\S+)\s+y : (\S+)\s+z : (\S+)\s+u : (\S+)")
+
+
+def run_app(name):
+    argv = APP_CASES[name]
+    with tempfile.TemporaryDirectory() as tmp:
+        out = subprocess.run([REF_BIN] + argv + ["-o", "run", "-p3d", "yes", "-p1d", "no",
+                                                 "-pint", "no"],
+                             cwd=tmp, capture_output=True, text=True)
+        boxes, _, _ = parse_stdout(out.stdout)
+        arrays = {}
+        for b in boxes:
+            lv, bx = b["level"], b["box"]
+            u, org, d = parse_xyz(os.path.join(tmp, "run", "u-l%db%d-0.xyz" % (lv, bx)))
+            b["origin"], b["dx"] = org, d
+            arrays["u_l%d_b%d" % (lv, bx)] = u[::2, ::2, ::2].copy()
+        pu = [float(m.group(4)) for m in APP_RE.finditer(out.stdout)]
+        lines = [ln for ln in out.stdout.splitlines() if "Approximate ID" in ln or " u : " in ln]
+    meta = {"name": name, "argv": argv, "boxes": boxes, "puncture_u": pu, "stdout": lines,
+            "levels": max(b["level"] for b in boxes) + 1, "stride": 2,
+            "note": "reference -m AppPuncture run; u lexicographic (k,j,i), 12 digits; "
+                    "puncture u printed with 8 significant digits"}
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("%s: %d boxes, puncture u %s" % (name, len(boxes), pu), flush=True)
+
+
 # ---------------------------------------------------------------- exports
 def bam_text(pts, newline_end=True):
     """BAM initial-data records in the widths the reference writes back."""
@@ -309,10 +354,12 @@ def run_exports(name="io_cfg1"):
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES) + list(OUTPUT_CASES) + ["io_cfg1"]
+    names = sys.argv[1:] or list(CASES) + list(OUTPUT_CASES) + list(APP_CASES) + ["io_cfg1"]
     for n in names:
         if n in OUTPUT_CASES:
             run_outputs(n)
+        elif n in APP_CASES:
+            run_app(n)
         elif n == "io_cfg1":
             run_exports(n)
         else:
