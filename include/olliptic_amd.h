@@ -259,6 +259,11 @@ int oll_engine_profile(const oll_engine *e, double *build_ms, double *run_ms, do
  * per updated point */
 int oll_engine_set_kernel_timing(oll_engine *e, int on);
 int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, double *bytes);
+/* The timed smoother launches one by one, in issue order: algorithmic bytes,
+ * device ms and kind (0 FP64 sweep colour, 1 FP32 correction colour) of the
+ * first cap of them; *n is how many there are. */
+int oll_engine_kernel_launches(oll_engine *e, int cap, double *bytes, double *ms, int *kind,
+                               int *n);
 
 #ifdef __cplusplus
 }

@@ -367,6 +367,8 @@ int oll_set_kernel_timing(oll_hier *h, int on) {
   h->timing.on = on != 0;
   h->timing.used = 0;
   h->timing.bytes = 0;
+  h->timing.launch_bytes.clear();
+  h->timing.launch_kind.clear();
   return OLL_OK;
 }
 
@@ -637,6 +639,25 @@ int oll_engine_set_kernel_timing(oll_engine *e, int on) {
   e->timing.on = on != 0;
   e->timing.used = 0;
   e->timing.bytes = 0;
+  e->timing.launch_bytes.clear();
+  e->timing.launch_kind.clear();
+  return OLL_OK;
+}
+
+int oll_engine_kernel_launches(oll_engine *e, int cap, double *bytes, double *ms, int *kind,
+                               int *n) {
+  if (!e || !n) return fail(OLL_EINVAL, "null argument");
+  CAPI_CHECK(hipStreamSynchronize(e->stream));
+  const Timing &t = e->timing;
+  const int total = int(std::min(t.launch_bytes.size(), t.used / 2));
+  *n = total;
+  for (int i = 0; i < total && i < cap; ++i) {
+    float m = 0;
+    CAPI_CHECK(hipEventElapsedTime(&m, t.ev[size_t(2 * i)], t.ev[size_t(2 * i + 1)]));
+    if (bytes) bytes[i] = t.launch_bytes[size_t(i)];
+    if (ms) ms[i] = m;
+    if (kind) kind[i] = t.launch_kind[size_t(i)];
+  }
   return OLL_OK;
 }
 

@@ -740,6 +740,8 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
     if (e1) {
       (void)hipEventRecord(e1, s);
       t->bytes += L.bytes;
+      t->launch_bytes.push_back(L.bytes);
+      t->launch_kind.push_back(L.kind == Launch::SMOOTH32 ? 1 : 0);
     }
   }
   ENG_CHECK(hipGetLastError());

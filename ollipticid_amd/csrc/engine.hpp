@@ -116,6 +116,8 @@ struct Timing {
   std::vector<hipEvent_t> ev;
   size_t used = 0;
   double bytes = 0;
+  std::vector<double> launch_bytes;  // per timed launch, in issue order
+  std::vector<int> launch_kind;      // 0: FP64 smoother, 1: FP32 correction smoother
 };
 
 // the full V-cycle of every member; all members must share nu1, nu2

@@ -39,17 +39,6 @@ constexpr double kEps = DBL_EPSILON;
 // point operators
 // ------------------------------------------------------------------
 
-// offsets of the i-neighbours i+m (xp[m-1]) and i-m (xm[m-1]), m = 1..NC-1,
-// of a point in sub-row s of a colour-split row with rl slots per sub-row
-template <int NC>
-__device__ __forceinline__ void xoffs(int s, int rl, int *xp, int *xm) {
-#pragma unroll
-  for (int m = 1; m < NC; ++m) {
-    xp[m - 1] = (s + m < NC) ? m * rl : (m - NC) * rl + 1;
-    xm[m - 1] = (s >= m) ? -m * rl : (NC - m) * rl - 1;
-  }
-}
-
 // axis-star second differences (explicit fma; shared by all stencil kernels)
 template <int ORD, typename T>
 __device__ __forceinline__ T star(T u0, T s1, T s2, T s3, T s4) {
@@ -72,33 +61,23 @@ __device__ __forceinline__ T star_sum(T ax, T ay, T az, T ix2, T iy2, T iz2) {
   else return s * (T(1) / T(5040));
 }
 
-// axis-star Laplacian at storage index p; i-neighbours at p + xp[m-1] and
-// p + xm[m-1], j- and k-neighbours m * sy and m * sz away
+// axis-star Laplacian at element index p of an array (LDS); neighbour
+// element offsets from stencil_offs<NC, 1>
 template <int ORD, typename T = double>
-__device__ __forceinline__ T lap(const T *__restrict__ u, long long p, const int *xp,
-                                 const int *xm, long long sy, long long sz, T ix2, T iy2, T iz2) {
+__device__ __forceinline__ T lap(const T *__restrict__ u, int p, const Nbr<ORD / 2 + 1> &n, T ix2,
+                                 T iy2, T iz2) {
   constexpr int R = ORD / 2;
   T sx[4] = {0, 0, 0, 0}, syv[4] = {0, 0, 0, 0}, szv[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int m = 1; m <= R; ++m) {
-    sx[m - 1] = u[p + xp[m - 1]] + u[p + xm[m - 1]];
-    syv[m - 1] = u[p + m * sy] + u[p - m * sy];
-    szv[m - 1] = u[p + m * sz] + u[p - m * sz];
+    sx[m - 1] = u[p + int(n.xp[m - 1])] + u[p + int(n.xm[m - 1])];
+    syv[m - 1] = u[p + int(n.yp[m - 1])] + u[p + int(n.ym[m - 1])];
+    szv[m - 1] = u[p + int(n.zp[m - 1])] + u[p + int(n.zm[m - 1])];
   }
   const T u0 = u[p];
   return star_sum<ORD, T>(star<ORD, T>(u0, sx[0], sx[1], sx[2], sx[3]),
                           star<ORD, T>(u0, syv[0], syv[1], syv[2], syv[3]),
                           star<ORD, T>(u0, szv[0], szv[1], szv[2], szv[3]), ix2, iy2, iz2);
-}
-
-// Laplacian of a box at (i, j, k), storage index p
-template <int ORD>
-__device__ __forceinline__ double lap_at(const DevBox &B, const double *__restrict__ u, long long p,
-                                         int i) {
-  constexpr int NC = ORD / 2 + 1;
-  int xp[NC - 1], xm[NC - 1];
-  xoffs<NC>(i % NC, B.rl, xp, xm);
-  return lap<ORD>(u, p, xp, xm, B.rs, B.ps, B.idx2[0], B.idx2[1], B.idx2[2]);
 }
 
 // N(u) = Lap u + b (c+u)^-7 [+ d (c+u) + e (c+u)^5] and dN/du_p
@@ -197,10 +176,12 @@ typedef __attribute__((address_space(1))) char gchar;
 __device__ __forceinline__ ConstBox &cbox(const DevBox *b) { return *(ConstBox *)b; }
 // the generic helpers (robin_op_at) take a generic reference
 __device__ __forceinline__ const DevBox &cbox_plain(ConstBox &b) { return *(const DevBox *)&b; }
-// offset of point i within its row (kernels.hpp ioff) from a constant-space box
-__device__ __forceinline__ int ioffc(ConstBox &B, int i) {
-  const int q = int((unsigned(i) * B.ncm) >> 16);
-  return (i - q * B.nc) * B.rl + q;
+// x / nc and the colour of (i, j, k) from a constant-space box (kernels.hpp
+// split_div, colour_of)
+__device__ __forceinline__ unsigned divc(ConstBox &B, unsigned x) { return (x * B.ncm) >> 16; }
+__device__ __forceinline__ unsigned colourc(ConstBox &B, int i, int j, int k) {
+  const unsigned s = unsigned(i + j + k);
+  return s - divc(B, s) * unsigned(B.nc);
 }
 
 template <typename T>
@@ -212,18 +193,18 @@ __device__ __forceinline__ void st(void *base, unsigned off, T v) {
   *(__attribute__((address_space(1))) T *)((gchar *)base + off) = v;
 }
 
-// axis-star Laplacian at byte offset o: i-neighbours at o + xp[m-1] and
-// o + xm[m-1] (bytes), j- and k-neighbours m * sy and m * sz bytes away
+// axis-star Laplacian at byte offset o of a field; neighbour byte offsets
+// from stencil_offs<NC, sizeof(T)>
 template <int ORD, typename T>
-__device__ __forceinline__ T lapb(const void *u, unsigned o, const int *xp, const int *xm,
-                                  unsigned sy, unsigned sz, T ix2, T iy2, T iz2) {
+__device__ __forceinline__ T lapb(const void *u, unsigned o, const Nbr<ORD / 2 + 1> &n, T ix2, T iy2,
+                                  T iz2) {
   constexpr int R = ORD / 2;
   T sx[4] = {0, 0, 0, 0}, syv[4] = {0, 0, 0, 0}, szv[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int m = 1; m <= R; ++m) {
-    sx[m - 1] = ld<T>(u, o + xp[m - 1]) + ld<T>(u, o + xm[m - 1]);
-    syv[m - 1] = ld<T>(u, o + m * sy) + ld<T>(u, o - m * sy);
-    szv[m - 1] = ld<T>(u, o + m * sz) + ld<T>(u, o - m * sz);
+    sx[m - 1] = ld<T>(u, o + n.xp[m - 1]) + ld<T>(u, o + n.xm[m - 1]);
+    syv[m - 1] = ld<T>(u, o + n.yp[m - 1]) + ld<T>(u, o + n.ym[m - 1]);
+    szv[m - 1] = ld<T>(u, o + n.zp[m - 1]) + ld<T>(u, o + n.zm[m - 1]);
   }
   const T u0 = ld<T>(u, o);
   return star_sum<ORD, T>(star<ORD, T>(u0, sx[0], sx[1], sx[2], sx[3]),
@@ -278,52 +259,53 @@ __device__ __forceinline__ void row_jk(ConstBox &B, unsigned r, int &j, int &k) 
   j = R + int(r - kq * nyi);
 }
 
-// colour passes: the thread's slot t of local row lr:
-// element index p of its colour point and the sub-row; false when the row
-// is past the block or the slot past the row's colour points
+// colour passes: the thread's slot t of local row lr: element index p of
+// its point of colour `colour` and that point's i mod NC; false when the row
+// is past the block or the slot past the row's interior colour points
 template <int R>
 __device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, unsigned lr, int t,
-                                            int colour, unsigned &p, int &sub) {
+                                            int colour, unsigned &p, int &s) {
   constexpr int NC = R + 1;
   if (lr >= unsigned(Bk.rows)) return false;
   int j, k;
   row_jk<R>(B, unsigned(Bk.row0) + lr, j, k);
+  // i of the row's first interior point of this colour
   const int first = R + int(unsigned(colour + NC * 3 * kMaxAxisPoints - j - k - R) % NC);
   if (first + t * NC >= B.n[0] - R) return false;
-  sub = first % NC;
-  p = unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) + unsigned(sub * B.rl + first / NC + t);
+  s = first % NC;
+  p = unsigned(colour) * unsigned(B.cs) + unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) +
+      unsigned(first / NC + t);
   return true;
 }
 
-// whole-row kernels: the thread's slot t of colour sub-row `sub` of an
-// interior row; false for ghost and padding slots
+// whole-row kernels: sub-row sr = NC * row + c is colour c of interior row
+// `row`; the thread owns slot t of it.  Element index p, colour c and
+// i mod NC; false for ghost and padding slots
 template <int R>
 __device__ __forceinline__ bool row_slot(const RowBlock &Bk, ConstBox &B, unsigned per_row,
-                                         float inv_per_row, unsigned &p, int &sub) {
+                                         float inv_per_row, unsigned &p, int &c, int &s) {
   constexpr int NC = R + 1;
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   if (lr >= unsigned(Bk.rows)) return false;
   const unsigned sr = unsigned(Bk.row0) + lr;
   const unsigned row = sr / NC;
-  sub = int(sr - row * NC);
-  const int i = sub + t * NC;
-  if (i < R || i >= B.n[0] - R || t >= B.rl) return false;
+  c = int(sr - row * NC);
   int j, k;
   row_jk<R>(B, row, j, k);
-  p = unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) + unsigned(sub * B.rl + t);
+  s = int(unsigned(c + NC * 3 * kMaxAxisPoints - j - k) % NC);
+  const int i = s + t * NC;
+  if (i < R || i >= B.n[0] - R) return false;
+  p = unsigned(c) * unsigned(B.cs) + unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) +
+      unsigned(t);
   return true;
 }
 
-// byte offsets of the i-neighbours of a point in sub-row `sub`
+// neighbour byte offsets (ES bytes per element) of a point of colour c
+// with i mod NC = s in box B
 template <int NC, int ES>
-__device__ __forceinline__ void xoffs_bytes(int sub, int rl, int *xp, int *xm) {
-  xoffs<NC>(sub, rl, xp, xm);
-#pragma unroll
-  for (int m = 0; m < NC - 1; ++m) {
-    xp[m] *= ES;
-    xm[m] *= ES;
-  }
+__device__ __forceinline__ Nbr<NC> box_offs(ConstBox &B, int c, int s) {
+  return stencil_offs<NC, ES>(c, s, unsigned(B.cs), unsigned(B.rs), unsigned(B.ps));
 }
 
 // ------------------------------------------------------------------
@@ -344,14 +326,12 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restri
   const int t = int(threadIdx.x - lr * per_row);
   double rmax = 0.0;
   unsigned p;
-  int sub;
-  if (colour_slot<R>(Bk, B, lr, t, colour, p, sub)) {
-    int xp[NC - 1], xm[NC - 1];
-    xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
+  int s;
+  if (colour_slot<R>(Bk, B, lr, t, colour, p, s)) {
+    const Nbr<NC> nb = box_offs<NC, 8>(B, colour, s);
     const unsigned o = p * 8u;
     double *u = B.f[F_U];
-    const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
-                                        B.idx2[0], B.idx2[1], B.idx2[2]);
+    const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
     const double u0 = ld<double>(u, o);
     double N, dN;
     nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + u0,
@@ -376,14 +356,12 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
   ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
   unsigned p;
-  int sub;
-  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, sub)) {
-    int xp[NC - 1], xm[NC - 1];
-    xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
+  int c, s;
+  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s)) {
+    const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u;
     const double *u = B.f[F_U];
-    const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
-                                        B.idx2[0], B.idx2[1], B.idx2[2]);
+    const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
     double N, dN;
     nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + ld<double>(u, o),
                   EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
@@ -414,14 +392,12 @@ __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__rest
   ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
   unsigned p;
-  int sub;
-  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, sub)) {
-    int xp[NC - 1], xm[NC - 1];
-    xoffs_bytes<NC, 8>(sub, B.rl, xp, xm);
+  int c, s;
+  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s)) {
+    const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u, o4 = p * 4u;
     const double *u = B.f[F_U];
-    const double lu = lapb<ORD, double>(u, o, xp, xm, unsigned(B.rs) * 8u, unsigned(B.ps) * 8u,
-                                        B.idx2[0], B.idx2[1], B.idx2[2]);
+    const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
     double N, dN;
     nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + ld<double>(u, o),
                   EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
@@ -446,14 +422,12 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__rest
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   unsigned p;
-  int sub;
-  if (!colour_slot<R>(Bk, B, lr, t, colour, p, sub)) return;
-  int xp[NC - 1], xm[NC - 1];
-  xoffs_bytes<NC, 4>(sub, B.rl, xp, xm);
+  int s;
+  if (!colour_slot<R>(Bk, B, lr, t, colour, p, s)) return;
+  const Nbr<NC> nb = box_offs<NC, 4>(B, colour, s);
   const unsigned o = p * 4u;
   float *e = B.g[G_E];
-  const float le = lapb<ORD, float>(e, o, xp, xm, unsigned(B.rs) * 4u, unsigned(B.ps) * 4u,
-                                    float(B.idx2[0]), float(B.idx2[1]), float(B.idx2[2]));
+  const float le = lapb<ORD, float>(e, o, nb, float(B.idx2[0]), float(B.idx2[1]), float(B.idx2[2]));
   const float q = ld<float>(B.g[G_Q], o);
   const float ep = ld<float>(e, o);
   const float res = fmaf(q, ep, le) - ld<float>(B.g[G_R], o);
@@ -468,8 +442,8 @@ __global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__rest
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   unsigned p;
-  int sub;
-  if (!row_slot<R>(Bk, B, per_row, inv_per_row, p, sub)) return;
+  int c, s;
+  if (!row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s)) return;
   double *u = B.f[F_U];
   st<double>(u, p * 8u, ld<double>(u, p * 8u) + double(ld<float>(B.g[G_E], p * 4u)));
 }
@@ -478,22 +452,21 @@ __global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__rest
 // Robin boundary r^2/n sum_d (1/x_d) d_d u + u = S on the ghost shell
 // ------------------------------------------------------------------
 
-// value of u at offset m along axis a from the point (i, j, k) whose
-// storage index is p
-__device__ __forceinline__ double axis_at(const DevBox &B, const double *__restrict__ u,
-                                          long long p, int a, int i, int m) {
-  if (a == 0) return u[p - ioff(B, i) + ioff(B, i + m)];
-  return u[p + (long long)m * (a == 1 ? (long long)B.rs : B.ps)];
+// value of u at offset m along axis a from the point (i, j, k)
+__device__ __forceinline__ double axis_at(const DevBox &B, const double *__restrict__ u, int i, int j,
+                                          int k, int a, int m) {
+  return u[pidx(B, i + (a == 0 ? m : 0), j + (a == 1 ? m : 0), k + (a == 2 ? m : 0))];
 }
 
-// first derivative along axis a at (i, j, k) (ia = index along a):
+// first derivative along axis a at (i, j, k), storage index p:
 // dx2 (diff_2nd.cc:237-335) for order 2, dx4 (diff_4th.cc:6-89) otherwise.
 // coef = coefficient of u at the point itself.
 __device__ __forceinline__ double deriv(const DevBox &B, const double *__restrict__ u, long long p,
-                                        int a, int i, int ia, double &coef) {
+                                        int a, int i, int j, int k, double &coef) {
   const int n = B.n[a];
+  const int ia = a == 0 ? i : (a == 1 ? j : k);
   const double id = B.idx[a];
-#define UA(m) axis_at(B, u, p, a, i, (m))
+#define UA(m) axis_at(B, u, i, j, k, a, (m))
   if (B.order == 2) {
     if (ia == 0) {
       coef = -3.0 * 0.5 * id;
@@ -546,7 +519,7 @@ __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, i
   for (int a = 0; a < 3; ++a) {
     if (!(mask & (1 << a))) continue;
     double cf;
-    const double dv = deriv(B, u, p, a, i, ijk[a], cf);
+    const double dv = deriv(B, u, p, a, i, j, k, cf);
     sF += dv / X[a];
     sdF += cf / X[a];
   }
@@ -689,7 +662,7 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int
   for (int a = 0; a < 3; ++a) {
     if (!(mask & (1 << a))) continue;
     double cf;
-    sF += deriv(B, u, p, a, i, ijk[a], cf) / X[a];
+    sF += deriv(B, u, p, a, i, j, k, cf) / X[a];
   }
   return r2 * B.ipn * sF + u[p];
 }
@@ -701,8 +674,11 @@ __device__ __forceinline__ double robin_op_at(const DevBox &B, int i, int j, int
 typedef const __attribute__((address_space(4))) DevXfer ConstXfer;
 typedef const __attribute__((address_space(4))) PtBlock ConstPtBlock;
 
+// byte offset of point (i, j, k) of a float64 field
 __device__ __forceinline__ unsigned boff(ConstBox &S, int i, int j, int k) {
-  return (unsigned(k) * unsigned(S.ps) + unsigned(j) * unsigned(S.rs) + unsigned(ioffc(S, i))) * 8u;
+  return (colourc(S, i, j, k) * unsigned(S.cs) + unsigned(k) * unsigned(S.ps) +
+          unsigned(j) * unsigned(S.rs) + divc(S, unsigned(i))) *
+         8u;
 }
 
 // the block's target point in lexicographic order (point q0 + thread,
@@ -759,17 +735,24 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
   double wz[KP];
 #pragma unroll
   for (int K = 0; K < KP; ++K) wz[K] = w[2] ? w[2][K] : 1.0;
-  const unsigned sy = unsigned(S.rs) * 8u, sz = unsigned(S.ps) * 8u;
+  // a step along k moves one plane and one colour block on
+  const unsigned sz = unsigned(S.ps) * 8u, cs = unsigned(S.cs) * 8u;
+  const unsigned nc = unsigned(S.nc), wrap = sz - (nc - 1u) * cs;
   double acc = 0.0;
   for (int I = 0; I < kk[0]; ++I) {
-    const unsigned xo = unsigned(ioffc(S, s0[0] + I)) * 8u;
     double uj = 0.0;
     for (int J = 0; J < kk[1]; ++J) {
-      const unsigned base = xo + unsigned(s0[1] + J) * sy + unsigned(s0[2]) * sz;
+      unsigned off = boff(S, s0[0] + I, s0[1] + J, s0[2]);
+      unsigned c = colourc(S, s0[0] + I, s0[1] + J, s0[2]);
       double uk = 0.0;
 #pragma unroll
       for (int K = 0; K < KP; ++K)
-        if (K < kk[2]) uk += ld<double>(f, base + unsigned(K) * sz) * wz[K];
+        if (K < kk[2]) {
+          uk += ld<double>(f, off) * wz[K];
+          const bool last = c + 1u == nc;
+          off += last ? wrap : sz + cs;
+          c = last ? 0u : c + 1u;
+        }
       uj += uk * (w[1] ? w[1][J] : 1.0);
     }
     acc += uj * (w[0] ? w[0][I] : 1.0);
@@ -781,18 +764,15 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
 __device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, const int t[3]) {
   const void *u = S.f[F_U];
   const void *ww = S.f[F_WW];
-  const unsigned sy = unsigned(S.rs) * 8u, sz = unsigned(S.ps) * 8u;
-  const int x0 = X.ax[0].start[t[0]];
-  const unsigned b = boff(S, x0, X.ax[1].start[t[1]], X.ax[2].start[t[2]]);
-  const unsigned x1 = unsigned(ioffc(S, x0 + 1) - ioffc(S, x0)) * 8u;
+  const int x0 = X.ax[0].start[t[0]], y0 = X.ax[1].start[t[1]], z0 = X.ax[2].start[t[2]];
   const double *wx = X.ax[0].w + 2 * t[0];
   const double *wy = X.ax[1].w + 2 * t[1];
   const double *wzp = X.ax[2].w + 2 * t[2];
   const double dxi = wx[0], dxf = wx[1], dyi = wy[0], dyf = wy[1], dzi = wzp[0], dzf = wzp[1];
-#define V(o) (ld<double>(u, b + (o)) - ld<double>(ww, b + (o)))
-  return ((V(0) * dxi + V(x1) * dxf) * dyi + (V(sy) * dxi + V(sy + x1) * dxf) * dyf) * dzi +
-         ((V(sz) * dxi + V(sz + x1) * dxf) * dyi + (V(sz + sy) * dxi + V(sz + sy + x1) * dxf) * dyf) *
-             dzf;
+#define V(a, b, c) \
+  (ld<double>(u, boff(S, x0 + a, y0 + b, z0 + c)) - ld<double>(ww, boff(S, x0 + a, y0 + b, z0 + c)))
+  return ((V(0, 0, 0) * dxi + V(1, 0, 0) * dxf) * dyi + (V(0, 1, 0) * dxi + V(1, 1, 0) * dxf) * dyf) * dzi +
+         ((V(0, 0, 1) * dxi + V(1, 0, 1) * dxf) * dyi + (V(0, 1, 1) * dxi + V(1, 1, 1) * dxf) * dyf) * dzf;
 #undef V
 }
 
@@ -845,18 +825,16 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
     ConstBox &S = cbox(X.src);
     const void *w = S.f[F_W];
     if (f & 8) {
-      const unsigned sy = unsigned(S.rs) * 8u, sz = unsigned(S.ps) * 8u;
-      const int ci = X.ax[0].coinc[i];
-      const unsigned c = boff(S, ci, X.ax[1].coinc[j], X.ax[2].coinc[k]);
-      const int xo = ioffc(S, ci);
-      const int xm = (ioffc(S, ci - 1) - xo) * 8, xp = (ioffc(S, ci + 1) - xo) * 8;
+      const int ci = X.ax[0].coinc[i], cj = X.ax[1].coinc[j], ck = X.ax[2].coinc[k];
       double acc = 0.0;
       for (int dz = -1; dz <= 1; ++dz) {
         const double wz = dz == 0 ? 0.5 : 0.25;
         for (int dy = -1; dy <= 1; ++dy) {
           const double wy = dy == 0 ? 0.5 : 0.25;
-          const unsigned r = c + unsigned(dy) * sy + unsigned(dz) * sz;
-          acc += wz * wy * (0.25 * ld<double>(w, r + xm) + 0.5 * ld<double>(w, r) + 0.25 * ld<double>(w, r + xp));
+          const double wm = ld<double>(w, boff(S, ci - 1, cj + dy, ck + dz));
+          const double w0 = ld<double>(w, boff(S, ci, cj + dy, ck + dz));
+          const double wp = ld<double>(w, boff(S, ci + 1, cj + dy, ck + dz));
+          acc += wz * wy * (0.25 * wm + 0.5 * w0 + 0.25 * wp);
         }
       }
       wc = acc;
@@ -871,11 +849,9 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
   double Rc;
   if (inside) {
     constexpr int NC = ORD / 2 + 1;
-    int xp[NC - 1], xm[NC - 1];
-    xoffs_bytes<NC, 8>(i % NC, T.rl, xp, xm);
+    const Nbr<NC> nb = box_offs<NC, 8>(T, int(colourc(T, i, j, k)), i % NC);
     double dN;
-    nonlinear<EQ>(lapb<ORD, double>(T.f[F_U], o, xp, xm, unsigned(T.rs) * 8u, unsigned(T.ps) * 8u,
-                                    T.idx2[0], T.idx2[1], T.idx2[2]),
+    nonlinear<EQ>(lapb<ORD, double>(T.f[F_U], o, nb, T.idx2[0], T.idx2[1], T.idx2[2]),
                   T.dulap, ld<double>(T.f[F_B], o), ld<double>(T.f[F_C], o) + uc,
                   EQ ? ld<double>(T.f[F_D], o) : 0.0, EQ ? ld<double>(T.f[F_E], o) : 0.0, Rc, dN);
   } else {
@@ -984,18 +960,17 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
 // ------------------------------------------------------------------
 struct CoarseGeo {
   int n[3];
-  int rl, rs, ps, nc;
+  int rl, rs, ps, cs, nc;
   unsigned ncm;
   double lo[3], d[3], idx[3], idx2[3];
   double ipn, dulap;
 };
 
-__device__ __forceinline__ int coarse_ioff(const CoarseGeo &G, int i) {
-  const int q = int((unsigned(i) * G.ncm) >> 16);
-  return (i - q * G.nc) * G.rl + q;
-}
+// element index of (i, j, k) in the LDS copy (the box's colour-block layout)
 __device__ __forceinline__ int coarse_p(const CoarseGeo &G, int i, int j, int k) {
-  return k * G.ps + j * G.rs + coarse_ioff(G, i);
+  const unsigned s = unsigned(i + j + k);
+  const int c = int(s - ((s * G.ncm) >> 16) * unsigned(G.nc));
+  return c * G.cs + k * G.ps + j * G.rs + int((unsigned(i) * G.ncm) >> 16);
 }
 
 // Robin update at shell point (i, j, k) of the LDS field; mask has bit a
@@ -1015,16 +990,9 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
     if (!(mask & (1 << a))) continue;
     const bool lo = ijk[a] == 0;
     const int dir = lo ? 1 : -1;
-    int o1, o2;
-    if (a == 0) {
-      const int x0 = coarse_ioff(G, i);
-      o1 = coarse_ioff(G, i + dir) - x0;
-      o2 = coarse_ioff(G, i + 2 * dir) - x0;
-    } else {
-      const int st = a == 1 ? G.rs : G.ps;
-      o1 = dir * st;
-      o2 = 2 * dir * st;
-    }
+    const int e0 = a == 0 ? dir : 0, e1 = a == 1 ? dir : 0, e2 = a == 2 ? dir : 0;
+    const int o1 = coarse_p(G, i + e0, j + e1, k + e2) - p;
+    const int o2 = coarse_p(G, i + 2 * e0, j + 2 * e1, k + 2 * e2) - p;
     const double id = G.idx[a];
     double cf, dv;
     if (lo) {
@@ -1066,6 +1034,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
   G.rl = B.rl;
   G.rs = B.rs;
   G.ps = int(B.ps);
+  G.cs = int(B.cs);
   G.nc = B.nc;
   G.ncm = B.ncm;
   G.ipn = B.ipn;
@@ -1170,9 +1139,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
         const int q = col * OWN + m;
         if (own_p[q] < 0) continue;
         const int p = own_p[q] & 0x3fffffff;
-        int xp[1], xm[1];
-        xoffs<2>(own_p[q] >> 30, G.rl, xp, xm);
-        const double lu = lap<2>(su, p, xp, xm, G.rs, G.ps, G.idx2[0], G.idx2[1], G.idx2[2]);
+        const Nbr<2> nb = stencil_offs<2, 1>(col, own_p[q] >> 30, unsigned(G.cs), unsigned(G.rs),
+                                             unsigned(G.ps));
+        const double lu = lap<2>(su, p, nb, G.idx2[0], G.idx2[1], G.idx2[2]);
         double N, dN;
         nonlinear<EQ>(lu, G.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
         const double res = N - srho[p];
@@ -1360,7 +1329,7 @@ __device__ double dU_at(const DevBox &B, const double ctr[3], int i, int j, int 
   for (int a = 0; a < 3; ++a) {
     X[a] = coord(B, a, ijk[a]) - ctr[a];
     double cf;
-    g[a] = deriv(B, B.f[F_C], p, a, i, ijk[a], cf) + deriv(B, B.f[F_U], p, a, i, ijk[a], cf);
+    g[a] = deriv(B, B.f[F_C], p, a, i, j, k, cf) + deriv(B, B.f[F_U], p, a, i, j, k, cf);
   }
   const double ir = 1.0 / sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
   return ir * (X[0] * g[0] + X[1] * g[1] + X[2] * g[2]);
@@ -1424,7 +1393,6 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
     c[a] = cell_of(B, a, X[a]);
     if (!(fabs(X[a] - coord(B, a, c[a])) <= 3 * kEps)) coinc = false;
   }
-  const long long sy = B.rs, sz = B.ps;
   if (coinc) {
     out[q] = f[pidx(B, c[0], c[1], c[2])];
     return;
@@ -1448,11 +1416,10 @@ __global__ __launch_bounds__(64) void k_interp_points(const DevBox *__restrict__
   }
   double acc = 0.0;
   for (int I = 0; I <= ord; ++I) {
-    const int xo = ioff(B, s[0] + I);
     double uj = 0.0;
     for (int J = 0; J <= ord; ++J) {
       double uk = 0.0;
-      for (int K = 0; K <= ord; ++K) uk += f[xo + sy * (s[1] + J) + sz * (s[2] + K)] * w[2][K];
+      for (int K = 0; K <= ord; ++K) uk += f[pidx(B, s[0] + I, s[1] + J, s[2] + K)] * w[2][K];
       uj += uk * w[1][J];
     }
     acc += uj * w[0][I];

@@ -2,14 +2,17 @@
 //
 // HBM layout: one slab per hierarchy (request).  Each field (u, b, c, d, e,
 // rho, r, v, w, ww) of a box is a contiguous float64 array covering the box
-// including its ghost shell of width R.  Rows (fixed j, k) are stored
-// colour-split: a row holds nc = R+1 sub-rows of rl = ceil(n0/nc) slots and
-// point i sits in sub-row i mod nc at slot i / nc.  The points of one colour
-// (i+j+k) mod nc of a row are then contiguous, so a colour pass of the
-// smoother reads and writes unit-stride runs; neighbours along j and k are
-// rs and ps elements away, neighbours along i are in the adjacent sub-rows.
-// A DevBox carries the base pointers of its fields, so one launch can work
-// on boxes of many hierarchies.  Launches read device-resident item lists.
+// including its ghost shell of width R, stored by colour: the points of
+// colour c = (i+j+k) mod nc (nc = R+1, the smoother's colours) form block c
+// of the array, and inside a block point (i, j, k) sits in row (j, k) at
+// slot i / nc (rl = ceil(n0/nc) slots per row, rs = rl, plane stride ps,
+// block stride cs).  A colour pass of the smoother then reads its own
+// points' u, b, c, rho as one unit-stride stream per field and the
+// neighbours from the other blocks, rows and planes as further unit-stride
+// streams; every neighbour of a point (one step of m < nc along any axis)
+// lies in block (c+m) mod nc.  A DevBox carries the base pointers of its
+// fields, so one launch can work on boxes of many hierarchies.  Launches
+// read device-resident item lists.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,12 +37,13 @@ struct DevBox {
   double idx2[3];    // (1/d)^2
   double dulap;      // diagonal of the Laplacian stencil
   double ipn;        // 1 / Robin fall-off power
-  int nc;            // colours = sub-rows per row (R+1)
-  int rl;            // slots per sub-row, ceil(n0 / nc)
-  int rs;            // row stride, nc * rl
-  unsigned ncm;      // ceil(65536 / nc): i / nc == (i * ncm) >> 16 for i < 4096
-  long long ps;      // plane stride, rs * n1
-  long long size;    // elements per field, ps * n2
+  int nc;            // colours (R+1)
+  int rl;            // slots per row of a colour block, ceil(n0 / nc)
+  int rs;            // row stride inside a colour block (= rl)
+  unsigned ncm;      // ceil(65536 / nc): x / nc == (x * ncm) >> 16 for x < 16384
+  long long ps;      // plane stride inside a colour block, rs * n1
+  long long cs;      // colour block stride, ps * n2 rounded up to 32 elements
+  long long size;    // elements per field, cs * nc
   float inyi;        // 1 / (n1 - 2R): interior row -> (j, k) by float division
   float in0, in01;   // 1 / n0, 1 / (n0 n1): point index -> (i, j, k)
   double *f[NFIELDS];
@@ -48,32 +52,65 @@ struct DevBox {
 
 constexpr int kMaxAxisPoints = 4096;
 
-__host__ __device__ inline int split_div(const DevBox &B, int i) {
-  return int((unsigned(i) * B.ncm) >> 16);
+// x / nc for 0 <= x < 16384
+__host__ __device__ inline int split_div(const DevBox &B, int x) {
+  return int((unsigned(x) * B.ncm) >> 16);
 }
-// offset of point i within its row
-__host__ __device__ inline int ioff(const DevBox &B, int i) {
-  const int q = split_div(B, i);
-  return (i - q * B.nc) * B.rl + q;
+// colour of point (i, j, k)
+__host__ __device__ inline int colour_of(const DevBox &B, int i, int j, int k) {
+  const int s = i + j + k;
+  return s - split_div(B, s) * B.nc;
+}
+// block shift of a step of m (|m| < nc) from colour c: colour(c+m) - c
+__host__ __device__ inline int colour_shift(int nc, int c, int m) {
+  return c + m >= nc ? m - nc : (c + m < 0 ? m + nc : m);
 }
 __host__ __device__ inline long long pidx(const DevBox &B, int i, int j, int k) {
-  return (long long)k * B.ps + (long long)j * B.rs + ioff(B, i);
+  return (long long)colour_of(B, i, j, k) * B.cs + (long long)k * B.ps + (long long)j * B.rs +
+         split_div(B, i);
 }
 // fills the layout members from n, R
 inline void set_layout(DevBox &B) {
   B.nc = B.R + 1;
   B.rl = (B.n[0] + B.nc - 1) / B.nc;
-  B.rs = B.nc * B.rl;
+  B.rs = B.rl;
   B.ncm = (65536u + unsigned(B.nc) - 1u) / unsigned(B.nc);
   B.ps = (long long)B.rs * B.n[1];
-  B.size = B.ps * B.n[2];
+  B.cs = (B.ps * B.n[2] + 31) & ~31LL;
+  B.size = B.cs * B.nc;
   B.inyi = 1.0f / float(B.n[1] - 2 * B.R);
   B.in0 = 1.0f / float(B.n[0]);
   B.in01 = 1.0f / float(B.n[0] * B.n[1]);
 }
+// Offsets, in units of ES bytes and modulo 2^32, from a point of colour c
+// whose i is s mod NC to its neighbours at +-m (m = 1..NC-1) along x, y
+// and z: the neighbour lies in colour block (c+m) mod NC, in the same slot
+// one row (rs) or plane (ps) away, or along x in the next or previous slot.
+template <int NC>
+struct Nbr {
+  unsigned xp[NC - 1], xm[NC - 1], yp[NC - 1], ym[NC - 1], zp[NC - 1], zm[NC - 1];
+};
+template <int NC, int ES>
+__host__ __device__ inline Nbr<NC> stencil_offs(int c, int s, unsigned cs, unsigned rs, unsigned ps) {
+  Nbr<NC> n;
+#pragma unroll
+  for (int m = 1; m < NC; ++m) {
+    const unsigned up = unsigned(colour_shift(NC, c, m)) * cs;
+    const unsigned dn = unsigned(colour_shift(NC, c, -m)) * cs;
+    n.xp[m - 1] = (up + (s + m >= NC ? 1u : 0u)) * unsigned(ES);
+    n.xm[m - 1] = (dn - (s - m < 0 ? 1u : 0u)) * unsigned(ES);
+    n.yp[m - 1] = (up + unsigned(m) * rs) * unsigned(ES);
+    n.ym[m - 1] = (dn - unsigned(m) * rs) * unsigned(ES);
+    n.zp[m - 1] = (up + unsigned(m) * ps) * unsigned(ES);
+    n.zm[m - 1] = (dn - unsigned(m) * ps) * unsigned(ES);
+  }
+  return n;
+}
+
 inline long long layout_size(const int n[3], int R) {
   const int nc = R + 1;
-  return (long long)nc * ((n[0] + nc - 1) / nc) * n[1] * n[2];
+  const long long block = (long long)((n[0] + nc - 1) / nc) * n[1] * n[2];
+  return nc * ((block + 31) & ~31LL);
 }
 
 // one axis of a transfer table (geometry.hpp AxisTable)

@@ -100,6 +100,9 @@ SIGNATURES = {
     "oll_engine_set_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "oll_engine_kernel_timing": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                            C.POINTER(C.c_double)]),
+    "oll_engine_kernel_launches": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]),
     "oll_engine_profile": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_longlong)]),
@@ -480,6 +483,19 @@ class Engine:
         n, ms, b = C.c_int(), C.c_double(), C.c_double()
         _check(self.L.oll_engine_kernel_timing(self.h, C.byref(n), C.byref(ms), C.byref(b)))
         return n.value, ms.value, b.value
+
+    def kernel_launches(self):
+        """per timed smoother launch: (algorithmic bytes, device ms, kind) arrays,
+        kind 0 for FP64 sweep colours and 1 for FP32 correction colours"""
+        n = C.c_int()
+        _check(self.L.oll_engine_kernel_launches(self.h, 0, None, None, None, C.byref(n)))
+        b, ms = np.zeros(n.value), np.zeros(n.value)
+        k = np.zeros(n.value, dtype=np.int32)
+        _check(self.L.oll_engine_kernel_launches(
+            self.h, n.value, b.ctypes.data_as(C.POINTER(C.c_double)),
+            ms.ctypes.data_as(C.POINTER(C.c_double)), k.ctypes.data_as(C.POINTER(C.c_int)),
+            C.byref(n)))
+        return b, ms, k
 
     def profile(self):
         """host ms in schedule builds, cycle runs, admissions; rebuilds; steps"""
