@@ -210,7 +210,7 @@ int oll_get_field(oll_hier *h, int ibox, int field, double *host) {
   CAPI_CHECK(hipMemcpyAsync(st.data(), b.f[field], st.size() * sizeof(double),
                             hipMemcpyDeviceToHost, h->stream));
   CAPI_CHECK(hipStreamSynchronize(h->stream));
-  // colour-split rows -> plain lexicographic order (i fastest)
+  // colour blocks -> plain lexicographic order (i fastest)
   size_t q = 0;
   for (int k = 0; k < b.n[2]; ++k)
     for (int j = 0; j < b.n[1]; ++j)

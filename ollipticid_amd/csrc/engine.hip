@@ -161,7 +161,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   H.prob = p;
   if ((rc = geometry(p, H.geo, err))) return rc;
   const Hierarchy &g = H.geo;
-  // slab: field-major, boxes back to back (colour-split rows, kernels.hpp)
+  // slab: field-major, boxes back to back (colour blocks, kernels.hpp)
   long long total = 0;
   std::vector<long long> off;
   for (const Box &b : g.boxes) {
@@ -375,7 +375,7 @@ struct Builder {
   Schedule &S;
 
   // threads per row of a level: colour points per interior row (colour
-  // passes) or slots per colour sub-row (whole-row kernels)
+  // passes) or slots per colour row (whole-row kernels)
   static int per_row(const Hier &H, int level, bool subrows) {
     const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
     int m = 0;
@@ -387,7 +387,7 @@ struct Builder {
   }
 
   // row blocks of one level of one member appended to `out`: the interior
-  // rows (or colour sub-rows) of each box in runs of kRowThreads / pr
+  // rows (or colour rows) of each box in runs of kRowThreads / pr
   static void row_blocks(const Hier &H, int level, unsigned long long *norm, bool subrows, int pr,
                          std::vector<RowBlock> &out) {
     const int rpb = std::max(1, kRowThreads / pr);

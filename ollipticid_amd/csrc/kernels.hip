@@ -452,10 +452,11 @@ __global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__rest
 // Robin boundary r^2/n sum_d (1/x_d) d_d u + u = S on the ghost shell
 // ------------------------------------------------------------------
 
-// value of u at offset m along axis a from the point (i, j, k)
-__device__ __forceinline__ double axis_at(const DevBox &B, const double *__restrict__ u, int i, int j,
-                                          int k, int a, int m) {
-  return u[pidx(B, i + (a == 0 ? m : 0), j + (a == 1 ? m : 0), k + (a == 2 ? m : 0))];
+// value of u at offset m (any sign, |m| <= 4) along axis a from the point
+// at storage index p with colour c and i mod nc = s
+__device__ __forceinline__ double axis_at(const DevBox &B, const double *__restrict__ u, long long p,
+                                          int c, int s, int a, int m) {
+  return u[p + axis_offset(B, c, s, a, m)];
 }
 
 // first derivative along axis a at (i, j, k), storage index p:
@@ -466,7 +467,8 @@ __device__ __forceinline__ double deriv(const DevBox &B, const double *__restric
   const int n = B.n[a];
   const int ia = a == 0 ? i : (a == 1 ? j : k);
   const double id = B.idx[a];
-#define UA(m) axis_at(B, u, i, j, k, a, (m))
+  const int c = colour_of(B, i, j, k), s = i - split_div(B, i) * B.nc;
+#define UA(m) axis_at(B, u, p, c, s, a, (m))
   if (B.order == 2) {
     if (ia == 0) {
       coef = -3.0 * 0.5 * id;
@@ -681,6 +683,31 @@ __device__ __forceinline__ unsigned boff(ConstBox &S, int i, int j, int k) {
          8u;
 }
 
+// Walks a float64 field of box S one step at a time from a boff: every step
+// of +1 along an axis moves to the next colour block (wrapping after the
+// last), plus one row (y), one plane (z), or, along x, one slot when the
+// point's x leaves the last residue mod nc.
+struct Walk {
+  unsigned nc, cs8, wrap8, rs8, ps8;
+  __device__ __forceinline__ Walk(ConstBox &S)
+      : nc(unsigned(S.nc)), cs8(unsigned(S.cs) * 8u), wrap8((unsigned(S.nc) - 1u) * unsigned(S.cs) * 8u),
+        rs8(unsigned(S.rs) * 8u), ps8(unsigned(S.ps) * 8u) {}
+  // byte delta of the colour step from colour c; advances c
+  __device__ __forceinline__ unsigned colour(unsigned &c) const {
+    const bool last = c + 1u == nc;
+    c = last ? 0u : c + 1u;
+    return last ? 0u - wrap8 : cs8;
+  }
+  // x of residue r (mod nc) -> x + 1; advances c and r
+  __device__ __forceinline__ unsigned x(unsigned &c, unsigned &r) const {
+    const bool slot = r + 1u == nc;
+    r = slot ? 0u : r + 1u;
+    return colour(c) + (slot ? 8u : 0u);
+  }
+  __device__ __forceinline__ unsigned y(unsigned &c) const { return colour(c) + rs8; }
+  __device__ __forceinline__ unsigned z(unsigned &c) const { return colour(c) + ps8; }
+};
+
 // the block's target point in lexicographic order (point q0 + thread,
 // stepped from the block's first point so that only small offsets are
 // divided); false past the block's end
@@ -735,27 +762,28 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
   double wz[KP];
 #pragma unroll
   for (int K = 0; K < KP; ++K) wz[K] = w[2] ? w[2][K] : 1.0;
-  // a step along k moves one plane and one colour block on
-  const unsigned sz = unsigned(S.ps) * 8u, cs = unsigned(S.cs) * 8u;
-  const unsigned nc = unsigned(S.nc), wrap = sz - (nc - 1u) * cs;
+  const Walk wk(S);
+  unsigned oI = boff(S, s0[0], s0[1], s0[2]);
+  unsigned cI = colourc(S, s0[0], s0[1], s0[2]);
+  unsigned rI = unsigned(s0[0]) - divc(S, unsigned(s0[0])) * wk.nc;
   double acc = 0.0;
   for (int I = 0; I < kk[0]; ++I) {
     double uj = 0.0;
+    unsigned oJ = oI, cJ = cI;
     for (int J = 0; J < kk[1]; ++J) {
-      unsigned off = boff(S, s0[0] + I, s0[1] + J, s0[2]);
-      unsigned c = colourc(S, s0[0] + I, s0[1] + J, s0[2]);
+      unsigned off = oJ, c = cJ;
       double uk = 0.0;
 #pragma unroll
       for (int K = 0; K < KP; ++K)
         if (K < kk[2]) {
           uk += ld<double>(f, off) * wz[K];
-          const bool last = c + 1u == nc;
-          off += last ? wrap : sz + cs;
-          c = last ? 0u : c + 1u;
+          off += wk.z(c);
         }
       uj += uk * (w[1] ? w[1][J] : 1.0);
+      oJ += wk.y(cJ);
     }
     acc += uj * (w[0] ? w[0][I] : 1.0);
+    oI += wk.x(cI, rI);
   }
   return acc;
 }
@@ -769,10 +797,28 @@ __device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, cons
   const double *wy = X.ax[1].w + 2 * t[1];
   const double *wzp = X.ax[2].w + 2 * t[2];
   const double dxi = wx[0], dxf = wx[1], dyi = wy[0], dyf = wy[1], dzi = wzp[0], dzf = wzp[1];
-#define V(a, b, c) \
-  (ld<double>(u, boff(S, x0 + a, y0 + b, z0 + c)) - ld<double>(ww, boff(S, x0 + a, y0 + b, z0 + c)))
-  return ((V(0, 0, 0) * dxi + V(1, 0, 0) * dxf) * dyi + (V(0, 1, 0) * dxi + V(1, 1, 0) * dxf) * dyf) * dzi +
-         ((V(0, 0, 1) * dxi + V(1, 0, 1) * dxf) * dyi + (V(0, 1, 1) * dxi + V(1, 1, 1) * dxf) * dyf) * dzf;
+  // the eight corners, walked from (x0, y0, z0)
+  const Walk wk(S);
+  const unsigned c0 = colourc(S, x0, y0, z0);
+  unsigned r = unsigned(x0) - divc(S, unsigned(x0)) * wk.nc;
+  const unsigned o000 = boff(S, x0, y0, z0);
+  unsigned c = c0;
+  const unsigned o100 = o000 + wk.x(c, r);  // c = c0 + 1
+  unsigned c1 = c;
+  const unsigned o110 = o100 + wk.y(c1);     // c1 = c0 + 2
+  unsigned c2 = c1;
+  const unsigned o111 = o110 + wk.z(c2);
+  unsigned cb = c;
+  const unsigned o101 = o100 + wk.z(cb);
+  unsigned cy = c0;
+  const unsigned o010 = o000 + wk.y(cy);    // cy = c0 + 1
+  unsigned cz = cy;
+  const unsigned o011 = o010 + wk.z(cz);
+  unsigned cw = c0;
+  const unsigned o001 = o000 + wk.z(cw);
+#define V(o) (ld<double>(u, o) - ld<double>(ww, o))
+  return ((V(o000) * dxi + V(o100) * dxf) * dyi + (V(o010) * dxi + V(o110) * dxf) * dyf) * dzi +
+         ((V(o001) * dxi + V(o101) * dxf) * dyi + (V(o011) * dxi + V(o111) * dxf) * dyf) * dzf;
 #undef V
 }
 
@@ -826,16 +872,28 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
     const void *w = S.f[F_W];
     if (f & 8) {
       const int ci = X.ax[0].coinc[i], cj = X.ax[1].coinc[j], ck = X.ax[2].coinc[k];
+      // walk the 27 points from (ci-1, cj-1, ck-1)
+      const Walk wk(S);
+      unsigned oz = boff(S, ci - 1, cj - 1, ck - 1);
+      unsigned cz = colourc(S, ci - 1, cj - 1, ck - 1);
+      const unsigned r0 = unsigned(ci - 1) - divc(S, unsigned(ci - 1)) * wk.nc;
       double acc = 0.0;
       for (int dz = -1; dz <= 1; ++dz) {
         const double wz = dz == 0 ? 0.5 : 0.25;
+        unsigned oy = oz, cy = cz;
         for (int dy = -1; dy <= 1; ++dy) {
           const double wy = dy == 0 ? 0.5 : 0.25;
-          const double wm = ld<double>(w, boff(S, ci - 1, cj + dy, ck + dz));
-          const double w0 = ld<double>(w, boff(S, ci, cj + dy, ck + dz));
-          const double wp = ld<double>(w, boff(S, ci + 1, cj + dy, ck + dz));
+          unsigned cx = cy, r = r0;
+          const unsigned om = oy;
+          const unsigned o0 = om + wk.x(cx, r);
+          const unsigned op = o0 + wk.x(cx, r);
+          const double wm = ld<double>(w, om);
+          const double w0 = ld<double>(w, o0);
+          const double wp = ld<double>(w, op);
           acc += wz * wy * (0.25 * wm + 0.5 * w0 + 0.25 * wp);
+          oy += wk.y(cy);
         }
+        oz += wk.z(cz);
       }
       wc = acc;
     } else {
@@ -949,7 +1007,7 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
 // ------------------------------------------------------------------
 // coarsest level: {Robin, one multi-colour sweep} until the sweep norm is
 // <= eps or stops changing (elliptic.cc:1782-1816); one block per
-// hierarchy, u and rho in LDS (colour-split layout) for the whole loop, b,
+// hierarchy, u and rho in LDS (the box's colour-block layout) for the whole loop, b,
 // c (d, e) of the owned interior points in registers.  Interior updates are
 // over-relaxed by omega (omega = 1 is the reference's Gauss-Seidel-Newton);
 // the loop's fixed point, the coarse solution, does not depend on omega.

@@ -107,6 +107,26 @@ __host__ __device__ inline Nbr<NC> stencil_offs(int c, int s, unsigned cs, unsig
   return n;
 }
 
+// element offset from a point of colour c with i mod nc = s to its
+// neighbour m steps (any sign, |m| <= 4) along axis a
+__host__ __device__ inline long long axis_offset(const DevBox &B, int c, int s, int a, int m) {
+  // with |m| <= 4 and nc >= 2 two wraps either way suffice
+  const int nc = B.nc;
+  int c2 = c + m;  // the neighbour's colour, reduced into [0, nc)
+  c2 += c2 < 0 ? nc : 0;
+  c2 += c2 < 0 ? nc : 0;
+  c2 -= c2 >= nc ? nc : 0;
+  c2 -= c2 >= nc ? nc : 0;
+  long long off = (long long)(c2 - c) * B.cs;
+  if (a == 0) {
+    // slot change floor((s + m) / nc), s + m in [-4, nc + 3]
+    const int q = s + m;
+    off += (q < 0 ? -1 : 0) + (q < -nc ? -1 : 0) + (q >= nc ? 1 : 0) + (q >= 2 * nc ? 1 : 0);
+  } else {
+    off += (long long)m * (a == 1 ? (long long)B.rs : B.ps);
+  }
+  return off;
+}
 inline long long layout_size(const int n[3], int R) {
   const int nc = R + 1;
   const long long block = (long long)((n[0] + nc - 1) / nc) * n[1] * n[2];
@@ -133,8 +153,8 @@ struct DevXfer {
 // consecutive rows of one box starting at row0.  For the colour passes
 // (k_smooth, k_smooth32) a row is an interior row (j, k) and a thread owns
 // one colour point of it; for the whole-row kernels (residual, defect,
-// update) a row is one colour sub-row of an interior row and a thread owns
-// one slot of it, so a workgroup covers one contiguous run of memory.
+// update) a row is the points of one colour of an interior row (one row of
+// that colour's block) and a thread owns one slot of it.
 struct RowBlock {
   const DevBox *box;
   unsigned long long *norm;  // null: no norm
@@ -249,7 +269,7 @@ inline bool coarse_box_fits(const int n[3], long long size) {
 void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa,
                              double robin_a);
 // per_row: threads per row (colour points per interior row for the colour
-// passes, slots per sub-row for the whole-row kernels); the host sizes
+// passes, slots per colour row for the whole-row kernels); the host sizes
 // RowBlock::rows to at most kRowThreads / per_row
 // Row kernels address fields with 32-bit byte offsets: a box field must stay
 // below 4 GiB (checked when the hierarchy is built).

@@ -6,8 +6,9 @@
 //    it to 32-bit byte offsets), for element size 1, 4 and 8;
 //  * the colour passes' and whole-row kernels' slot arithmetic (first
 //    interior point of a colour in a row, element index of slot t) and the
-//    transfers' step along k (one plane and one colour block on, wrapping
-//    after the last block) agree with pidx.
+//    transfers' steps along i, j and k (one colour block on, wrapping after
+//    the last block, plus one slot, row or plane) and the Robin derivatives'
+//    axis offsets (up to 4 points either way) agree with pidx.
 // Prints "ok" and exits 0, or the first mismatch and exits 1.
 #include <cstdio>
 #include <vector>
@@ -67,14 +68,28 @@ int check_box(int R, int n0, int n1, int n2) {
         const long long p = pidx(B, i, j, k);
         if (p < 0 || p >= B.size || seen[size_t(p)]) return fail("pidx", R, n0, n1, n2, i, j, k, 0, 0);
         seen[size_t(p)] = 1;
-        // transfers: a step along k moves one plane and one block on
-        if (k + 1 < n2) {
-          const unsigned c = unsigned(colour_of(B, i, j, k));
-          const unsigned ps8 = unsigned(B.ps) * 8u, cs8 = unsigned(B.cs) * 8u;
-          const unsigned step = c + 1u == unsigned(NC) ? ps8 - unsigned(NC - 1) * cs8 : ps8 + cs8;
-          if (unsigned(p * 8) + step != unsigned(pidx(B, i, j, k + 1) * 8))
-            return fail("k step", R, n0, n1, n2, i, j, k, 2, 1);
-        }
+        // transfers walk stencils one step at a time: +1 along any axis
+        // moves one colour block on (wrapping after the last) plus one
+        // plane (k), one row (j) or, when i leaves residue NC-1, one slot
+        const unsigned c = unsigned(colour_of(B, i, j, k));
+        const unsigned cs8 = unsigned(B.cs) * 8u;
+        const unsigned dc = c + 1u == unsigned(NC) ? 0u - unsigned(NC - 1) * cs8 : cs8;
+        if (k + 1 < n2 && unsigned(p * 8) + dc + unsigned(B.ps) * 8u != unsigned(pidx(B, i, j, k + 1) * 8))
+          return fail("k step", R, n0, n1, n2, i, j, k, 2, 1);
+        if (j + 1 < n1 && unsigned(p * 8) + dc + unsigned(B.rs) * 8u != unsigned(pidx(B, i, j + 1, k) * 8))
+          return fail("j step", R, n0, n1, n2, i, j, k, 1, 1);
+        if (i + 1 < n0 &&
+            unsigned(p * 8) + dc + (i % NC == NC - 1 ? 8u : 0u) != unsigned(pidx(B, i + 1, j, k) * 8))
+          return fail("i step", R, n0, n1, n2, i, j, k, 0, 1);
+        // one-sided derivatives reach up to 4 points along an axis
+        for (int a = 0; a < 3; ++a)
+          for (int m = -4; m <= 4; ++m) {
+            int q[3] = {i, j, k};
+            q[a] += m;
+            if (q[a] < 0 || q[a] >= B.n[a]) continue;
+            if (p + axis_offset(B, int(c), i % NC, a, m) != pidx(B, q[0], q[1], q[2]))
+              return fail("axis_offset", R, n0, n1, n2, i, j, k, a, m);
+          }
       }
   // colour passes: slot t of interior row (j, k) for colour c
   for (int c = 0; c < NC; ++c)
