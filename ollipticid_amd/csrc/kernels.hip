@@ -759,9 +759,12 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
       w[a] = X.ax[a].w + t[a] * KP;
     }
   }
-  double wz[KP];
+  double wz[KP], wy[KP];
 #pragma unroll
-  for (int K = 0; K < KP; ++K) wz[K] = w[2] ? w[2][K] : 1.0;
+  for (int K = 0; K < KP; ++K) {
+    wz[K] = w[2] ? w[2][K] : 1.0;
+    wy[K] = w[1] ? w[1][K] : 1.0;
+  }
   const Walk wk(S);
   unsigned oI = boff(S, s0[0], s0[1], s0[2]);
   unsigned cI = colourc(S, s0[0], s0[1], s0[2]);
@@ -770,17 +773,42 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
   for (int I = 0; I < kk[0]; ++I) {
     double uj = 0.0;
     unsigned oJ = oI, cJ = cI;
-    for (int J = 0; J < kk[1]; ++J) {
-      unsigned off = oJ, c = cJ;
-      double uk = 0.0;
+    if constexpr (KP <= 6) {
+      // the (J, K) plane of nodes: every load of it issued before the sums
+      double v[KP][KP];
 #pragma unroll
-      for (int K = 0; K < KP; ++K)
-        if (K < kk[2]) {
-          uk += ld<double>(f, off) * wz[K];
+      for (int J = 0; J < KP; ++J) {
+        unsigned off = oJ, c = cJ;
+#pragma unroll
+        for (int K = 0; K < KP; ++K) {
+          v[J][K] = (J < kk[1] && K < kk[2]) ? ld<double>(f, off) : 0.0;
           off += wk.z(c);
         }
-      uj += uk * (w[1] ? w[1][J] : 1.0);
-      oJ += wk.y(cJ);
+        oJ += wk.y(cJ);
+      }
+#pragma unroll
+      for (int J = 0; J < KP; ++J)
+        if (J < kk[1]) {
+          double uk = 0.0;
+#pragma unroll
+          for (int K = 0; K < KP; ++K)
+            if (K < kk[2]) uk += v[J][K] * wz[K];
+          uj += uk * wy[J];
+        }
+    } else {
+      // wider stencils: one row of nodes at a time (register budget)
+      for (int J = 0; J < kk[1]; ++J) {
+        unsigned off = oJ, c = cJ;
+        double uk = 0.0;
+#pragma unroll
+        for (int K = 0; K < KP; ++K)
+          if (K < kk[2]) {
+            uk += ld<double>(f, off) * wz[K];
+            off += wk.z(c);
+          }
+        uj += uk * wy[J];
+        oJ += wk.y(cJ);
+      }
     }
     acc += uj * (w[0] ? w[0][I] : 1.0);
     oI += wk.x(cI, rI);
@@ -927,6 +955,9 @@ __global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restric
   if (!pt_lex(Bk, T, i, j, k)) return;
   if (!Bk.mg && !is_interior(T, i, j, k)) return;
   const int t[3] = {i, j, k};
+  // the target value is loaded first; its latency overlaps the source's
+  const unsigned o = boff(T, i, j, k);
+  const double u0 = ld<double>(T.f[F_U], o);
   ConstXfer *xs = (ConstXfer *)Bk.src;
   for (int x = Bk.nsrc - 1; x >= 0; --x) {
     ConstXfer &X = xs[x];
@@ -941,8 +972,7 @@ __global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restric
     } else {
       v = trilinear_corr(S, X, t);
     }
-    const unsigned o = boff(T, i, j, k);
-    st<double>(T.f[F_U], o, ld<double>(T.f[F_U], o) + v);
+    st<double>(T.f[F_U], o, u0 + v);
     return;
   }
 }
