@@ -59,11 +59,12 @@ def request_argv(rng):
     return argv
 
 
-def stream_problem(argv, precision="mixed", stop="tolerance"):
+def stream_problem(argv, precision="mixed", stop="tolerance", guess="zero"):
     import ollipticid_amd as oa
 
     p = oa.problem_from_argv(argv)
     p.precision = oa.PREC_MIXED if precision == "mixed" else oa.PREC_FP64
+    p.initial_guess = oa.GUESS_APPROX if guess == "approx" else oa.GUESS_ZERO
     if stop == "tolerance":
         p.stop_rule = oa.STOP_TOLERANCE
         p.max_cycles = MAX_CYCLES
@@ -71,11 +72,11 @@ def stream_problem(argv, precision="mixed", stop="tolerance"):
     return p
 
 
-def make_stream(n, seed, precision="mixed", stop="tolerance"):
+def make_stream(n, seed, precision="mixed", stop="tolerance", guess="zero"):
     """n request command lines (seeded) and their problems."""
     rng = np.random.default_rng(seed)
     argvs = [request_argv(rng) for _ in range(n)]
-    return argvs, [stream_problem(a, precision, stop) for a in argvs]
+    return argvs, [stream_problem(a, precision, stop, guess) for a in argvs]
 
 
 def serve(probs, arrivals, hbm_budget, graphs=True, max_active=0, log=None, keep=()):
@@ -209,11 +210,13 @@ def main():
     ap.add_argument("--stop", choices=("tolerance", "reference"), default="tolerance",
                     help="tolerance: V-cycles until tol (at most %d, or %d without a new best); "
                          "reference: the reference's stall rule" % (MAX_CYCLES, PATIENCE))
+    ap.add_argument("--guess", choices=("zero", "approx"), default="zero",
+                    help="initial u: the reference's 0, or the analytic puncture approximation")
     ap.add_argument("--hbm-gib", type=float, default=160.0, help="admission budget")
     ap.add_argument("--check", type=int, default=4, help="requests re-solved alone to check")
     args = ap.parse_args()
 
-    argvs, probs = make_stream(args.requests, args.seed, args.precision, args.stop)
+    argvs, probs = make_stream(args.requests, args.seed, args.precision, args.stop, args.guess)
     budget = int(args.hbm_gib * (1 << 30))
     capacity = None
     if args.rate == "auto":
@@ -230,7 +233,8 @@ def main():
     out, wall, extra = serve(probs, list(arrivals), budget, log=sys.stderr, keep=set(picks))
     res = summarize(out, wall, extra)
     res.update({"seed": args.seed, "rate_per_s": rate, "capacity_per_s": capacity,
-                "precision": args.precision, "hbm_budget_gib": args.hbm_gib, "L": L_DOMAIN,
+                "precision": args.precision, "guess": args.guess, "hbm_budget_gib": args.hbm_gib,
+                "L": L_DOMAIN,
                 "stop": args.stop, "max_cycles": MAX_CYCLES, "patience": PATIENCE,
                 "errors": sum(1 for r in out if "error" in r)})
     if picks:
