@@ -832,6 +832,22 @@ int apply_cycle(Hier &H, std::string &err) {
   return OLL_OK;
 }
 
+// device buffers of one host call, freed on every return path
+struct TempBuffers {
+  std::vector<void *> p;
+  ~TempBuffers() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <typename T>
+  hipError_t alloc(T **out, size_t count) {
+    void *q = nullptr;
+    const hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+    if (e == hipSuccess) p.push_back(q);
+    *out = static_cast<T *>(q);
+    return e;
+  }
+};
+
 int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::string &err) {
   const oll_problem &p = H.prob;
   const int top = H.top();
@@ -853,27 +869,23 @@ int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::str
   int *d_pbox = nullptr;
   double *d_pxyz = nullptr, *d_up = nullptr, *d_ctr = nullptr, *d_sph = nullptr;
   const size_t nsph = surface ? size_t(nb) * nt * nt : 0;
-  ENG_CHECK(hipMalloc(&d_pbox, pbox.size() * sizeof(int)));
-  ENG_CHECK(hipMalloc(&d_pxyz, pxyz.size() * sizeof(double)));
-  ENG_CHECK(hipMalloc(&d_up, pbox.size() * sizeof(double)));
+  TempBuffers tmp;
+  ENG_CHECK(tmp.alloc(&d_pbox, pbox.size()));
+  ENG_CHECK(tmp.alloc(&d_pxyz, pxyz.size()));
+  ENG_CHECK(tmp.alloc(&d_up, pbox.size()));
   ENG_CHECK(hipMemcpyAsync(d_pbox, pbox.data(), pbox.size() * sizeof(int), hipMemcpyHostToDevice, s));
   ENG_CHECK(hipMemcpyAsync(d_pxyz, pxyz.data(), pxyz.size() * sizeof(double), hipMemcpyHostToDevice, s));
   launch_interp_points(s, H.dbox, F_U, d_pbox, d_pxyz, p.num_bh, 4, d_up);
   std::vector<double> up(pbox.size()), sph(nsph);
   if (surface) {
-    ENG_CHECK(hipMalloc(&d_ctr, ctr.size() * sizeof(double)));
-    ENG_CHECK(hipMalloc(&d_sph, nsph * sizeof(double)));
+    ENG_CHECK(tmp.alloc(&d_ctr, ctr.size()));
+    ENG_CHECK(tmp.alloc(&d_sph, nsph));
     ENG_CHECK(hipMemcpyAsync(d_ctr, ctr.data(), ctr.size() * sizeof(double), hipMemcpyHostToDevice, s));
     launch_adm_sphere(s, H.dbox, nb, d_ctr, d_sph, nt);
     ENG_CHECK(hipMemcpyAsync(sph.data(), d_sph, nsph * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   ENG_CHECK(hipMemcpyAsync(up.data(), d_up, up.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   ENG_CHECK(hipStreamSynchronize(s));
-  (void)hipFree(d_pbox);
-  (void)hipFree(d_pxyz);
-  (void)hipFree(d_up);
-  (void)hipFree(d_ctr);
-  (void)hipFree(d_sph);
   // elliptic.cc:2448-2469
   for (int i = 0; i < p.num_bh && puncture; ++i) {
     double bhmj = 0;
@@ -934,18 +946,10 @@ int interp_points(Hier &H, hipStream_t s, int field, int npts, const int *box, c
   if (npts <= 0) return OLL_OK;
   int *d_box = nullptr;
   double *d_xyz = nullptr, *d_out = nullptr;
-  struct Release {  // frees whatever was allocated, on every return path
-    int *&b;
-    double *&x, *&o;
-    ~Release() {
-      (void)hipFree(b);
-      (void)hipFree(x);
-      (void)hipFree(o);
-    }
-  } rel{d_box, d_xyz, d_out};
-  ENG_CHECK(hipMalloc(&d_box, size_t(npts) * sizeof(int)));
-  ENG_CHECK(hipMalloc(&d_xyz, size_t(3 * npts) * sizeof(double)));
-  ENG_CHECK(hipMalloc(&d_out, size_t(npts) * sizeof(double)));
+  TempBuffers tmp;
+  ENG_CHECK(tmp.alloc(&d_box, size_t(npts)));
+  ENG_CHECK(tmp.alloc(&d_xyz, size_t(3 * npts)));
+  ENG_CHECK(tmp.alloc(&d_out, size_t(npts)));
   ENG_CHECK(hipMemcpyAsync(d_box, box, size_t(npts) * sizeof(int), hipMemcpyHostToDevice, s));
   ENG_CHECK(hipMemcpyAsync(d_xyz, xyz, size_t(3 * npts) * sizeof(double), hipMemcpyHostToDevice, s));
   launch_interp_points(s, H.dbox, field, d_box, d_xyz, npts, ord, d_out);

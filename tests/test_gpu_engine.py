@@ -123,3 +123,24 @@ def test_budget_admission():
         pass
     (tag, cycles, res, adm), = e.pop_done()
     assert tag == 3 and res <= 1e-10
+
+
+def test_kernel_launch_log_matches_totals():
+    """The per-launch smoother log holds one entry per timed launch and adds
+    up to the engine's totals; FP64 and FP32 colour passes are told apart."""
+    e = oa.Engine(graphs=False)
+    try:
+        for t, argv in enumerate(MIX[:2] + MIX[4:]):
+            assert e.admit(oa.problem_from_argv(argv), t)
+        e.step()
+        e.set_kernel_timing(True)
+        e.step()
+        n, ms, nbytes = e.kernel_timing()
+        b, lms, kind = e.kernel_launches()
+        assert n == len(b) == len(lms) == len(kind) > 0
+        assert abs(b.sum() - nbytes) <= 1e-9 * nbytes
+        assert abs(lms.sum() - ms) <= 1e-6 * ms + 1e-6
+        assert set(kind.tolist()) == {0, 1}
+        assert (b > 0).all() and (lms > 0).all()
+    finally:
+        e.close()
