@@ -5,8 +5,10 @@
 // entry is matched against every registry, a repeated flag keeps its last
 // value, unknown flags are ignored, nothing is parsed when argc <= 2, N is
 // snapped by Valid_N and dx = L/(N-1).  -BA/-Bn (the Robin value at infinity
-// and fall-off power) are registered and honoured; their defaults 0 and 1
-// are the values the reference hard-codes for the puncture equations.
+// and fall-off power) are flags of this solver; without -BA the Robin value
+// is the one the reference hard-codes per equation (boundary_coeff,
+// operator.cc:665, 744, 1093): 1 for Brill-waves and Punctures_EM, 0
+// otherwise.
 #pragma once
 
 #include <map>
@@ -15,7 +17,10 @@
 
 namespace olliptic {
 
-enum t_equation { PUNCTURES = 0, PUNCTURES_SCALAR = 1, OTHER_EQUATION = 99 };
+enum t_equation {
+  PUNCTURES = 0, PUNCTURES_SCALAR = 1, POISSON = 2, BRILL_WAVES = 3, PUNCTURES_EM = 4,
+  OTHER_EQUATION = 99
+};
 enum t_boundary { DIRICHLET, ROBIN, ASYMPTOTIC };
 enum t_method { GAUSS_SEIDEL, FMG, VCYCLE, WCYCLE, APP };
 enum t_output { YES, PARTIAL, NO };
@@ -74,7 +79,7 @@ class interface {
   std::map<std::string, bool> l_par;
   std::map<std::string, std::string> s_par;
   std::map<std::string, Keys> keys;
-  bool by_domain = false, by_points = false, by_grid_size = false;
+  bool by_domain = false, by_points = false, by_grid_size = false, robin_value_given = false;
   bool exec = true;
   std::string error;
   std::vector<double> bhmp, bhqp, bhx, bhy, bhz, bhpx, bhpy, bhpz, bhsx, bhsy, bhsz;

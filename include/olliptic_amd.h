@@ -34,9 +34,31 @@ extern "C" {
 #define OLL_ENOMEM (-5)     /* device allocation failed */
 #define OLL_EBUSY (-6)      /* engine: admitting would exceed the HBM budget */
 
-/* equations (interface.cc:1184-1323 names "Punctures", "Punctures_scalar_field") */
+/* equations (interface.cc:1184-1323 names "Punctures", "Punctures_scalar_field",
+ * "Poisson", "Brill-waves", "Punctures_EM"), all N(u) = Lap u + terms of u
+ * (and, for PUNCTURES_EM, of grad u) on the same kernels:
+ *   PUNCTURES         b (c+u)^-7                     (elliptic.h:405-421)
+ *   PUNCTURES_SCALAR  + d (c+u) + e (c+u)^5          (elliptic.h:429-445)
+ *   POISSON           u^2, rho = Gaussian source, u0 = Gaussian, Robin value 0
+ *                     (elliptic.h:357-370, operator.cc:640-662, 1865-1892)
+ *   BRILL             Lq u with Lq the Brill-wave seed, rho = 0, u0 = 1, Robin
+ *                     value 1 (elliptic.h:381-398, operator.cc:707-725, 1688-1698)
+ *   PUNCTURES_EM      [B |grad u|^2 + (B1 u + B2).grad u + C u^2 + D u + E]
+ *                     / (u^3 + A1 u^2 + A2 u + A3), coefficients from the mass
+ *                     and charge potentials of the punctures (oll_puncture.q),
+ *                     rho = 0, u0 = 1, Robin value 1 (elliptic.h:510-626,
+ *                     1458-1530, operator.cc:1010-1100, 3043-3310); order 2
+ *                     only, FP64 only (below)
+ * The Robin value of the shell is oll_problem.robin_a (set it to 1 for BRILL
+ * and PUNCTURES_EM, as the reference's boundary_coeff is).
+ * PUNCTURES_EM is refused at orders 4-8: there the reference's first
+ * derivatives are one-sided five-point stencils (four points to one side),
+ * which couple points of one colour of the (order/2 + 1)-colour sweeps. */
 #define OLL_EQ_PUNCTURES 0
 #define OLL_EQ_PUNCTURES_SCALAR 1
+#define OLL_EQ_POISSON 2
+#define OLL_EQ_BRILL 3
+#define OLL_EQ_PUNCTURES_EM 4
 
 /* arithmetic of the smoothing blocks.  FP64: the reference's Newton-Gauss-
  * Seidel sweeps in float64.  MIXED: each block of nu sweeps computes the
@@ -82,12 +104,22 @@ extern "C" {
 #define OLL_F_V 7
 #define OLL_F_W 8
 #define OLL_F_WW 9
+/* PUNCTURES_EM only: with eta = sum m / (2r) in OLL_F_B and phi = sum q / r in
+ * OLL_F_D, their gradients (operator.cc:3043-3310) */
+#define OLL_F_ETA_X 10
+#define OLL_F_ETA_Y 11
+#define OLL_F_ETA_Z 12
+#define OLL_F_PHI_X 13
+#define OLL_F_PHI_Y 14
+#define OLL_F_PHI_Z 15
 #define OLL_NFIELDS 10
 
-/* One puncture: bare mass, position, linear momentum, spin
- * (reference -bhmp<i> -bhx<i> ... -bhsz<i>, interface.cc:748-864). */
+/* One puncture: bare mass, position, linear momentum, spin, charge
+ * (reference -bhmp<i> -bhx<i> ... -bhsz<i> -bhqp<i>, interface.cc:748-864;
+ * the charge enters PUNCTURES_EM only). */
 typedef struct oll_puncture {
   double m, x, y, z, px, py, pz, sx, sy, sz;
+  double q;
 } oll_puncture;
 
 /* Problem description: the reference's hot-path CLI parameters

@@ -5,7 +5,8 @@ include/olliptic_amd.h); this package is its Python binding and serving
 front end.  Build with `python -m ollipticid_amd.build`.
 """
 from .native import (  # noqa: F401
-    EQ_PUNCTURES, EQ_PUNCTURES_SCALAR, F_B, F_C, F_D, F_E, F_R, F_RHO, F_U, F_V, F_W,
+    EQ_BRILL, EQ_POISSON, EQ_PUNCTURES, EQ_PUNCTURES_EM, EQ_PUNCTURES_SCALAR, F_B,
+    F_ETA_X, F_ETA_Y, F_ETA_Z, F_PHI_X, F_PHI_Y, F_PHI_Z, F_C, F_D, F_E, F_R, F_RHO, F_U, F_V, F_W,
     F_WW, GUESS_APPROX, GUESS_ZERO, METHOD_APP, METHOD_VCYCLE, PREC_FP64, PREC_MIXED,
     STOP_REFERENCE, STOP_TOLERANCE, Engine, Hierarchy, OllError,
     Problem, default_problem, load,

@@ -203,7 +203,7 @@ int oll_problem_footprint(const oll_problem *p, size_t *bytes) {
 
 int oll_get_field(oll_hier *h, int ibox, int field, double *host) {
   if (!h || !host) return fail(OLL_EINVAL, "null argument");
-  if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= NFIELDS)
+  if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= fields_of(h->H.prob.eq))
     return fail(OLL_EINVAL, "box or field out of range");
   const DevBox &b = h->H.hbox[size_t(ibox)];
   std::vector<double> st(size_t(b.size));
@@ -220,7 +220,7 @@ int oll_get_field(oll_hier *h, int ibox, int field, double *host) {
 
 int oll_set_field(oll_hier *h, int ibox, int field, const double *host) {
   if (!h || !host) return fail(OLL_EINVAL, "null argument");
-  if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= NFIELDS)
+  if (ibox < 0 || ibox >= int(h->H.hbox.size()) || field < 0 || field >= fields_of(h->H.prob.eq))
     return fail(OLL_EINVAL, "box or field out of range");
   const DevBox &b = h->H.hbox[size_t(ibox)];
   std::vector<double> st(size_t(b.size), 0.0);
@@ -410,7 +410,7 @@ int oll_interp(oll_hier *h, int field, int npts, const int *box, const double *x
                double *out) {
   if (!h) return fail(OLL_EINVAL, "null handle");
   if (npts < 0 || (npts > 0 && (!box || !xyz || !out))) return fail(OLL_EINVAL, "null argument");
-  if (field < 0 || field >= NFIELDS) return fail(OLL_EINVAL, "field out of range");
+  if (field < 0 || field >= fields_of(h->H.prob.eq)) return fail(OLL_EINVAL, "field out of range");
   if (ord < 1 || ord > 9) return fail(OLL_EINVAL, "interpolation order must be 1..9");
   const int nb = int(h->H.hbox.size());
   for (int q = 0; q < npts; ++q)

@@ -284,14 +284,20 @@ elliptic::~elliptic() {
 // elliptic.cc:31-221 Make; interface.cc:1497-1527 output directory
 void elliptic::Make(interface *Oll) {
   oll_problem_default(&prob_);
-  prob_.eq = Oll->Get_Equation() == PUNCTURES_SCALAR ? OLL_EQ_PUNCTURES_SCALAR : OLL_EQ_PUNCTURES;
+  switch (Oll->Get_Equation()) {
+    case PUNCTURES_SCALAR: prob_.eq = OLL_EQ_PUNCTURES_SCALAR; break;
+    case POISSON: prob_.eq = OLL_EQ_POISSON; break;
+    case BRILL_WAVES: prob_.eq = OLL_EQ_BRILL; break;
+    case PUNCTURES_EM: prob_.eq = OLL_EQ_PUNCTURES_EM; break;
+    default: prob_.eq = OLL_EQ_PUNCTURES; break;
+  }
   const std::vector<double> m = Oll->Get_bhmp(), x = Oll->Get_bhx(), y = Oll->Get_bhy(),
                             z = Oll->Get_bhz(), px = Oll->Get_bhpx(), py = Oll->Get_bhpy(),
                             pz = Oll->Get_bhpz(), sx = Oll->Get_bhsx(), sy = Oll->Get_bhsy(),
-                            sz = Oll->Get_bhsz();
+                            sz = Oll->Get_bhsz(), qp = Oll->Get_bhqp();
   prob_.num_bh = int(m.size());
   for (size_t i = 0; i < m.size() && i < size_t(OLL_MAX_BH); ++i)
-    prob_.bh[i] = oll_puncture{m[i], x[i], y[i], z[i], px[i], py[i], pz[i], sx[i], sy[i], sz[i]};
+    prob_.bh[i] = oll_puncture{m[i], x[i], y[i], z[i], px[i], py[i], pz[i], sx[i], sy[i], sz[i], qp[i]};
   prob_.L = Oll->Get_double_par("Lx");
   prob_.N = int(Oll->Get_integer_par("nx"));
   prob_.levels = int(Oll->Get_integer_par("levels"));
@@ -469,14 +475,20 @@ void elliptic::Solve() {
     oll_last_norms(h_, pre.data(), post.data(), top + 1);
     for (int k = top; k >= 0; --k) print_pre(k, top, pre[size_t(k)], cycle);
     for (int k = 1; k <= top; ++k) print_post(k, top, post[size_t(k)], cycle);
-    compute_adm(true);
+    // elliptic.cc:1939-1941: the ADM mass only for the puncture equations
+    if (prob_.eq == OLL_EQ_PUNCTURES || prob_.eq == OLL_EQ_PUNCTURES_SCALAR) compute_adm(true);
     ++cycle;
   } while (res > tol_ && res < res_prev);
   cycles_ = cycle;
   res_ = res;
   if (!quiet_) {
     std::cout << std::scientific << std::setprecision(8);
-    for (int j = 0; j < prob_.num_bh; ++j) std::cout << "mp [" << j << "] = " << prob_.bh[j].m << "\n";
+    // elliptic.cc:1333-1335: up to the first zero mass
+    for (int j = 0; j < prob_.num_bh && prob_.bh[j].m != 0.0; ++j)
+      std::cout << "mp [" << j << "] = " << prob_.bh[j].m << "\n";
+    // elliptic.cc:1337-1339: the charges up to the first zero one
+    for (int j = 0; j < prob_.num_bh && prob_.bh[j].q != 0.0; ++j)
+      std::cout << "qp [" << j << "] = " << prob_.bh[j].q << "\n";
   }
 }
 

@@ -78,7 +78,8 @@ interface::interface(int argc, char *argv[]) {
   add_double("ps_sigma", 8, "-sigma", "--sigma_sf", "Sigma of the scalar field perturbation");
   add_double("ps_phi0", 0.025, "-phi0", "--phi0_sf", "Amplitude of the scalar field perturbation");
   add_double("ps_a2", 1, "-a2", "--a2_sf", "Scalar field perturbation parameter a2");
-  add_double("robin_a", 0.0, "-BA", "--boundary-value", "Robin value at infinity");
+  add_double("robin_a", 0.0, "-BA", "--boundary-value",
+             "Robin value at infinity (default 1 for Brill-waves, Punctures_EM, else 0)");
   add_logic("io_bam", false, "-iob", "--io-bam", "Produce ID for bam");
   add_logic("io_Zcode", false, "-ioZ", "--io-Zcode", "Produce ID for Zcode");
   add_logic("print_coeff", false, "-pc", "--print-coeff", "Print coefficients");
@@ -104,7 +105,8 @@ interface::interface(int argc, char *argv[]) {
   add_string("file_bam", "none", "-iof", "--io-bam-file", "Bam initial data file");
   add_string("file_TOV", "none", "-tovf", "--tov-file", "TOV ID file");
   add_string("help", "h", "-h", "--help", "Displays this usage screen");
-  add_string("equation", "none", "-eq", "--equation", "Punctures or Punctures_scalar_field");
+  add_string("equation", "none", "-eq", "--equation",
+             "Punctures, Punctures_scalar_field, Poisson, Brill-waves or Punctures_EM");
   add_string("method", "Vcycle", "-m", "--method", "Vcycle");
   add_string("output", "yes", "-so", "--std-output", "Standard output: yes, partial or no");
   add_string("domain_sym", "full", "-ds", "--domain-sym", "full");
@@ -136,6 +138,7 @@ void interface::read_console(int argc, char *argv[]) {
           }
           const double v = std::strtod(argv[i + 1], nullptr);
           d_par[n] = v;
+          if (n == "robin_a") robin_value_given = true;
           if (n == "L" || n == "Lx" || n == "Ly" || n == "Lz") {
             if (n == "L") d_par["Lx"] = d_par["Ly"] = d_par["Lz"] = v;
             by_domain = true;
@@ -244,8 +247,20 @@ void interface::proc_parameters() {
     equation = PUNCTURES;
   else if (eq == "Punctures_scalar_field")
     equation = PUNCTURES_SCALAR;
+  else if (eq == "Poisson")
+    equation = POISSON;
+  else if (eq == "Brill-waves")
+    equation = BRILL_WAVES;
+  else if (eq == "Punctures_EM")
+    equation = PUNCTURES_EM;
   else
-    fail("equation '" + eq + "' is not solved here (Punctures, Punctures_scalar_field)");
+    fail("equation '" + eq +
+         "' is not solved here (Punctures, Punctures_scalar_field, Poisson, Brill-waves, "
+         "Punctures_EM)");
+  if ((equation == BRILL_WAVES || equation == PUNCTURES_EM) && !robin_value_given)
+    d_par["robin_a"] = 1.0;
+  if (equation == PUNCTURES_EM && i_par["order"] != 2)
+    fail("Punctures_EM runs at order 2 only (-O 2)");
   if (equation == PUNCTURES)
     for (double m : bhmp)
       if (m <= 0) fail("Wrong punture mass parameters");

@@ -53,8 +53,12 @@ int validate(const oll_problem &p, std::string &err) {
   if (p.N < 3) return err = "N too small", OLL_EINVAL;
   if (p.nu1 < 0 || p.nu2 < 0) return err = "negative sweep count", OLL_EINVAL;
   if (p.robin_n < 1) return err = "Robin fall-off power must be >= 1", OLL_EINVAL;
-  if (p.eq != OLL_EQ_PUNCTURES && p.eq != OLL_EQ_PUNCTURES_SCALAR)
-    return err = "unknown equation", OLL_EINVAL;
+  if (p.eq < OLL_EQ_PUNCTURES || p.eq > OLL_EQ_PUNCTURES_EM) return err = "unknown equation", OLL_EINVAL;
+  if (p.eq == OLL_EQ_PUNCTURES_EM && p.order != 2)
+    return err = "Punctures_EM runs at order 2 only: at orders 4-8 its one-sided first "
+                 "derivatives reach four points, past the multi-colour sweeps", OLL_EINVAL;
+  if (p.eq == OLL_EQ_PUNCTURES_EM && p.precision != OLL_PREC_FP64)
+    return err = "Punctures_EM runs in FP64 only", OLL_EINVAL;
   if (p.eq == OLL_EQ_PUNCTURES)
     for (int i = 0; i < p.num_bh; ++i)
       if (p.bh[i].m <= 0) return err = "Wrong punture mass parameters", OLL_EINVAL;
@@ -70,6 +74,8 @@ int validate(const oll_problem &p, std::string &err) {
     return err = "App puncture only works with puncture equation", OLL_EINVAL;
   if (p.initial_guess != OLL_GUESS_ZERO && p.initial_guess != OLL_GUESS_APPROX)
     return err = "unknown initial guess", OLL_EINVAL;
+  if (p.initial_guess == OLL_GUESS_APPROX && p.eq != OLL_EQ_PUNCTURES && p.eq != OLL_EQ_PUNCTURES_SCALAR)
+    return err = "the approximate initial guess is for the puncture equations", OLL_EINVAL;
   return OLL_OK;
 }
 
@@ -150,7 +156,7 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err) {
       const size_t nsrc = size_t(g.level_count[size_t(l - 1)]);
       tables += nsrc * (B.n[0] + B.n[1] + B.n[2]) * (9 + 8 * size_t(p.order + 2)) * 2;
     }
-  bytes = pts * (sizeof(double) * NFIELDS + (p.precision == OLL_PREC_MIXED ? sizeof(float) * NFIELDS32 : 0)) +
+  bytes = pts * (sizeof(double) * size_t(fields_of(p.eq)) + (p.precision == OLL_PREC_MIXED ? sizeof(float) * NFIELDS32 : 0)) +
           tables + (64 << 10);
   return OLL_OK;
 }
@@ -178,7 +184,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     off.push_back(total);
     total += (layout_size(n, b.R) + 31) & ~31LL;  // 256-byte aligned boxes
   }
-  const size_t slab_bytes = size_t(total) * sizeof(double) * NFIELDS;
+  const int nf = fields_of(p.eq);
+  const size_t slab_bytes = size_t(total) * sizeof(double) * size_t(nf);
   hipError_t e = hipMalloc(&H.slab, slab_bytes);
   if (e != hipSuccess) {
     err = std::string("hipMalloc slab: ") + hipGetErrorString(e);
@@ -215,7 +222,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.dulap = dulap_of(b.order, d.idx2);
     d.ipn = ipn;
     set_layout(d);
-    for (int f = 0; f < NFIELDS; ++f) d.f[f] = H.slab + size_t(f) * size_t(total) + size_t(off[ib]);
+    for (int f = 0; f < NFIELDS; ++f)
+      d.f[f] = f < nf ? H.slab + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     for (int f = 0; f < NFIELDS32; ++f)
       d.g[f] = H.slab32 ? H.slab32 + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     H.hbox.push_back(d);
@@ -295,7 +303,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   pa.eq = p.eq;
   for (int i = 0; i < p.num_bh; ++i) {
     const oll_puncture &q = p.bh[i];
-    pa.p[i] = Puncture{q.m, q.x, q.y, q.z, q.px, q.py, q.pz, q.sx, q.sy, q.sz};
+    pa.p[i] = Puncture{q.m, q.x, q.y, q.z, q.px, q.py, q.pz, q.sx, q.sy, q.sz, q.q};
   }
   pa.ps_case = p.ps_case;
   pa.ps_r0 = p.ps_r0;
@@ -441,7 +449,8 @@ struct Builder {
         if (mixed) row_blocks(H, l, slot(H, l), true, ps, sb);
         // algorithmic bytes per updated point: FP64 reads u, b, c, rho
         // (+ d, e) and writes u; FP32 reads e, q, r and writes e
-        const double per = mixed ? 16.0 : (eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : 40.0);
+        const double per = mixed ? 16.0
+                           : (eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : (eq == OLL_EQ_PUNCTURES_EM ? 88.0 : 40.0));
         for (int q = 0; q < nc; ++q) bytes[q] += per * H.colour_pts[size_t(l)][size_t(q)];
       }
       const size_t coff = S.add(cb);

@@ -80,10 +80,20 @@ __device__ __forceinline__ T lap(const T *__restrict__ u, int p, const Nbr<ORD /
                           star<ORD, T>(u0, szv[0], szv[1], szv[2], szv[3]), ix2, iy2, iz2);
 }
 
-// N(u) = Lap u + b (c+u)^-7 [+ d (c+u) + e (c+u)^5] and dN/du_p
+// N(u) and dN/du_p: EQ 0 Lap u + b (c+u)^-7, EQ 1 + d (c+u) + e (c+u)^5,
+// EQ 2 Lap u + u^2, EQ 3 Lap u + b u (c = 0 for EQ 2, 3, so cu = u)
 template <int EQ>
 __device__ __forceinline__ void nonlinear(double lapu, double dulap, double b, double cu,
                                           double dd, double ee, double &N, double &dN) {
+  if constexpr (EQ == 2) {
+    N = lapu + cu * cu;
+    dN = dulap + 2.0 * cu;
+    return;
+  } else if constexpr (EQ == 3) {
+    N = lapu + b * cu;
+    dN = dulap + b;
+    return;
+  }
   const double inv = 1.0 / cu;
   const double inv2 = inv * inv;
   const double inv4 = inv2 * inv2;
@@ -98,6 +108,42 @@ __device__ __forceinline__ void nonlinear(double lapu, double dulap, double b, d
     dN = fma(5.0 * ee, c4, dN + dd);
   }
 }
+
+// Punctures_EM (elliptic.h:510-626) at an order-2 interior point: u, its
+// centred gradient g and, from the mass and charge potentials eta, phi and
+// their gradients ge, gf, the thirteen coefficients (elliptic.h:1458-1530)
+//   N  = Lap u + num / den,
+//   num = B |g|^2 + (B1 u + B2).g + C u^2 + D u + E,
+//   den = u^3 + A1 u^2 + A2 u + A3,
+// and dN/du_p (the centred derivatives do not depend on u_p)
+__device__ __forceinline__ void em_nonlinear(double lapu, double dulap, double u, const double g[3],
+                                             double eta, double phi, const double ge[3],
+                                             const double gf[3], double &N, double &dN) {
+  const double f2 = gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2];
+  const double fe = gf[0] * ge[0] + gf[1] * ge[1] + gf[2] * ge[2];
+  const double e2 = ge[0] * ge[0] + ge[1] * ge[1] + ge[2] * ge[2];
+  const double a1 = 3 * eta;
+  const double a2 = 3 * (eta * eta) - 0.25 * (phi * phi);
+  const double a3 = eta * eta * eta - 0.25 * eta * phi * phi;
+  const double bb = -0.25 * (phi * phi);
+  const double cc = -0.25 * f2;
+  const double dd = 0.5 * phi * fe - 0.5 * eta * f2;
+  const double ee = -0.25 * phi * phi * e2 + 0.5 * phi * eta * fe + 0.25 * (1 - eta * eta) * f2;
+  double b1[3], b2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b1[a] = 0.5 * phi * gf[a];
+    b2[a] = 0.5 * phi * (eta * gf[a] - phi * ge[a]);
+  }
+  const double u2 = u * u;
+  const double num = bb * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) + (b1[0] * u + b2[0]) * g[0] +
+                     (b1[1] * u + b2[1]) * g[1] + (b1[2] * u + b2[2]) * g[2] + cc * u2 + dd * u + ee;
+  const double den = u2 * u + a1 * u2 + a2 * u + a3;
+  const double dnum = b1[0] * g[0] + b1[1] * g[1] + b1[2] * g[2] + 2 * cc * u + dd;
+  N = lapu + num / den;
+  dN = dulap + (dnum * den - num * (3 * u2 + 2 * a1 * u + a2)) / (den * den);
+}
+
 
 __device__ __forceinline__ double warp_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -210,6 +256,27 @@ __device__ __forceinline__ T lapb(const void *u, unsigned o, const Nbr<ORD / 2 +
   return star_sum<ORD, T>(star<ORD, T>(u0, sx[0], sx[1], sx[2], sx[3]),
                           star<ORD, T>(u0, syv[0], syv[1], syv[2], syv[3]),
                           star<ORD, T>(u0, szv[0], szv[1], szv[2], szv[3]), ix2, iy2, iz2);
+}
+
+// N(u) and dN/du at byte offset o of box B (u0 = u there): Lap u from the
+// neighbour offsets nb, the pointwise terms from the coefficient fields
+template <int ORD, int EQ, class BoxT>
+__device__ __forceinline__ void point_op(BoxT &B, const void *u, unsigned o, const Nbr<ORD / 2 + 1> &nb,
+                                         double u0, double &N, double &dN) {
+  const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
+  if constexpr (EQ == kEqPuncturesEm) {
+    static_assert(ORD == 2, "Punctures_EM runs at order 2");
+    // dx2 (diff_2nd.cc:238-300) at interior points: (u+ - u-) 0.5 / dx
+    const double g[3] = {(ld<double>(u, o + nb.xp[0]) - ld<double>(u, o + nb.xm[0])) * 0.5 * B.idx[0],
+                         (ld<double>(u, o + nb.yp[0]) - ld<double>(u, o + nb.ym[0])) * 0.5 * B.idx[1],
+                         (ld<double>(u, o + nb.zp[0]) - ld<double>(u, o + nb.zm[0])) * 0.5 * B.idx[2]};
+    const double ge[3] = {ld<double>(B.f[F_X0], o), ld<double>(B.f[F_X1], o), ld<double>(B.f[F_X2], o)};
+    const double gf[3] = {ld<double>(B.f[F_X3], o), ld<double>(B.f[F_X4], o), ld<double>(B.f[F_X5], o)};
+    em_nonlinear(lu, B.dulap, u0, g, ld<double>(B.f[F_B], o), ld<double>(B.f[F_D], o), ge, gf, N, dN);
+  } else {
+    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + u0,
+                  EQ == 1 ? ld<double>(B.f[F_D], o) : 0.0, EQ == 1 ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+  }
 }
 
 // one DPP step on a double: lanes of rows in ROW_MASK read lane CTRL's value
@@ -331,11 +398,9 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restri
     const Nbr<NC> nb = box_offs<NC, 8>(B, colour, s);
     const unsigned o = p * 8u;
     double *u = B.f[F_U];
-    const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
     const double u0 = ld<double>(u, o);
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + u0,
-                  EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+    point_op<ORD, EQ>(B, u, o, nb, u0, N, dN);
     const double res = N - ld<double>(B.f[F_RHO], o);
     const double un = u0 - res / dN;
     st<double>(u, o, un);
@@ -361,10 +426,8 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u;
     const double *u = B.f[F_U];
-    const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + ld<double>(u, o),
-                  EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+    point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), N, dN);
     if (write_defect)
       st<double>(B.f[F_W], o, ld<double>(B.f[F_RHO], o) - N);
     else
@@ -397,10 +460,8 @@ __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__rest
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u, o4 = p * 4u;
     const double *u = B.f[F_U];
-    const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
     double N, dN;
-    nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + ld<double>(u, o),
-                  EQ ? ld<double>(B.f[F_D], o) : 0.0, EQ ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
+    point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), N, dN);
     const double res = ld<double>(B.f[F_RHO], o) - N;
     st<float>(B.g[G_R], o4, float(res));
     st<float>(B.g[G_Q], o4, float(dN - B.dulap));
@@ -937,9 +998,7 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
     constexpr int NC = ORD / 2 + 1;
     const Nbr<NC> nb = box_offs<NC, 8>(T, int(colourc(T, i, j, k)), i % NC);
     double dN;
-    nonlinear<EQ>(lapb<ORD, double>(T.f[F_U], o, nb, T.idx2[0], T.idx2[1], T.idx2[2]),
-                  T.dulap, ld<double>(T.f[F_B], o), ld<double>(T.f[F_C], o) + uc,
-                  EQ ? ld<double>(T.f[F_D], o) : 0.0, EQ ? ld<double>(T.f[F_E], o) : 0.0, Rc, dN);
+    point_op<ORD, EQ>(T, T.f[F_U], o, nb, uc, Rc, dN);
   } else {
     Rc = robin_op_at(cbox_plain(T), i, j, k);
   }
@@ -1162,7 +1221,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
       const unsigned o = unsigned(p & 0x3fffffff) * 8u;
       own_b[q] = ld<double>(B.f[F_B], o);
       own_cc[q] = ld<double>(B.f[F_C], o);
-      if (EQ) {
+      if (EQ == 1 || EQ == kEqPuncturesEm) {
         own_d[q] = ld<double>(B.f[F_D], o);
         own_e[q] = ld<double>(B.f[F_E], o);
       }
@@ -1231,7 +1290,17 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
                                              unsigned(G.ps));
         const double lu = lap<2>(su, p, nb, G.idx2[0], G.idx2[1], G.idx2[2]);
         double N, dN;
-        nonlinear<EQ>(lu, G.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
+        if constexpr (EQ == kEqPuncturesEm) {
+          const unsigned o = unsigned(p) * 8u;
+          const double g[3] = {(su[p + int(nb.xp[0])] - su[p + int(nb.xm[0])]) * 0.5 * G.idx[0],
+                               (su[p + int(nb.yp[0])] - su[p + int(nb.ym[0])]) * 0.5 * G.idx[1],
+                               (su[p + int(nb.zp[0])] - su[p + int(nb.zm[0])]) * 0.5 * G.idx[2]};
+          const double ge[3] = {ld<double>(B.f[F_X0], o), ld<double>(B.f[F_X1], o), ld<double>(B.f[F_X2], o)};
+          const double gf[3] = {ld<double>(B.f[F_X3], o), ld<double>(B.f[F_X4], o), ld<double>(B.f[F_X5], o)};
+          em_nonlinear(lu, G.dulap, su[p], g, own_b[q], own_d[q], ge, gf, N, dN);
+        } else {
+          nonlinear<EQ>(lu, G.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
+        }
         const double res = N - srho[p];
         const double un = su[p] - omega * (res / dN);
         su[p] = un;
@@ -1362,6 +1431,62 @@ __global__ __launch_bounds__(256) void k_coefficients(DevBox B, PunctureArgs pa,
   point_ijk(B, q, i, j, k);
   const long long p = pidx(B, i, j, k);
   const double x = coord(B, 0, i), y = coord(B, 1, j), z = coord(B, 2, k);
+  const bool inner = interior(B, i, j, k);
+  if (pa.eq == 2) {
+    // operator.cc:640-662: rho = Gaussian_source, u = Gaussian (elliptic.h)
+    const double r2 = x * x + y * y + z * z, a = 0.004;
+    B.f[F_B][p] = 0.0;
+    B.f[F_C][p] = 0.0;
+    B.f[F_RHO][p] = inner ? a * exp(-r2) * (a + exp(0.5 * r2) * (r2 - 3.0)) : robin_a;
+    B.f[F_U][p] = a * exp(-0.5 * r2);
+    return;
+  }
+  if (pa.eq == kEqPuncturesEm) {
+    // operator.cc:1010-1100 and 3043-3310: eta = sum m / (2r) and phi =
+    // sum q / r with their gradients (r and r^3 clamped at 2 eps); rho = 0,
+    // u = 1
+    double e[4] = {0, 0, 0, 0}, g[4] = {0, 0, 0, 0};
+    for (int b = 0; b < pa.n; ++b) {
+      const Puncture &P = pa.p[b];
+      const double X = x - P.x, Y = y - P.y, Z = z - P.z;
+      double r = sqrt(X * X + Y * Y + Z * Z);
+      if (r < 2 * kEps) r = 2 * kEps;
+      double r3 = pow(X * X + Y * Y + Z * Z, 1.5);
+      if (r3 < 2 * kEps) r3 = 2 * kEps;
+      e[0] += P.m / r;
+      e[1] -= P.m * X / r3;
+      e[2] -= P.m * Y / r3;
+      e[3] -= P.m * Z / r3;
+      g[0] += P.q / r;
+      g[1] -= P.q * X / r3;
+      g[2] -= P.q * Y / r3;
+      g[3] -= P.q * Z / r3;
+    }
+    B.f[F_B][p] = 0.5 * e[0];
+    B.f[F_C][p] = 0.0;
+    B.f[F_D][p] = g[0];
+    B.f[F_E][p] = 0.0;
+    B.f[F_X0][p] = 0.5 * e[1];
+    B.f[F_X1][p] = 0.5 * e[2];
+    B.f[F_X2][p] = 0.5 * e[3];
+    B.f[F_X3][p] = g[1];
+    B.f[F_X4][p] = g[2];
+    B.f[F_X5][p] = g[3];
+    B.f[F_RHO][p] = inner ? 0.0 : robin_a;
+    B.f[F_U][p] = 1.0;
+    return;
+  }
+  if (pa.eq == 3) {
+    // operator.cc:707-725 and brill (operator.cc:1688-1698): b = Lq, rho = 0, u = 1
+    const double amp = 1.9, z2 = z * z, rho2 = x * x + y * y;
+    const double qrr = 2.0 * amp * exp(-rho2 - z2) * (1.0 - 5.0 * rho2 + 2.0 * rho2 * rho2);
+    const double qzz = 2.0 * amp * exp(-rho2 - z2) * rho2 * (2.0 * z2 - 1.0);
+    B.f[F_B][p] = 0.25 * (qrr + qzz);
+    B.f[F_C][p] = 0.0;
+    B.f[F_RHO][p] = inner ? 0.0 : robin_a;
+    B.f[F_U][p] = 1.0;
+    return;
+  }
   B.f[F_B][p] = punctures_b(pa, x, y, z);
   B.f[F_C][p] = punctures_c(pa, x, y, z);
   if (pa.eq == 1) {
@@ -1369,7 +1494,7 @@ __global__ __launch_bounds__(256) void k_coefficients(DevBox B, PunctureArgs pa,
     B.f[F_D][p] = sf_d(pa, r);
     B.f[F_E][p] = sf_e(pa, r);
   }
-  B.f[F_RHO][p] = interior(B, i, j, k) ? 0.0 : robin_a;
+  B.f[F_RHO][p] = inner ? 0.0 : robin_a;
   B.f[F_U][p] = 0.0;
 }
 
@@ -1577,22 +1702,22 @@ constexpr int kLdsLimit = 160 * 1024;
 // launchers
 // ------------------------------------------------------------------
 
+#define OLL_DISPATCH_ORDER_EQ(ORDER, E, KERNEL, ...)                  \
+  switch (ORDER) {                                                    \
+    case 2: KERNEL<2, E>(__VA_ARGS__); break;                         \
+    case 4: KERNEL<4, E>(__VA_ARGS__); break;                         \
+    case 6: KERNEL<6, E>(__VA_ARGS__); break;                         \
+    default: KERNEL<8, E>(__VA_ARGS__); break;                        \
+  }
 #define OLL_DISPATCH_ORDER(ORDER, EQ, KERNEL, ...)                    \
   do {                                                                \
-    if (EQ) {                                                         \
-      switch (ORDER) {                                                \
-        case 2: KERNEL<2, 1>(__VA_ARGS__); break;                     \
-        case 4: KERNEL<4, 1>(__VA_ARGS__); break;                     \
-        case 6: KERNEL<6, 1>(__VA_ARGS__); break;                     \
-        default: KERNEL<8, 1>(__VA_ARGS__); break;                    \
-      }                                                               \
-    } else {                                                          \
-      switch (ORDER) {                                                \
-        case 2: KERNEL<2, 0>(__VA_ARGS__); break;                     \
-        case 4: KERNEL<4, 0>(__VA_ARGS__); break;                     \
-        case 6: KERNEL<6, 0>(__VA_ARGS__); break;                     \
-        default: KERNEL<8, 0>(__VA_ARGS__); break;                    \
-      }                                                               \
+    switch (EQ) {                                                     \
+      case 1: OLL_DISPATCH_ORDER_EQ(ORDER, 1, KERNEL, __VA_ARGS__) break; \
+      case 2: OLL_DISPATCH_ORDER_EQ(ORDER, 2, KERNEL, __VA_ARGS__) break; \
+      case 3: OLL_DISPATCH_ORDER_EQ(ORDER, 3, KERNEL, __VA_ARGS__) break; \
+      /* Punctures_EM: order 2 only (the engine refuses other orders) */  \
+      case kEqPuncturesEm: KERNEL<2, kEqPuncturesEm>(__VA_ARGS__); break; \
+      default: OLL_DISPATCH_ORDER_EQ(ORDER, 0, KERNEL, __VA_ARGS__) break; \
     }                                                                 \
   } while (0)
 
@@ -1669,17 +1794,19 @@ bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_
   if (lds + 2048 > size_t(kLdsLimit) || max_int > kCoarseThreads * 8) return false;
   if (n == 0) return true;
   const bool small = max_int <= kCoarseThreads * 4;
-  if (eq) {
-    if (small)
-      hipLaunchKernelGGL((k_coarse_solve<1, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
-    else
-      hipLaunchKernelGGL((k_coarse_solve<1, 8>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
-  } else {
-    if (small)
-      hipLaunchKernelGGL((k_coarse_solve<0, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
-    else
-      hipLaunchKernelGGL((k_coarse_solve<0, 8>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+#define OLL_COARSE(E)                                                                              \
+  if (small)                                                                                       \
+    hipLaunchKernelGGL((k_coarse_solve<E, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter); \
+  else                                                                                             \
+    hipLaunchKernelGGL((k_coarse_solve<E, 8>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+  switch (eq) {
+    case 1: OLL_COARSE(1) break;
+    case 2: OLL_COARSE(2) break;
+    case 3: OLL_COARSE(3) break;
+    case kEqPuncturesEm: OLL_COARSE(kEqPuncturesEm) break;
+    default: OLL_COARSE(0) break;
   }
+#undef OLL_COARSE
   return true;
 }
 
@@ -1694,7 +1821,13 @@ void kernels_init() {
   const void *fns[] = {reinterpret_cast<const void *>(&k_coarse_solve<0, 4>),
                        reinterpret_cast<const void *>(&k_coarse_solve<0, 8>),
                        reinterpret_cast<const void *>(&k_coarse_solve<1, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<1, 8>)};
+                       reinterpret_cast<const void *>(&k_coarse_solve<1, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<2, 4>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<2, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<3, 4>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<3, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 4>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 8>)};
   for (const void *f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
   done = true;

@@ -21,7 +21,13 @@
 
 namespace oll {
 
-enum Field { F_U = 0, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW, NFIELDS };
+// F_X0 .. F_X5 exist for PUNCTURES_EM only: grad eta and grad phi, with
+// eta in F_B and phi in F_D (F_C, F_E stay 0)
+enum Field { F_U = 0, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW,
+             F_X0, F_X1, F_X2, F_X3, F_X4, F_X5, NFIELDS };
+constexpr int kBaseFields = F_X0;
+constexpr int kEqPuncturesEm = 4;
+inline int fields_of(int eq) { return eq == kEqPuncturesEm ? int(NFIELDS) : kBaseFields; }
 // float32 fields of the mixed-precision smoother (same element layout):
 // correction e, linearised diagonal q = dN/du - dulap, defect r = rho - N(u)
 enum Field32 { G_E = 0, G_Q, G_R, NFIELDS32 };
@@ -193,7 +199,7 @@ struct CoarseItem {
 };
 
 struct Puncture {
-  double m, x, y, z, px, py, pz, sx, sy, sz;
+  double m, x, y, z, px, py, pz, sx, sy, sz, q;
 };
 
 constexpr int kMaxBH = 16;

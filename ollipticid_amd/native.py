@@ -12,19 +12,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libolliptic_amd.so")
 
 MAX_BH = 16
-EQ_PUNCTURES, EQ_PUNCTURES_SCALAR = 0, 1
+EQ_PUNCTURES, EQ_PUNCTURES_SCALAR, EQ_POISSON, EQ_BRILL, EQ_PUNCTURES_EM = 0, 1, 2, 3, 4
+EQ_NAMES = {"Punctures": EQ_PUNCTURES, "Punctures_scalar_field": EQ_PUNCTURES_SCALAR,
+            "Poisson": EQ_POISSON, "Brill-waves": EQ_BRILL, "Punctures_EM": EQ_PUNCTURES_EM}
 PREC_FP64, PREC_MIXED = 0, 1
 STOP_REFERENCE, STOP_TOLERANCE = 0, 1
 METHOD_VCYCLE, METHOD_APP = 0, 1
 GUESS_ZERO, GUESS_APPROX = 0, 1
 F_U, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW = range(10)
+# Punctures_EM only (eta in F_B, phi in F_D): their gradients
+F_ETA_X, F_ETA_Y, F_ETA_Z, F_PHI_X, F_PHI_Y, F_PHI_Z = range(10, 16)
 NFIELDS = 10
 ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EDEVICE", -4: "ENAN", -5: "ENOMEM", -6: "EBUSY"}
 EBUSY = -6
 
 
 class Puncture(C.Structure):
-    _fields_ = [(k, C.c_double) for k in ("m", "x", "y", "z", "px", "py", "pz", "sx", "sy", "sz")]
+    _fields_ = [(k, C.c_double) for k in ("m", "x", "y", "z", "px", "py", "pz", "sx", "sy", "sz", "q")]
 
 
 class Problem(C.Structure):
@@ -157,13 +161,16 @@ def problem_from_argv(argv):
     """Maps a reference command line onto Problem with the reference's rules
     (interface.cc:366-874): flags match exactly, a repeated flag keeps its
     last value, unknown flags are ignored, nothing is parsed when the line has
-    fewer than two arguments.  -BA/-Bn are honoured (defaults 0 and 1, the
-    values the reference hard-codes for Punctures)."""
+    fewer than two arguments.  -BA/-Bn (Robin value at infinity and fall-off
+    power) are flags of this solver; without -BA the Robin value is the one
+    the reference hard-codes per equation (boundary_coeff, operator.cc:665,
+    744, 1093): 1 for Brill-waves and Punctures_EM, 0 otherwise."""
     p = default_problem()
     a = list(argv)
     if len(a) < 2:
         return p
     eq = None
+    robin_a = None
     nbh = p.num_bh
     bh = {}
     for i, tok in enumerate(a):
@@ -188,8 +195,8 @@ def problem_from_argv(argv):
             p.nu2 = int(float(v))
         elif tok in ("-s", "--tolerance"):
             p.tol = float(v)
-        elif tok == "-BA":
-            p.robin_a = float(v)
+        elif tok in ("-BA", "--boundary-value"):
+            robin_a = float(v)
         elif tok == "-Bn":
             p.robin_n = int(float(v))
         elif tok in ("-m", "--method"):
@@ -215,23 +222,24 @@ def problem_from_argv(argv):
         elif tok in ("-a2", "--a2_sf"):
             p.ps_a2 = float(v)
         elif tok.startswith("-bh"):
-            for key in ("mp", "px", "py", "pz", "sx", "sy", "sz", "x", "y", "z"):
+            for key in ("mp", "qp", "px", "py", "pz", "sx", "sy", "sz", "x", "y", "z"):
                 rest = tok[3 + len(key):]
                 if tok.startswith("-bh" + key) and rest.isdigit():
                     bh[(key, int(rest) - 1)] = float(v)
                     break
-    if eq == "Punctures_scalar_field":
-        p.eq = EQ_PUNCTURES_SCALAR
-    elif eq == "Punctures":
-        p.eq = EQ_PUNCTURES
-    elif eq is not None:
-        raise ValueError("equation %r is outside this solver (Punctures, "
-                         "Punctures_scalar_field)" % eq)
+    if eq is not None:
+        if eq not in EQ_NAMES:
+            raise ValueError("equation %r is outside this solver (%s)" % (eq, ", ".join(EQ_NAMES)))
+        p.eq = EQ_NAMES[eq]
+    if robin_a is not None:
+        p.robin_a = robin_a
+    elif p.eq in (EQ_BRILL, EQ_PUNCTURES_EM):
+        p.robin_a = 1.0
     if not 1 <= nbh <= MAX_BH:
         raise ValueError("NumBH must be in 1..%d" % MAX_BH)
     p.num_bh = nbh
     names = {"mp": "m", "x": "x", "y": "y", "z": "z", "px": "px", "py": "py", "pz": "pz",
-             "sx": "sx", "sy": "sy", "sz": "sz"}
+             "sx": "sx", "sy": "sy", "sz": "sz", "qp": "q"}
     for j in range(MAX_BH):
         for key, field in names.items():
             setattr(p.bh[j], field, bh.get((key, j), 0.0))

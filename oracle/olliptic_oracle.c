@@ -380,6 +380,50 @@ static double punctures_c(const oracle_solver *s, double X, double Y, double Z) 
   return cval;
 }
 
+/* operator.cc:3043-3310 EM_BH_phi / EM_BH_eta and their gradients: the
+ * charge and mass potentials phi = sum q / r, eta = sum m / (2 r); g[0] is
+ * the potential, g[1..3] its x, y, z derivatives (r and r^3 clamped at
+ * 2 eps) */
+static void em_potential(const oracle_solver *s, const double *w, double scale, double X,
+                         double Y, double Z, double g[4]) {
+  g[0] = g[1] = g[2] = g[3] = 0;
+  for (int i = 0; i < s->p.nbh; i++) {
+    const double x = X - s->p.x[i], y = Y - s->p.y[i], z = Z - s->p.z[i];
+    double r = sqrt(x * x + y * y + z * z);
+    if (r < 2 * EPS) r = 2 * EPS;
+    g[0] += w[i] / r;
+    double r3 = pow(x * x + y * y + z * z, 1.5);
+    if (r3 < 2 * EPS) r3 = 2 * EPS;
+    g[1] -= w[i] * x / r3;
+    g[2] -= w[i] * y / r3;
+    g[3] -= w[i] * z / r3;
+  }
+  for (int a = 0; a < 4; a++) g[a] *= scale;
+}
+
+/* elliptic.h:1458-1530 EM_BH_A1 .. EM_BH_E: the thirteen coefficients of
+ * Punctures_EM at one point, in oracle_field order from OF_A1 */
+static void em_coefficients(const oracle_solver *s, double x, double y, double z, double out[13]) {
+  double e[4], f[4];
+  em_potential(s, s->p.mp, 0.5, x, y, z, e);
+  em_potential(s, s->p.qp, 1.0, x, y, z, f);
+  const double eta = e[0], phi = f[0];
+  const double f2 = pow(f[1], 2) + pow(f[2], 2) + pow(f[3], 2);
+  const double fe = f[1] * e[1] + f[2] * e[2] + f[3] * e[3];
+  out[0] = 3 * eta;
+  out[1] = 3 * pow(eta, 2) - 0.25 * pow(phi, 2);
+  out[2] = pow(eta, 3) - 0.25 * eta * phi * phi;
+  out[3] = -0.25 * pow(phi, 2);
+  for (int a = 0; a < 3; a++) {
+    out[4 + a] = 0.5 * phi * f[1 + a];
+    out[7 + a] = 0.5 * phi * (eta * f[1 + a] - phi * e[1 + a]);
+  }
+  out[10] = -0.25 * f2;
+  out[11] = 0.5 * phi * fe - 0.5 * eta * f2;
+  out[12] = -0.25 * phi * phi * (e[1] * e[1] + e[2] * e[2] + e[3] * e[3]) + 0.5 * phi * eta * fe +
+            0.25 * (1 - eta * eta) * (f[1] * f[1] + f[2] * f[2] + f[3] * f[3]);
+}
+
 /* operator.cc:2228-2333 */
 static double punctures_b(const oracle_solver *s, double X, double Y, double Z) {
   double A[3][3] = {{0}};
@@ -571,9 +615,45 @@ static void op_at(const oracle_solver *s, const obox *B, size_t i, size_t j,
   const size_t p = IDX(B, i, j, k);
   const double bb = B->f[OF_B][p], cu = B->f[OF_C][p] + u[p];
   double lap = lap_at(B, u, i, j, k);
+  if (s->p.eq == ORACLE_PUNCTURES_EM) {
+    /* elliptic.h:510-626 puncture_EM_OP / puncture_EM_duOP:
+     *   N = Lap U + [B |grad U|^2 + (B1 U + B2) . grad U + C U^2 + D U + E]
+     *               / (U^3 + A1 U^2 + A2 U + A3)
+     * and its derivative in U at the point.  Dx, Dy, Dz and their
+     * coefficients at the point are deriv's (centred at order 2, one-sided
+     * five-point at orders 4-8: the reference's Dx as its solver runs it) */
+    const double uu = u[p], uu2 = uu * uu;
+    const size_t ijk[3] = {i, j, k};
+    double du[3];
+    const double udx = deriv(B, u, 0, ijk, &du[0]), udy = deriv(B, u, 1, ijk, &du[1]),
+                 udz = deriv(B, u, 2, ijk, &du[2]);
+    double c[13];
+    for (int q = 0; q < 13; q++) c[q] = B->f[OF_A1 + q][p];
+    const double num = c[3] * (udx * udx + udy * udy + udz * udz) + (c[4] * uu + c[7]) * udx +
+                       (c[5] * uu + c[8]) * udy + (c[6] * uu + c[9]) * udz + c[10] * uu2 +
+                       c[11] * uu + c[12];
+    const double den = uu2 * uu + c[0] * uu2 + c[1] * uu + c[2];
+    *N = lap + num / den;
+    if (dN) {
+      const double dnum = 2 * c[3] * (udx * du[0] + udy * du[1] + udz * du[2]) +
+                          (c[4] * uu + c[7]) * du[0] + c[4] * udx + (c[5] * uu + c[8]) * du[1] +
+                          c[5] * udy + (c[6] * uu + c[9]) * du[2] + c[6] * udz + 2 * c[10] * uu +
+                          c[11];
+      *dN = dulap(B) + (dnum * den - num * (3 * uu2 + 2 * c[0] * uu + c[1])) / pow(den, 2);
+    }
+    return;
+  }
   if (s->p.eq == ORACLE_PUNCTURES) {
     *N = lap + bb / pow(cu, 7);
     if (dN) *dN = dulap(B) - 7.0 * bb / pow(cu, 8);
+  } else if (s->p.eq == ORACLE_POISSON) {
+    /* elliptic.h:357-370 poisson_OP / poisson_duOP (c = 0: cu = u) */
+    *N = lap + cu * cu;
+    if (dN) *dN = dulap(B) + 2.0 * cu;
+  } else if (s->p.eq == ORACLE_BRILL) {
+    /* elliptic.h:381-398 brill_OP / brill_duOP, b = Lq (c = 0) */
+    *N = lap + bb * cu;
+    if (dN) *dN = dulap(B) + bb;
   } else {
     const double dd = B->f[OF_D][p], ee = B->f[OF_E][p];
     *N = lap + bb / pow(cu, 7) + dd * cu + ee * pow(cu, 5);
@@ -1270,7 +1350,8 @@ oracle_solver *oracle_create(const oracle_params *p) {
   for (int ib = 0; ib < s->nbox; ib++) {
     obox *B = &s->box[ib];
     B->npts = B->D.n[0] * B->D.n[1] * B->D.n[2];
-    for (int f = 0; f < OF_NFIELDS; f++) B->f[f] = calloc(B->npts, sizeof(double));
+    for (int f = 0; f < OF_NFIELDS; f++)
+      B->f[f] = (f < OF_NBASE || p->eq == ORACLE_PUNCTURES_EM) ? calloc(B->npts, sizeof(double)) : NULL;
     for (int a = 0; a < 3; a++) {
       B->idx[a] = 1.0 / B->D.d[a];
       B->idx2[a] = B->idx[a] * B->idx[a];
@@ -1280,6 +1361,37 @@ oracle_solver *oracle_create(const oracle_params *p) {
         for (size_t i = 0; i < B->D.n[0]; i++) {
           double x = dom_x(&B->D, 0, (long)i), y = dom_x(&B->D, 1, (long)j), z = dom_x(&B->D, 2, (long)k);
           size_t q = IDX(B, i, j, k);
+          const int shell = i < B->D.b[0] || i >= B->D.n[0] - B->D.b[0] || j < B->D.b[1] ||
+                            j >= B->D.n[1] - B->D.b[1] || k < B->D.b[2] || k >= B->D.n[2] - B->D.b[2];
+          if (p->eq == ORACLE_POISSON) {
+            /* operator.cc:640-662: rho0 = Gaussian_source, u = Gaussian,
+             * Robin value 0 on the shell (boundary.cc:8-84) */
+            const double r2 = x * x + y * y + z * z, a = 0.004;
+            B->f[OF_RHO][q] = shell ? 0.0 : a * exp(-r2) * (a + exp(0.5 * r2) * (r2 - 3.0));
+            B->f[OF_U][q] = a * exp(-0.5 * r2);
+            continue;
+          }
+          if (p->eq == ORACLE_PUNCTURES_EM) {
+            /* operator.cc:1010-1100: the thirteen coefficients, rho0 = 0,
+             * u = 1 (elliptic::one), Robin value 1 on the shell */
+            double cf[13];
+            em_coefficients(s, x, y, z, cf);
+            for (int c = 0; c < 13; c++) B->f[OF_A1 + c][q] = cf[c];
+            B->f[OF_RHO][q] = shell ? 1.0 : 0.0;
+            B->f[OF_U][q] = 1.0;
+            continue;
+          }
+          if (p->eq == ORACLE_BRILL) {
+            /* operator.cc:707-725 and brill (:1688-1698): Lq, rho0 = 0,
+             * u = 1, Robin value 1 on the shell */
+            const double Amp = 1.9, z2 = z * z, rho2 = x * x + y * y;
+            const double qrr = 2.0 * Amp * exp(-rho2 - z2) * (1.0 - 5.0 * rho2 + 2.0 * rho2 * rho2);
+            const double qzz = 2.0 * Amp * exp(-rho2 - z2) * rho2 * (2.0 * z2 - 1.0);
+            B->f[OF_B][q] = 0.25 * (qrr + qzz);
+            B->f[OF_RHO][q] = shell ? 1.0 : 0.0;
+            B->f[OF_U][q] = 1.0;
+            continue;
+          }
           B->f[OF_B][q] = punctures_b(s, x, y, z);
           B->f[OF_C][q] = punctures_c(s, x, y, z);
           if (p->eq == ORACLE_PUNCTURES_SCALAR) {

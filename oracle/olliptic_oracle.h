@@ -1,7 +1,8 @@
 /*
  * Oracle: plain-C restatement of the reference FAS multigrid solver for the
- * puncture Hamiltonian constraint (Punctures and Punctures_scalar_field
- * equations, Robin outer boundary, V-cycles).
+ * puncture Hamiltonian constraint (Punctures, Punctures_scalar_field,
+ * Punctures_EM, and the Poisson and Brill-waves test equations; Robin outer
+ * boundary, V-cycles).
  *
  * TEST INFRASTRUCTURE ONLY.  Only the test-suite, the smoke check and the
  * CPU-baseline leg of the benchmark may load this library, and only as the
@@ -27,11 +28,20 @@ extern "C" {
 
 #define ORACLE_MAX_BH 16
 
-enum oracle_eq { ORACLE_PUNCTURES = 0, ORACLE_PUNCTURES_SCALAR = 1 };
-enum oracle_order { ORACLE_LEX = 0, ORACLE_COLOUR = 1 };
-enum oracle_field {
-  OF_U = 0, OF_B, OF_C, OF_D, OF_E, OF_RHO, OF_R, OF_V, OF_W, OF_WW, OF_NFIELDS
+enum oracle_eq {
+  ORACLE_PUNCTURES = 0, ORACLE_PUNCTURES_SCALAR = 1, ORACLE_POISSON = 2, ORACLE_BRILL = 3,
+  ORACLE_PUNCTURES_EM = 4
 };
+enum oracle_order { ORACLE_LEX = 0, ORACLE_COLOUR = 1 };
+/* OF_A1 .. OF_EME: the thirteen Punctures_EM coefficients A1, A2, A3, B,
+ * B1x, B1y, B1z, B2x, B2y, B2z, C, D, E (operator.cc:191-221), allocated
+ * only for that equation (oracle_field returns NULL otherwise) */
+enum oracle_field {
+  OF_U = 0, OF_B, OF_C, OF_D, OF_E, OF_RHO, OF_R, OF_V, OF_W, OF_WW,
+  OF_A1, OF_A2, OF_A3, OF_EMB, OF_B1X, OF_B1Y, OF_B1Z, OF_B2X, OF_B2Y, OF_B2Z,
+  OF_EMC, OF_EMD, OF_EME, OF_NFIELDS
+};
+#define OF_NBASE OF_A1
 enum oracle_region { OR_ALL = 0, OR_INSIDE = 1, OR_BOUNDARY = 2 };
 enum oracle_interp { OI_LINEAR = 0, OI_LAGRANGE = 1 };
 
@@ -61,6 +71,7 @@ typedef struct oracle_params {
                            factor, <0 2 - 1.6/(n-1) (the GPU's choice) */
   int method_app;       /* 1: -m AppPuncture, whose grids have no multigrid
                            coarsening (elliptic.cc:510, 678) */
+  double qp[ORACLE_MAX_BH]; /* puncture charges -bhqp (Punctures_EM) */
 } oracle_params;
 
 typedef struct oracle_solver oracle_solver;

@@ -13,9 +13,13 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "Olliptic.x")
 
 MAX_BH = 16
-PUNCTURES, PUNCTURES_SCALAR = 0, 1
+PUNCTURES, PUNCTURES_SCALAR, POISSON, BRILL, PUNCTURES_EM = 0, 1, 2, 3, 4
+EQ_NAMES = {"Punctures": PUNCTURES, "Punctures_scalar_field": PUNCTURES_SCALAR,
+            "Poisson": POISSON, "Brill-waves": BRILL, "Punctures_EM": PUNCTURES_EM}
 LEX, COLOUR = 0, 1
 F_U, F_B, F_C, F_D, F_E, F_RHO, F_R, F_V, F_W, F_WW = range(10)
+# Punctures_EM coefficients A1, A2, A3, B, B1x, B1y, B1z, B2x, B2y, B2z, C, D, E
+F_EM = list(range(10, 23))
 R_ALL, R_INSIDE, R_BOUNDARY = 0, 1, 2
 I_LINEAR, I_LAGRANGE = 0, 1
 
@@ -35,7 +39,7 @@ class Params(C.Structure):
         ("ps_phi0", C.c_double), ("ps_a2", C.c_double),
         ("sweep_order", C.c_int), ("max_cycles", C.c_int),
         ("restrict_fw", C.c_int), ("coarse_omega", C.c_double),
-        ("method_app", C.c_int),
+        ("method_app", C.c_int), ("qp", _D16),
     ]
 
 
@@ -137,13 +141,13 @@ def params_from_argv(argv, sweep_order=LEX, max_cycles=0, restrict_fw=None):
         elif tok == "-a2":
             p.ps_a2 = float(v)
         elif tok.startswith("-bh"):
-            for key in ("mp", "px", "py", "pz", "sx", "sy", "sz", "x", "y", "z"):
+            for key in ("mp", "qp", "px", "py", "pz", "sx", "sy", "sz", "x", "y", "z"):
                 if tok.startswith("-bh" + key) and tok[3 + len(key):].isdigit():
                     bh[(key, int(tok[3 + len(key):]) - 1)] = float(v)
                     break
-    p.eq = PUNCTURES_SCALAR if eq == "Punctures_scalar_field" else PUNCTURES
+    p.eq = EQ_NAMES.get(eq, PUNCTURES)
     p.nbh = nbh
-    for key in ("mp", "x", "y", "z", "px", "py", "pz", "sx", "sy", "sz"):
+    for key in ("mp", "qp", "x", "y", "z", "px", "py", "pz", "sx", "sy", "sz"):
         arr = getattr(p, key)
         for j in range(MAX_BH):
             arr[j] = bh.get((key, j), 0.0)
@@ -203,6 +207,8 @@ class Solver:
         """numpy view (k, j, i) of one field of one box (shares memory)."""
         n = self.box_info(ib)["n"]
         ptr = self.L.oracle_field(self.h, ib, fid)
+        if not ptr:
+            raise ValueError("field %d is not held for this equation" % fid)
         return np.ctypeslib.as_array(ptr, shape=(n[2], n[1], n[0]))
 
     def smooth(self, ib, nu):

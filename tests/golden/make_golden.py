@@ -39,6 +39,10 @@ CFG2 = PUNC + ["-NumBH", "2", "-bhmp1", "0.487209", "-bhy1", "5.5",
                "-bhpx2", "0.0901099", "-bhpy2", "0.000703975",
                "-L", "1000", "-N", "65", "-l", "9", "-O", "4", "-nu2", "20"]
 
+EM2 = ["-eq", "Punctures_EM", "-B", "Robin", "-NumBH", "2", "-bhmp1", "0.5", "-bhqp1", "0.2",
+       "-bhx1", "3.13", "-bhy1", "0.17", "-bhz1", "0.07", "-bhmp2", "0.4", "-bhqp2", "-0.1",
+       "-bhx2", "-3.29", "-bhy2", "0.41", "-bhz2", "-0.11", "-L", "20", "-N", "25"]
+
 # name -> (argv, stride for the stored 3-D field)
 CASES = {
     "cfg1_tol": (CFG1 + ["-s", "1e-10"], 2),
@@ -81,6 +85,23 @@ CASES = {
         "-bhpx1", "0.05", "-bhpy2", "-0.04", "-bhsz3", "0.03",
         "-L", "64", "-N", "25", "-l", "3", "-O", "4", "-nu2", "8",
         "-s", "1e-15"], 1),
+    # other equations through the same kernels (operator.cc:640-725): the
+    # Poisson test Lap u + u^2 = Gaussian source (u0 = Gaussian, Robin 0) and
+    # Brill waves Lap u + Lq u = 0 (u0 = 1, Robin 1), iterated to stagnation
+    "small_poisson_o2": (["-eq", "Poisson", "-B", "Robin", "-NumBH", "1", "-L", "20", "-N", "25",
+                          "-l", "3", "-O", "2", "-nu2", "6", "-s", "1e-16"], 1),
+    "small_poisson_o6": (["-eq", "Poisson", "-B", "Robin", "-NumBH", "1", "-L", "20", "-N", "25",
+                          "-l", "1", "-O", "6", "-nu2", "6", "-s", "1e-16"], 1),
+    "small_brill_o4": (["-eq", "Brill-waves", "-B", "Robin", "-NumBH", "1", "-L", "20", "-N", "25",
+                        "-l", "2", "-O", "4", "-nu2", "6", "-s", "1e-16"], 1),
+    "small_brill_o8": (["-eq", "Brill-waves", "-B", "Robin", "-NumBH", "1", "-L", "20", "-N", "25",
+                        "-l", "1", "-O", "8", "-nu2", "6", "-s", "1e-16"], 1),
+    # Punctures_EM: two charged punctures off the grid nodes; 2nd order
+    # converges to round-off, 4th and 6th order stagnate where the
+    # reference's finest-level stopping rule ends them
+    "small_em_o2": (EM2 + ["-l", "1", "-O", "2", "-nu2", "6", "-s", "1e-16"], 1),
+    "small_em_o4": (EM2 + ["-l", "2", "-O", "4", "-nu2", "6", "-s", "1e-16"], 1),
+    "small_em_o6": (EM2 + ["-l", "1", "-O", "6", "-nu2", "6", "-s", "1e-16"], 1),
 }
 
 BOX_RE = re.compile(r"level: (\d+) \| box: (\d+)")

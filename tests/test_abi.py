@@ -68,3 +68,39 @@ def test_cli_parsing_rules():
     # fewer than two arguments: nothing parsed (interface.cc:373)
     q = oa.problem_from_argv(["-N"])
     assert q.N == 30
+
+
+def test_equation_names_and_robin_values():
+    """Equation names as the reference's -eq takes them; the Robin value is
+    the reference's boundary_coeff (operator.cc:665, 744, 791) unless -BA
+    sets it; the library accepts every equation and rejects others."""
+    base = ["-B", "Robin", "-NumBH", "1", "-bhmp1", "0.5", "-L", "20", "-N", "25", "-l", "1"]
+    want = {"Punctures": (oa.EQ_PUNCTURES, 0.0), "Punctures_scalar_field": (oa.EQ_PUNCTURES_SCALAR, 0.0),
+            "Poisson": (oa.EQ_POISSON, 0.0), "Brill-waves": (oa.EQ_BRILL, 1.0),
+            "Punctures_EM": (oa.EQ_PUNCTURES_EM, 1.0)}
+    for name, (eq, a) in want.items():
+        p = oa.problem_from_argv(["-eq", name] + base + ["-O", "2"])
+        assert (p.eq, p.robin_a) == (eq, a), name
+        assert oa.problem_footprint(p) > 0
+    assert oa.problem_from_argv(["-eq", "Brill-waves", "-BA", "0.5"] + base).robin_a == 0.5
+    with pytest.raises(ValueError):
+        oa.problem_from_argv(["-eq", "Bowen-York"] + base)
+    p = oa.problem_from_argv(["-eq", "Poisson"] + base)
+    p.eq = 4
+    with pytest.raises(oa.OllError):
+        oa.problem_footprint(p)
+    p.eq = oa.EQ_POISSON
+    p.initial_guess = oa.GUESS_APPROX
+    with pytest.raises(oa.OllError):
+        oa.problem_footprint(p)
+    # Punctures_EM: charges parsed; order 2 and FP64 only; six more fields
+    p = oa.problem_from_argv(["-eq", "Punctures_EM", "-bhqp1", "0.25", "-O", "2"] + base)
+    assert p.bh[0].q == 0.25
+    em_bytes = oa.problem_footprint(p)
+    p.eq = oa.EQ_BRILL
+    assert em_bytes > oa.problem_footprint(p)
+    p.eq = oa.EQ_PUNCTURES_EM
+    for order, prec in ((4, oa.PREC_FP64), (2, oa.PREC_MIXED)):
+        p.order, p.precision = order, prec
+        with pytest.raises(oa.OllError):
+            oa.problem_footprint(p)
