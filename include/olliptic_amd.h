@@ -47,13 +47,13 @@ extern "C" {
  *                     / (u^3 + A1 u^2 + A2 u + A3), coefficients from the mass
  *                     and charge potentials of the punctures (oll_puncture.q),
  *                     rho = 0, u0 = 1, Robin value 1 (elliptic.h:510-626,
- *                     1458-1530, operator.cc:1010-1100, 3043-3310); order 2
- *                     only, FP64 only (below)
+ *                     1458-1530, operator.cc:1010-1100, 3043-3310); FP64
+ *                     only.  Its first derivatives are those the reference
+ *                     evaluates: centred at order 2, one-sided five-point
+ *                     (four points to one side) at orders 4-8, where its
+ *                     boxes are swept in 5 colours instead of order/2 + 1
  * The Robin value of the shell is oll_problem.robin_a (set it to 1 for BRILL
- * and PUNCTURES_EM, as the reference's boundary_coeff is).
- * PUNCTURES_EM is refused at orders 4-8: there the reference's first
- * derivatives are one-sided five-point stencils (four points to one side),
- * which couple points of one colour of the (order/2 + 1)-colour sweeps. */
+ * and PUNCTURES_EM, as the reference's boundary_coeff is). */
 #define OLL_EQ_PUNCTURES 0
 #define OLL_EQ_PUNCTURES_SCALAR 1
 #define OLL_EQ_POISSON 2

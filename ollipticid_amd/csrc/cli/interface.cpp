@@ -259,8 +259,6 @@ void interface::proc_parameters() {
          "Punctures_EM)");
   if ((equation == BRILL_WAVES || equation == PUNCTURES_EM) && !robin_value_given)
     d_par["robin_a"] = 1.0;
-  if (equation == PUNCTURES_EM && i_par["order"] != 2)
-    fail("Punctures_EM runs at order 2 only (-O 2)");
   if (equation == PUNCTURES)
     for (double m : bhmp)
       if (m <= 0) fail("Wrong punture mass parameters");

@@ -54,9 +54,6 @@ int validate(const oll_problem &p, std::string &err) {
   if (p.nu1 < 0 || p.nu2 < 0) return err = "negative sweep count", OLL_EINVAL;
   if (p.robin_n < 1) return err = "Robin fall-off power must be >= 1", OLL_EINVAL;
   if (p.eq < OLL_EQ_PUNCTURES || p.eq > OLL_EQ_PUNCTURES_EM) return err = "unknown equation", OLL_EINVAL;
-  if (p.eq == OLL_EQ_PUNCTURES_EM && p.order != 2)
-    return err = "Punctures_EM runs at order 2 only: at orders 4-8 its one-sided first "
-                 "derivatives reach four points, past the multi-colour sweeps", OLL_EINVAL;
   if (p.eq == OLL_EQ_PUNCTURES_EM && p.precision != OLL_PREC_FP64)
     return err = "Punctures_EM runs in FP64 only", OLL_EINVAL;
   if (p.eq == OLL_EQ_PUNCTURES)
@@ -148,7 +145,7 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err) {
   size_t pts = 0, tables = 0;
   for (const Box &b : g.boxes) {
     const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
-    pts += size_t((layout_size(n, b.R) + 31) & ~31LL);
+    pts += size_t((layout_size(n, colours_of(int(b.order), p.eq)) + 31) & ~31LL);
   }
   for (int l = 1; l < g.levels; ++l)
     for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
@@ -175,14 +172,15 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
         b.n[2] >= size_t(kMaxAxisPoints))
       return err = "a box exceeds 4095 points per axis", OLL_EINVAL;
     const int n[3] = {int(b.n[0]), int(b.n[1]), int(b.n[2])};
-    if (layout_size(n, b.R) * (long long)sizeof(double) >= kMaxFieldBytes)
+    const int nc = colours_of(int(b.order), p.eq);
+    if (layout_size(n, nc) * (long long)sizeof(double) >= kMaxFieldBytes)
       return err = "a box field exceeds 4 GiB", OLL_EINVAL;
     // the ghost fill splits shell point indices with 24-bit float division
     const long long inner = (long long)(n[0] - 2 * b.R) * (n[1] - 2 * b.R) * (n[2] - 2 * b.R);
     if ((long long)n[0] * n[1] * n[2] - inner >= (1LL << 24))
       return err = "a box exceeds 2^24 ghost-shell points", OLL_EINVAL;
     off.push_back(total);
-    total += (layout_size(n, b.R) + 31) & ~31LL;  // 256-byte aligned boxes
+    total += (layout_size(n, nc) + 31) & ~31LL;  // 256-byte aligned boxes
   }
   const int nf = fields_of(p.eq);
   const size_t slab_bytes = size_t(total) * sizeof(double) * size_t(nf);
@@ -221,7 +219,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.level = b.level;
     d.dulap = dulap_of(b.order, d.idx2);
     d.ipn = ipn;
-    set_layout(d);
+    set_layout(d, colours_of(b.order, p.eq));
     for (int f = 0; f < NFIELDS; ++f)
       d.f[f] = f < nf ? H.slab + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     for (int f = 0; f < NFIELDS32; ++f)
@@ -230,7 +228,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   }
   for (int l = 0; l < g.levels; ++l) {
     const DevBox &B = H.hbox[size_t(g.level_first[size_t(l)])];
-    const int nc = B.order / 2 + 1;
+    const int nc = B.nc;
     std::vector<double> cnt(size_t(nc), 0.0);
     int rows = 0, maxrow = 0;
     for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
@@ -431,7 +429,7 @@ struct Builder {
     for (auto &g : groups) {
       const int order = std::get<0>(g.first), eq = std::get<1>(g.first);
       const bool mixed = std::get<2>(g.first) != 0;
-      const int nc = order / 2 + 1;
+      const int nc = colours_of(order, eq);
       auto slot = [&](const Hier &H, int l) {
         return H.slot(norm_slot >= 0 ? norm_slot : post ? Hier::slot_post(l) : Hier::slot_pre(l));
       };

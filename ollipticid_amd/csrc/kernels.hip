@@ -63,8 +63,8 @@ __device__ __forceinline__ T star_sum(T ax, T ay, T az, T ix2, T iy2, T iz2) {
 
 // axis-star Laplacian at element index p of an array (LDS); neighbour
 // element offsets from stencil_offs<NC, 1>
-template <int ORD, typename T = double>
-__device__ __forceinline__ T lap(const T *__restrict__ u, int p, const Nbr<ORD / 2 + 1> &n, T ix2,
+template <int ORD, typename T = double, int NC = ORD / 2 + 1>
+__device__ __forceinline__ T lap(const T *__restrict__ u, int p, const Nbr<NC> &n, T ix2,
                                  T iy2, T iz2) {
   constexpr int R = ORD / 2;
   T sx[4] = {0, 0, 0, 0}, syv[4] = {0, 0, 0, 0}, szv[4] = {0, 0, 0, 0};
@@ -109,16 +109,18 @@ __device__ __forceinline__ void nonlinear(double lapu, double dulap, double b, d
   }
 }
 
-// Punctures_EM (elliptic.h:510-626) at an order-2 interior point: u, its
-// centred gradient g and, from the mass and charge potentials eta, phi and
-// their gradients ge, gf, the thirteen coefficients (elliptic.h:1458-1530)
+// Punctures_EM (elliptic.h:510-626) at an interior point: u, its gradient g
+// (du: each derivative's coefficient of u_p) and, from the mass and charge
+// potentials eta, phi and their gradients ge, gf, the thirteen coefficients
+// (elliptic.h:1458-1530)
 //   N  = Lap u + num / den,
 //   num = B |g|^2 + (B1 u + B2).g + C u^2 + D u + E,
 //   den = u^3 + A1 u^2 + A2 u + A3,
-// and dN/du_p (the centred derivatives do not depend on u_p)
+// and dN/du_p
 __device__ __forceinline__ void em_nonlinear(double lapu, double dulap, double u, const double g[3],
-                                             double eta, double phi, const double ge[3],
-                                             const double gf[3], double &N, double &dN) {
+                                             const double du[3], double eta, double phi,
+                                             const double ge[3], const double gf[3], double &N,
+                                             double &dN) {
   const double f2 = gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2];
   const double fe = gf[0] * ge[0] + gf[1] * ge[1] + gf[2] * ge[2];
   const double e2 = ge[0] * ge[0] + ge[1] * ge[1] + ge[2] * ge[2];
@@ -139,7 +141,9 @@ __device__ __forceinline__ void em_nonlinear(double lapu, double dulap, double u
   const double num = bb * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) + (b1[0] * u + b2[0]) * g[0] +
                      (b1[1] * u + b2[1]) * g[1] + (b1[2] * u + b2[2]) * g[2] + cc * u2 + dd * u + ee;
   const double den = u2 * u + a1 * u2 + a2 * u + a3;
-  const double dnum = b1[0] * g[0] + b1[1] * g[1] + b1[2] * g[2] + 2 * cc * u + dd;
+  const double dnum = 2 * bb * (g[0] * du[0] + g[1] * du[1] + g[2] * du[2]) + (b1[0] * u + b2[0]) * du[0] +
+                      b1[0] * g[0] + (b1[1] * u + b2[1]) * du[1] + b1[1] * g[1] +
+                      (b1[2] * u + b2[2]) * du[2] + b1[2] * g[2] + 2 * cc * u + dd;
   N = lapu + num / den;
   dN = dulap + (dnum * den - num * (3 * u2 + 2 * a1 * u + a2)) / (den * den);
 }
@@ -241,8 +245,8 @@ __device__ __forceinline__ void st(void *base, unsigned off, T v) {
 
 // axis-star Laplacian at byte offset o of a field; neighbour byte offsets
 // from stencil_offs<NC, sizeof(T)>
-template <int ORD, typename T>
-__device__ __forceinline__ T lapb(const void *u, unsigned o, const Nbr<ORD / 2 + 1> &n, T ix2, T iy2,
+template <int ORD, typename T, int NC = ORD / 2 + 1>
+__device__ __forceinline__ T lapb(const void *u, unsigned o, const Nbr<NC> &n, T ix2, T iy2,
                                   T iz2) {
   constexpr int R = ORD / 2;
   T sx[4] = {0, 0, 0, 0}, syv[4] = {0, 0, 0, 0}, szv[4] = {0, 0, 0, 0};
@@ -258,22 +262,54 @@ __device__ __forceinline__ T lapb(const void *u, unsigned o, const Nbr<ORD / 2 +
                           star<ORD, T>(u0, szv[0], szv[1], szv[2], szv[3]), ix2, iy2, iz2);
 }
 
-// N(u) and dN/du at byte offset o of box B (u0 = u there): Lap u from the
-// neighbour offsets nb, the pointwise terms from the coefficient fields
+// Punctures_EM's first derivative along one axis at an interior point, as
+// the reference's solver evaluates it: dx2 (centred, diff_2nd.cc:238-300) at
+// order 2; at orders 4-8 the one-sided five-point stencil, forward on the
+// lower half of the axis (index <= n/2, rounded down) and backward above,
+// with du its coefficient of the point itself.  p, m: the +1..+4 and
+// -1..-4 neighbour byte offsets; ia, na: index and points along the axis
+template <int ORD>
+__device__ __forceinline__ double em_deriv(const void *u, unsigned o, double u0, const unsigned *pp,
+                                           const unsigned *mm, int ia, int na, double id, double &du) {
+  if constexpr (ORD == 2) {
+    du = 0.0;
+    return (ld<double>(u, o + pp[0]) - ld<double>(u, o + mm[0])) * 0.5 * id;
+  } else {
+    const int nm = na % 2 == 0 ? na / 2 : (na - 1) / 2;
+    if (ia <= nm) {
+      du = -25 * id / 12.0;
+      return -(25 * u0 - 48 * ld<double>(u, o + pp[0]) + 36 * ld<double>(u, o + pp[1]) -
+               16 * ld<double>(u, o + pp[2]) + 3 * ld<double>(u, o + pp[3])) *
+             id / 12.0;
+    }
+    du = 25 * id / 12.0;
+    return (25 * u0 - 48 * ld<double>(u, o + mm[0]) + 36 * ld<double>(u, o + mm[1]) -
+            16 * ld<double>(u, o + mm[2]) + 3 * ld<double>(u, o + mm[3])) *
+           id / 12.0;
+  }
+}
+
+// N(u) and dN/du at byte offset o of box B (u0 = u there, (i, j, k) its
+// indices): Lap u from the neighbour offsets nb, the other terms from the
+// coefficient fields
 template <int ORD, int EQ, class BoxT>
-__device__ __forceinline__ void point_op(BoxT &B, const void *u, unsigned o, const Nbr<ORD / 2 + 1> &nb,
-                                         double u0, double &N, double &dN) {
-  const double lu = lapb<ORD, double>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
+__device__ __forceinline__ void point_op(BoxT &B, const void *u, unsigned o,
+                                         const Nbr<colours_of(ORD, EQ)> &nb, double u0, int i, int j,
+                                         int k, double &N, double &dN) {
+  constexpr int NC = colours_of(ORD, EQ);
+  const double lu = lapb<ORD, double, NC>(u, o, nb, B.idx2[0], B.idx2[1], B.idx2[2]);
   if constexpr (EQ == kEqPuncturesEm) {
-    static_assert(ORD == 2, "Punctures_EM runs at order 2");
-    // dx2 (diff_2nd.cc:238-300) at interior points: (u+ - u-) 0.5 / dx
-    const double g[3] = {(ld<double>(u, o + nb.xp[0]) - ld<double>(u, o + nb.xm[0])) * 0.5 * B.idx[0],
-                         (ld<double>(u, o + nb.yp[0]) - ld<double>(u, o + nb.ym[0])) * 0.5 * B.idx[1],
-                         (ld<double>(u, o + nb.zp[0]) - ld<double>(u, o + nb.zm[0])) * 0.5 * B.idx[2]};
+    double g[3], du[3];
+    g[0] = em_deriv<ORD>(u, o, u0, nb.xp, nb.xm, i, B.n[0], B.idx[0], du[0]);
+    g[1] = em_deriv<ORD>(u, o, u0, nb.yp, nb.ym, j, B.n[1], B.idx[1], du[1]);
+    g[2] = em_deriv<ORD>(u, o, u0, nb.zp, nb.zm, k, B.n[2], B.idx[2], du[2]);
     const double ge[3] = {ld<double>(B.f[F_X0], o), ld<double>(B.f[F_X1], o), ld<double>(B.f[F_X2], o)};
     const double gf[3] = {ld<double>(B.f[F_X3], o), ld<double>(B.f[F_X4], o), ld<double>(B.f[F_X5], o)};
-    em_nonlinear(lu, B.dulap, u0, g, ld<double>(B.f[F_B], o), ld<double>(B.f[F_D], o), ge, gf, N, dN);
+    em_nonlinear(lu, B.dulap, u0, g, du, ld<double>(B.f[F_B], o), ld<double>(B.f[F_D], o), ge, gf, N, dN);
   } else {
+    (void)i;
+    (void)j;
+    (void)k;
     nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + u0,
                   EQ == 1 ? ld<double>(B.f[F_D], o) : 0.0, EQ == 1 ? ld<double>(B.f[F_E], o) : 0.0, N, dN);
   }
@@ -326,19 +362,19 @@ __device__ __forceinline__ void row_jk(ConstBox &B, unsigned r, int &j, int &k) 
   j = R + int(r - kq * nyi);
 }
 
-// colour passes: the thread's slot t of local row lr: element index p of
-// its point of colour `colour` and that point's i mod NC; false when the row
-// is past the block or the slot past the row's interior colour points
-template <int R>
+// colour passes (NC colours): the thread's slot t of local row lr: element
+// index p of its point (i, j, k) of colour `colour` and that point's i mod
+// NC; false when the row is past the block or the slot past the row's
+// interior colour points
+template <int R, int NC = R + 1>
 __device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, unsigned lr, int t,
-                                            int colour, unsigned &p, int &s) {
-  constexpr int NC = R + 1;
+                                            int colour, unsigned &p, int &s, int &i, int &j, int &k) {
   if (lr >= unsigned(Bk.rows)) return false;
-  int j, k;
   row_jk<R>(B, unsigned(Bk.row0) + lr, j, k);
   // i of the row's first interior point of this colour
   const int first = R + int(unsigned(colour + NC * 3 * kMaxAxisPoints - j - k - R) % NC);
-  if (first + t * NC >= B.n[0] - R) return false;
+  i = first + t * NC;
+  if (i >= B.n[0] - R) return false;
   s = first % NC;
   p = unsigned(colour) * unsigned(B.cs) + unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) +
       unsigned(first / NC + t);
@@ -348,20 +384,19 @@ __device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, uns
 // whole-row kernels: sub-row sr = NC * row + c is colour c of interior row
 // `row`; the thread owns slot t of it.  Element index p, colour c and
 // i mod NC; false for ghost and padding slots
-template <int R>
+template <int R, int NC = R + 1>
 __device__ __forceinline__ bool row_slot(const RowBlock &Bk, ConstBox &B, unsigned per_row,
-                                         float inv_per_row, unsigned &p, int &c, int &s) {
-  constexpr int NC = R + 1;
+                                         float inv_per_row, unsigned &p, int &c, int &s, int &i,
+                                         int &j, int &k) {
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   if (lr >= unsigned(Bk.rows)) return false;
   const unsigned sr = unsigned(Bk.row0) + lr;
   const unsigned row = sr / NC;
   c = int(sr - row * NC);
-  int j, k;
   row_jk<R>(B, row, j, k);
   s = int(unsigned(c + NC * 3 * kMaxAxisPoints - j - k) % NC);
-  const int i = s + t * NC;
+  i = s + t * NC;
   if (i < R || i >= B.n[0] - R) return false;
   p = unsigned(c) * unsigned(B.cs) + unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) +
       unsigned(t);
@@ -386,21 +421,21 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restri
                                                         unsigned per_row, float inv_per_row,
                                                         int colour) {
   constexpr int R = ORD / 2;
-  constexpr int NC = R + 1;
+  constexpr int NC = colours_of(ORD, EQ);
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   double rmax = 0.0;
   unsigned p;
-  int s;
-  if (colour_slot<R>(Bk, B, lr, t, colour, p, s)) {
+  int s, i, j, k;
+  if (colour_slot<R, NC>(Bk, B, lr, t, colour, p, s, i, j, k)) {
     const Nbr<NC> nb = box_offs<NC, 8>(B, colour, s);
     const unsigned o = p * 8u;
     double *u = B.f[F_U];
     const double u0 = ld<double>(u, o);
     double N, dN;
-    point_op<ORD, EQ>(B, u, o, nb, u0, N, dN);
+    point_op<ORD, EQ>(B, u, o, nb, u0, i, j, k, N, dN);
     const double res = N - ld<double>(B.f[F_RHO], o);
     const double un = u0 - res / dN;
     st<double>(u, o, un);
@@ -416,18 +451,18 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
                                                           unsigned per_row, float inv_per_row,
                                                           int write_defect) {
   constexpr int R = ORD / 2;
-  constexpr int NC = R + 1;
+  constexpr int NC = colours_of(ORD, EQ);
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
   unsigned p;
-  int c, s;
-  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s)) {
+  int c, s, i, j, k;
+  if (row_slot<R, NC>(Bk, B, per_row, inv_per_row, p, c, s, i, j, k)) {
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u;
     const double *u = B.f[F_U];
     double N, dN;
-    point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), N, dN);
+    point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), i, j, k, N, dN);
     if (write_defect)
       st<double>(B.f[F_W], o, ld<double>(B.f[F_RHO], o) - N);
     else
@@ -450,18 +485,18 @@ template <int ORD, int EQ>
 __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__restrict__ blocks,
                                                           unsigned per_row, float inv_per_row) {
   constexpr int R = ORD / 2;
-  constexpr int NC = R + 1;
+  constexpr int NC = colours_of(ORD, EQ);
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
   unsigned p;
-  int c, s;
-  if (row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s)) {
+  int c, s, i, j, k;
+  if (row_slot<R, NC>(Bk, B, per_row, inv_per_row, p, c, s, i, j, k)) {
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u, o4 = p * 4u;
     const double *u = B.f[F_U];
     double N, dN;
-    point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), N, dN);
+    point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), i, j, k, N, dN);
     const double res = ld<double>(B.f[F_RHO], o) - N;
     st<float>(B.g[G_R], o4, float(res));
     st<float>(B.g[G_Q], o4, float(dN - B.dulap));
@@ -483,8 +518,8 @@ __global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__rest
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   unsigned p;
-  int s;
-  if (!colour_slot<R>(Bk, B, lr, t, colour, p, s)) return;
+  int s, i, j, k;
+  if (!colour_slot<R>(Bk, B, lr, t, colour, p, s, i, j, k)) return;
   const Nbr<NC> nb = box_offs<NC, 4>(B, colour, s);
   const unsigned o = p * 4u;
   float *e = B.g[G_E];
@@ -503,8 +538,8 @@ __global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__rest
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   unsigned p;
-  int c, s;
-  if (!row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s)) return;
+  int c, s, i, j, k;
+  if (!row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s, i, j, k)) return;
   double *u = B.f[F_U];
   st<double>(u, p * 8u, ld<double>(u, p * 8u) + double(ld<float>(B.g[G_E], p * 4u)));
 }
@@ -995,10 +1030,10 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
   if (covered) st<double>(T.f[F_W], o, wc); else wc = ld<double>(T.f[F_W], o);
   double Rc;
   if (inside) {
-    constexpr int NC = ORD / 2 + 1;
+    constexpr int NC = colours_of(ORD, EQ);
     const Nbr<NC> nb = box_offs<NC, 8>(T, int(colourc(T, i, j, k)), i % NC);
     double dN;
-    point_op<ORD, EQ>(T, T.f[F_U], o, nb, uc, Rc, dN);
+    point_op<ORD, EQ>(T, T.f[F_U], o, nb, uc, i, j, k, Rc, dN);
   } else {
     Rc = robin_op_at(cbox_plain(T), i, j, k);
   }
@@ -1297,7 +1332,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
                                (su[p + int(nb.zp[0])] - su[p + int(nb.zm[0])]) * 0.5 * G.idx[2]};
           const double ge[3] = {ld<double>(B.f[F_X0], o), ld<double>(B.f[F_X1], o), ld<double>(B.f[F_X2], o)};
           const double gf[3] = {ld<double>(B.f[F_X3], o), ld<double>(B.f[F_X4], o), ld<double>(B.f[F_X5], o)};
-          em_nonlinear(lu, G.dulap, su[p], g, own_b[q], own_d[q], ge, gf, N, dN);
+          const double du[3] = {0.0, 0.0, 0.0};  // order 2: centred derivatives
+          em_nonlinear(lu, G.dulap, su[p], g, du, own_b[q], own_d[q], ge, gf, N, dN);
         } else {
           nonlinear<EQ>(lu, G.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
         }
@@ -1715,8 +1751,7 @@ constexpr int kLdsLimit = 160 * 1024;
       case 1: OLL_DISPATCH_ORDER_EQ(ORDER, 1, KERNEL, __VA_ARGS__) break; \
       case 2: OLL_DISPATCH_ORDER_EQ(ORDER, 2, KERNEL, __VA_ARGS__) break; \
       case 3: OLL_DISPATCH_ORDER_EQ(ORDER, 3, KERNEL, __VA_ARGS__) break; \
-      /* Punctures_EM: order 2 only (the engine refuses other orders) */  \
-      case kEqPuncturesEm: KERNEL<2, kEqPuncturesEm>(__VA_ARGS__); break; \
+      case kEqPuncturesEm: OLL_DISPATCH_ORDER_EQ(ORDER, kEqPuncturesEm, KERNEL, __VA_ARGS__) break; \
       default: OLL_DISPATCH_ORDER_EQ(ORDER, 0, KERNEL, __VA_ARGS__) break; \
     }                                                                 \
   } while (0)

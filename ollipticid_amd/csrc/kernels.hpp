@@ -75,9 +75,15 @@ __host__ __device__ inline long long pidx(const DevBox &B, int i, int j, int k) 
   return (long long)colour_of(B, i, j, k) * B.cs + (long long)k * B.ps + (long long)j * B.rs +
          split_div(B, i);
 }
-// fills the layout members from n, R
-inline void set_layout(DevBox &B) {
-  B.nc = B.R + 1;
+// colours of the multi-colour sweeps: R + 1 for the axis-star Laplacian of
+// reach R; 5 for Punctures_EM at orders 4-8, whose one-sided first
+// derivatives reach 4 points to one side
+constexpr int colours_of(int order, int eq) {
+  return (eq == kEqPuncturesEm && order >= 4) ? 5 : order / 2 + 1;
+}
+// fills the layout members from n, R and the colour count nc
+inline void set_layout(DevBox &B, int nc) {
+  B.nc = nc;
   B.rl = (B.n[0] + B.nc - 1) / B.nc;
   B.rs = B.rl;
   B.ncm = (65536u + unsigned(B.nc) - 1u) / unsigned(B.nc);
@@ -133,8 +139,7 @@ __host__ __device__ inline long long axis_offset(const DevBox &B, int c, int s, 
   }
   return off;
 }
-inline long long layout_size(const int n[3], int R) {
-  const int nc = R + 1;
+inline long long layout_size(const int n[3], int nc) {
   const long long block = (long long)((n[0] + nc - 1) / nc) * n[1] * n[2];
   return nc * ((block + 31) & ~31LL);
 }

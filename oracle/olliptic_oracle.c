@@ -671,7 +671,10 @@ static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega) {
   double *u = B->f[OF_U], *r = B->f[OF_R], *rho = B->f[OF_RHO];
   const size_t R = B->D.b[0];
   const size_t n0 = B->D.n[0], n1 = B->D.n[1], n2 = B->D.n[2];
-  const int ncol = s->p.sweep_order == ORACLE_COLOUR ? (int)R + 1 : 1;
+  /* colours: R + 1; 5 for Punctures_EM at orders 4-8, whose one-sided
+   * first derivatives reach 4 points (the GPU's choice) */
+  const int ncol = s->p.sweep_order != ORACLE_COLOUR ? 1
+                   : (s->p.eq == ORACLE_PUNCTURES_EM && B->D.order >= 4) ? 5 : (int)R + 1;
   double gLinf = 0;
   for (int kk = 0; kk < nu; kk++) {
     double Linf = 0;

@@ -12,8 +12,10 @@
  * restates.  Two sweep orders are provided:
  *   ORACLE_LEX    - lexicographic point Gauss-Seidel-Newton, the reference's
  *                   own order (elliptic.cc:1126-1242);
- *   ORACLE_COLOUR - (i+j+k) mod (R+1) multi-colour order, R = stencil radius,
- *                   which is the order the GPU kernels use.  Within one colour
+ *   ORACLE_COLOUR - (i+j+k) mod (R+1) multi-colour order, R = stencil radius
+ *                   (mod 5 for Punctures_EM at orders 4-8, whose first
+ *                   derivatives reach 4 points to one side), which is the
+ *                   order the GPU kernels use.  Within one colour
  *                   no point reads another point of the same colour, so the
  *                   result does not depend on the order inside a colour.
  */

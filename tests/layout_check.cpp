@@ -1,7 +1,8 @@
 // Host check of the colour-block field layout (ollipticid_amd/csrc/kernels.hpp):
-// for boxes of every order and a range of shapes,
+// for boxes of every order and a range of shapes, with R + 1 colours and
+// with the 5 colours of Punctures_EM at orders 4-8,
 //  * pidx maps the points of a box one to one into [0, size);
-//  * stencil_offs gives, for every point and step m = 1..R along each axis,
+//  * stencil_offs gives, for every point and step m = 1..NC-1 along each axis,
 //    exactly pidx(neighbour) - pidx(point) (modulo 2^32, as the kernels add
 //    it to 32-bit byte offsets), for element size 1, 4 and 8;
 //  * the colour passes' and whole-row kernels' slot arithmetic (first
@@ -34,7 +35,7 @@ int check_offs(const DevBox &B) {
         const long long p = pidx(B, i, j, k);
         const Nbr<NC> nb = stencil_offs<NC, ES>(colour_of(B, i, j, k), i % NC, unsigned(B.cs),
                                                 unsigned(B.rs), unsigned(B.ps));
-        for (int m = 1; m <= R; ++m) {
+        for (int m = 1; m < NC; ++m) {  // every offset stencil_offs gives (>= R)
           const int ijk[3] = {i, j, k};
           const unsigned *pos[3] = {nb.xp, nb.yp, nb.zp}, *neg[3] = {nb.xm, nb.ym, nb.zm};
           for (int a = 0; a < 3; ++a) {
@@ -52,15 +53,14 @@ int check_offs(const DevBox &B) {
   return 0;
 }
 
-int check_box(int R, int n0, int n1, int n2) {
+int check_box(int R, int NC, int n0, int n1, int n2) {
   DevBox B{};
   B.n[0] = n0;
   B.n[1] = n1;
   B.n[2] = n2;
   B.R = R;
-  set_layout(B);
-  const int NC = R + 1;
-  if (B.size != layout_size(B.n, R)) return fail("layout_size", R, n0, n1, n2, 0, 0, 0, 0, 0);
+  set_layout(B, NC);
+  if (B.size != layout_size(B.n, NC)) return fail("layout_size", R, n0, n1, n2, 0, 0, 0, 0, 0);
   std::vector<char> seen(size_t(B.size), 0);
   for (int k = 0; k < n2; ++k)
     for (int j = 0; j < n1; ++j)
@@ -130,14 +130,15 @@ int main() {
   for (int R = 1; R <= 4; ++R)
     for (const auto &s : shapes) {
       if (s[0] <= 2 * R || s[1] <= 2 * R || s[2] <= 2 * R) continue;
-      if (check_box(R, s[0], s[1], s[2])) return 1;
+      if (check_box(R, R + 1, s[0], s[1], s[2])) return 1;
+      if (R >= 2 && R < 4 && check_box(R, colours_of(2 * R, kEqPuncturesEm), s[0], s[1], s[2])) return 1;
     }
   // the magic division covers i + j + k up to three times the largest axis
   for (int nc = 2; nc <= 5; ++nc) {
     DevBox B{};
     B.n[0] = B.n[1] = B.n[2] = 8;
     B.R = nc - 1;
-    set_layout(B);
+    set_layout(B, nc);
     for (int x = 0; x < 3 * kMaxAxisPoints; ++x)
       if (split_div(B, x) != x / nc) {
         std::printf("split_div(%d) wrong for nc=%d\n", x, nc);

@@ -86,21 +86,22 @@ def test_equation_names_and_robin_values():
     with pytest.raises(ValueError):
         oa.problem_from_argv(["-eq", "Bowen-York"] + base)
     p = oa.problem_from_argv(["-eq", "Poisson"] + base)
-    p.eq = 4
+    p.eq = 5
     with pytest.raises(oa.OllError):
         oa.problem_footprint(p)
     p.eq = oa.EQ_POISSON
     p.initial_guess = oa.GUESS_APPROX
     with pytest.raises(oa.OllError):
         oa.problem_footprint(p)
-    # Punctures_EM: charges parsed; order 2 and FP64 only; six more fields
+    # Punctures_EM: charges parsed; FP64 only; six more fields
     p = oa.problem_from_argv(["-eq", "Punctures_EM", "-bhqp1", "0.25", "-O", "2"] + base)
     assert p.bh[0].q == 0.25
     em_bytes = oa.problem_footprint(p)
     p.eq = oa.EQ_BRILL
     assert em_bytes > oa.problem_footprint(p)
     p.eq = oa.EQ_PUNCTURES_EM
-    for order, prec in ((4, oa.PREC_FP64), (2, oa.PREC_MIXED)):
-        p.order, p.precision = order, prec
-        with pytest.raises(oa.OllError):
-            oa.problem_footprint(p)
+    p.order = 4
+    assert oa.problem_footprint(p) > 0
+    p.precision = oa.PREC_MIXED
+    with pytest.raises(oa.OllError):
+        oa.problem_footprint(p)
