@@ -297,6 +297,15 @@ int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, dou
 int oll_engine_kernel_launches(oll_engine *e, int cap, double *bytes, double *ms, int *kind,
                                int *n);
 
+/* Launch-shape table (no reference counterpart; filled by the autotuner,
+ * ollipticid_amd/autotune.py): threads per workgroup of the smoothers'
+ * colour passes for one precision (OLL_PREC_*) and order, a multiple of 64
+ * from 64 to 512 (default 256; raised to hold one interior row).
+ * Process-wide; schedules built afterwards use it.  Results do not depend
+ * on it.  The setter returns OLL_EINVAL for other values. */
+int oll_set_sweep_threads(int precision, int order, int threads);
+int oll_get_sweep_threads(int precision, int order);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,6 @@ from .native import (  # noqa: F401
     F_ETA_X, F_ETA_Y, F_ETA_Z, F_PHI_X, F_PHI_Y, F_PHI_Z, F_C, F_D, F_E, F_R, F_RHO, F_U, F_V, F_W,
     F_WW, GUESS_APPROX, GUESS_ZERO, METHOD_APP, METHOD_VCYCLE, PREC_FP64, PREC_MIXED,
     STOP_REFERENCE, STOP_TOLERANCE, Engine, Hierarchy, OllError,
-    Problem, default_problem, load,
+    Problem, apply_launch_shapes, default_problem, load, set_sweep_threads, sweep_threads,
     problem_boxes, problem_footprint, problem_from_argv,
 )

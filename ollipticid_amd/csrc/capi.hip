@@ -676,6 +676,18 @@ int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, dou
   return OLL_OK;
 }
 
+int oll_set_sweep_threads(int precision, int order, int threads) {
+  if (precision != OLL_PREC_FP64 && precision != OLL_PREC_MIXED)
+    return fail(OLL_EINVAL, "precision must be OLL_PREC_FP64 or OLL_PREC_MIXED");
+  if (!set_sweep_threads(precision == OLL_PREC_MIXED, order, threads))
+    return fail(OLL_EINVAL, "order must be 2, 4, 6 or 8 and threads a multiple of 64 in 64..512");
+  return OLL_OK;
+}
+
+int oll_get_sweep_threads(int precision, int order) {
+  return sweep_threads(precision == OLL_PREC_MIXED, order);
+}
+
 int oll_engine_profile(const oll_engine *e, double *build_ms, double *run_ms, double *admit_ms,
                        long long *rebuilds, long long *cycles) {
   if (!e) return fail(OLL_EINVAL, "null handle");

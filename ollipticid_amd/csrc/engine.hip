@@ -230,11 +230,14 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     const DevBox &B = H.hbox[size_t(g.level_first[size_t(l)])];
     const int nc = B.nc;
     std::vector<double> cnt(size_t(nc), 0.0);
-    int rows = 0, maxrow = 0;
+    int rows = 0;
     for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
       const DevBox &X = H.hbox[size_t(g.level_first[size_t(l)] + b)];
       rows += interior_rows(X);
-      maxrow = std::max(maxrow, X.n[0] - 2 * X.R);
+      // a workgroup of the row kernels holds at least one whole row
+      if (X.rl > kRowThreads)
+        return err = "a box row exceeds 256 points of one colour (at most 256 x colours points along x)",
+               OLL_EINVAL;
       for (int k = X.R; k < X.n[2] - X.R; ++k)
         for (int j = X.R; j < X.n[1] - X.R; ++j)
           for (int i = X.R; i < X.n[0] - X.R; ++i) cnt[size_t((i + j + k) % nc)] += 1.0;
@@ -395,8 +398,8 @@ struct Builder {
   // row blocks of one level of one member appended to `out`: the interior
   // rows (or colour rows) of each box in runs of kRowThreads / pr
   static void row_blocks(const Hier &H, int level, unsigned long long *norm, bool subrows, int pr,
-                         std::vector<RowBlock> &out) {
-    const int rpb = std::max(1, kRowThreads / pr);
+                         std::vector<RowBlock> &out, int threads = kRowThreads) {
+    const int rpb = std::max(1, threads / pr);
     const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
     for (int b = f; b < f + n; ++b) {
       const DevBox &B = H.hbox[size_t(b)];
@@ -440,10 +443,14 @@ struct Builder {
       }
       std::vector<RowBlock> cb, sb;
       double bytes[5] = {0, 0, 0, 0, 0};
+      // colour-pass workgroup size from the launch-shape table, at least
+      // one interior row
+      int threads = sweep_threads(mixed ? 1 : 0, order);
+      while (threads < pc && threads < kMaxRowThreads) threads *= 2;
       for (int h : g.second) {
         const Hier &H = *M[size_t(h)];
         const int l = lv[size_t(h)];
-        row_blocks(H, l, mixed ? nullptr : slot(H, l), false, pc, cb);
+        row_blocks(H, l, mixed ? nullptr : slot(H, l), false, pc, cb, threads);
         if (mixed) row_blocks(H, l, slot(H, l), true, ps, sb);
         // algorithmic bytes per updated point: FP64 reads u, b, c, rho
         // (+ d, e) and writes u; FP32 reads e, q, r and writes e
@@ -472,6 +479,7 @@ struct Builder {
           L.off = coff;
           L.n = int(cb.size());
           L.max_row = pc;
+          L.threads = threads;
           L.bytes = bytes[q];
           S.launches.push_back(L);
         }
@@ -710,14 +718,14 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         break;
       case Launch::SMOOTH:
         launch_smooth(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq,
-                      L.colour);
+                      L.colour, L.threads);
         break;
       case Launch::DEFECT32:
         launch_defect32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq);
         break;
       case Launch::SMOOTH32:
         launch_smooth32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order,
-                        L.colour);
+                        L.colour, L.threads);
         break;
       case Launch::UPDATE32:
         launch_update32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order);

@@ -86,6 +86,7 @@ struct Launch {
   int n = 0;          // items
   long long total = 0;
   int max_row = 0;    // row kernels: threads per row
+  int threads = kRowThreads;  // colour passes: threads per workgroup
   int max_pts = 0, max_int = 0;
   double bytes = 0;   // algorithmic bytes (smoother launches, FP64 or FP32)
 };

@@ -19,6 +19,7 @@
 //   k_adm_sphere        elliptic.cc:2447-2611 Compute_ADM_Mass
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdio>
 
@@ -339,16 +340,17 @@ __device__ __forceinline__ double wave_max_dpp(double v) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// block max of v (every thread must call), one norm update per block
+// block max of v (every thread must call; blockDim.x a multiple of 64, at
+// most kMaxRowThreads), one norm update per block
 __device__ __forceinline__ void wave_norm(unsigned long long *slot, double v) {
-  __shared__ double sh[kRowThreads / 64];
+  __shared__ double sh[kMaxRowThreads / 64];
   const double m = wave_max_dpp(v);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0 && slot) {
     double b = sh[0];
-#pragma unroll
-    for (int w = 1; w < kRowThreads / 64; ++w) b = fmax(b, sh[w]);
+    const int nw = int(blockDim.x >> 6);
+    for (int w = 1; w < nw; ++w) b = fmax(b, sh[w]);
     if (b > 0.0) norm_update(slot, b);
   }
 }
@@ -417,7 +419,7 @@ __device__ __forceinline__ Nbr<NC> box_offs(ConstBox &B, int c, int s) {
 // smoother: one colour of one Newton-Gauss-Seidel sweep on the rows of
 // every listed block; thread (lr, t) owns colour point t of local row lr
 template <int ORD, int EQ>
-__global__ __launch_bounds__(kRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
+__global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
                                                         unsigned per_row, float inv_per_row,
                                                         int colour) {
   constexpr int R = ORD / 2;
@@ -508,7 +510,7 @@ __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__rest
 }
 
 template <int ORD>
-__global__ __launch_bounds__(kRowThreads) void k_smooth32(const RowBlock *__restrict__ blocks,
+__global__ __launch_bounds__(kMaxRowThreads) void k_smooth32(const RowBlock *__restrict__ blocks,
                                                           unsigned per_row, float inv_per_row,
                                                           int colour) {
   constexpr int R = ORD / 2;
@@ -1690,9 +1692,10 @@ __global__ void k_zero_lists(const ZeroItem *items) {
 inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1) / b); }
 
 template <int ORD, int EQ>
-void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour) {
+void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour,
+                 int threads) {
   if (nb == 0) return;
-  hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+  hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(threads), 0, s, blocks,
                      unsigned(per_row), 1.0f / float(per_row), colour);
 }
 
@@ -1711,9 +1714,10 @@ void defect32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
 }
 
 template <int ORD, int EQ>
-void smooth32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour) {
+void smooth32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour,
+                   int threads) {
   if (nb == 0) return;
-  hipLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
+  hipLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(threads), 0, s, blocks,
                      unsigned(per_row), 1.0f / float(per_row), colour);
 }
 
@@ -1767,9 +1771,23 @@ void launch_approx_box(hipStream_t s, const DevBox &B, const PunctureArgs &pa) {
   hipLaunchKernelGGL(k_approx_u, dim3(ceil_div(np, 256)), dim3(256), 0, s, B, pa);
 }
 
+// threads per colour-pass workgroup, [mixed][order / 2 - 1]
+static int g_sweep_threads[2][4] = {{256, 256, 256, 256}, {256, 256, 256, 256}};
+
+int sweep_threads(int mixed, int order) {
+  return g_sweep_threads[mixed ? 1 : 0][std::min(3, std::max(0, order / 2 - 1))];
+}
+
+bool set_sweep_threads(int mixed, int order, int threads) {
+  if (!(order == 2 || order == 4 || order == 6 || order == 8)) return false;
+  if (threads < 64 || threads > kMaxRowThreads || threads % 64 != 0) return false;
+  g_sweep_threads[mixed ? 1 : 0][order / 2 - 1] = threads;
+  return true;
+}
+
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                   int eq, int colour) {
-  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour);
+                   int eq, int colour, int threads) {
+  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour, threads);
 }
 
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
@@ -1783,8 +1801,8 @@ void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 }
 
 void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                     int colour) {
-  OLL_DISPATCH_ORDER(order, 0, smooth32_impl, s, blocks, nblocks, per_row, colour);
+                     int colour, int threads) {
+  OLL_DISPATCH_ORDER(order, 0, smooth32_impl, s, blocks, nblocks, per_row, colour, threads);
 }
 
 void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order) {

@@ -173,6 +173,12 @@ struct RowBlock {
   int rows;
 };
 constexpr int kRowThreads = 256;  // threads per row workgroup
+// the colour passes (k_smooth, k_smooth32) take 64..kMaxRowThreads threads
+// per workgroup, a multiple of 64 picked per (precision, order) by the
+// launch-shape table (sweep_threads); their results do not depend on it
+constexpr int kMaxRowThreads = 512;
+int sweep_threads(int mixed, int order);
+bool set_sweep_threads(int mixed, int order, int threads);
 
 // point-parallel launches (transfers): one workgroup per PtBlock, which
 // owns target points q0 .. q0+nq-1 (nq <= kPtThreads) of one box, either
@@ -286,7 +292,7 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs 
 // below 4 GiB (checked when the hierarchy is built).
 constexpr long long kMaxFieldBytes = 1LL << 32;
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                   int eq, int colour);
+                   int eq, int colour, int threads);
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                      int eq, int write_defect);
 // mixed precision: FP64 defect -> FP32 (r, q, e = 0), FP32 colour pass of
@@ -294,7 +300,7 @@ void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                      int eq);
 void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                     int colour);
+                     int colour, int threads);
 void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order);
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);

@@ -4,6 +4,7 @@ Loading fails loudly when the HIP library has not been built: there is no
 CPU fallback in the product path.
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -107,6 +108,8 @@ SIGNATURES = {
     "oll_engine_kernel_launches": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double),
                                              C.POINTER(C.c_double), C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)]),
+    "oll_set_sweep_threads": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "oll_get_sweep_threads": (C.c_int, [C.c_int, C.c_int]),
     "oll_engine_profile": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_longlong)]),
@@ -136,7 +139,35 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        apply_launch_shapes()
     return _lib
+
+
+SHAPES_PATH = os.path.join(HERE, "launch_shapes.json")
+PRECISION_NAMES = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+
+
+def set_sweep_threads(precision, order, threads):
+    """Threads per workgroup of the colour passes for one precision and
+    order (a multiple of 64, 64..512); schedules built afterwards use it."""
+    _check(load().oll_set_sweep_threads(precision, order, threads))
+
+
+def sweep_threads(precision, order):
+    return load().oll_get_sweep_threads(precision, order)
+
+
+def apply_launch_shapes(path=SHAPES_PATH):
+    """Applies the autotuner's table (ollipticid_amd/autotune.py writes it)
+    when the file exists; returns the applied {precision: {order: threads}}."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        table = json.load(f).get("sweep_threads", {})
+    for pname, per_order in table.items():
+        for order, threads in per_order.items():
+            set_sweep_threads(PRECISION_NAMES[pname], int(order), int(threads))
+    return table
 
 
 class OllError(RuntimeError):

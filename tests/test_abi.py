@@ -105,3 +105,19 @@ def test_equation_names_and_robin_values():
     p.precision = oa.PREC_MIXED
     with pytest.raises(oa.OllError):
         oa.problem_footprint(p)
+
+
+def test_sweep_threads_table():
+    """The launch-shape table: multiples of 64 from 64 to 512 per precision
+    and order; anything else refused (no GPU needed)."""
+    before = oa.sweep_threads(oa.PREC_MIXED, 6)
+    try:
+        oa.set_sweep_threads(oa.PREC_MIXED, 6, 128)
+        assert oa.sweep_threads(oa.PREC_MIXED, 6) == 128
+        assert oa.sweep_threads(oa.PREC_FP64, 6) in (64, 128, 256, 512)
+        for prec, order, threads in ((oa.PREC_FP64, 4, 96), (oa.PREC_FP64, 4, 1024),
+                                     (oa.PREC_FP64, 3, 256), (7, 4, 256)):
+            with pytest.raises(oa.OllError):
+                oa.set_sweep_threads(prec, order, threads)
+    finally:
+        oa.set_sweep_threads(oa.PREC_MIXED, 6, before)
