@@ -102,6 +102,7 @@ CASES = {
     "small_em_o2": (EM2 + ["-l", "1", "-O", "2", "-nu2", "6", "-s", "1e-16"], 1),
     "small_em_o4": (EM2 + ["-l", "2", "-O", "4", "-nu2", "6", "-s", "1e-16"], 1),
     "small_em_o6": (EM2 + ["-l", "1", "-O", "6", "-nu2", "6", "-s", "1e-16"], 1),
+    "small_em_o8": (EM2 + ["-l", "1", "-O", "8", "-nu2", "6", "-s", "1e-16"], 1),
 }
 
 BOX_RE = re.compile(r"level: (\d+) \| box: (\d+)")

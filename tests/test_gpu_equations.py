@@ -21,9 +21,9 @@ from tests.util import determined_mask, golden_u, load_golden
 
 pytestmark = pytest.mark.gpu
 CASES = ["small_poisson_o2", "small_poisson_o6", "small_brill_o4", "small_brill_o8", "small_em_o2"]
-# Punctures_EM at orders 4 and 6 (5-colour sweeps, one-sided derivatives):
+# Punctures_EM at orders 4, 6 and 8 (5-colour sweeps, one-sided derivatives):
 # stage parity only, the reference's own iteration stagnating near 1e-6
-EM_STAGES = ["small_em_o4", "small_em_o6"]
+EM_STAGES = ["small_em_o4", "small_em_o6", "small_em_o8"]
 
 
 def pair(name):

@@ -12,7 +12,7 @@ from tests.util import golden_u, load_golden
 
 FAST = ["cfg1_tol", "small_o4_offgrid", "small_o6_merge", "small_sf_3bh", "small_o2_4bh",
         "small_poisson_o2", "small_poisson_o6", "small_brill_o4",
-        "small_em_o2", "small_em_o4", "small_em_o6"]
+        "small_em_o2", "small_em_o4", "small_em_o6", "small_em_o8"]
 
 
 @pytest.mark.parametrize("name", FAST)
