@@ -150,25 +150,6 @@ __device__ __forceinline__ void em_nonlinear(double lapu, double dulap, double u
 }
 
 
-__device__ __forceinline__ double warp_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// block max of v; result valid in thread 0; blockDim.x multiple of 64
-__device__ __forceinline__ double block_max(double v, double *sh) {
-  v = warp_max(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) r = fmax(r, sh[w]);
-  }
-  return r;
-}
-
 // Norm slots: non-negative doubles order like their bit patterns, so a max
 // is an unsigned atomic max.  The atomic returns nothing, so the issuing
 // wave does not wait for it.
@@ -633,12 +614,6 @@ __device__ __forceinline__ void robin_point(const DevBox &B, BoxData D, int i, i
   } else {
     D.w[p] = D.rho[p] - op;
   }
-}
-
-__device__ __forceinline__ int face_columns(const DevBox &B) {
-  const int R = B.R;
-  const int ni = B.n[0] - 2 * R, nj = B.n[1] - 2 * R, nk = B.n[2] - 2 * R;
-  return 2 * (nj * nk + ni * nk + ni * nj);
 }
 
 // face column t of a box: all R layers, inner -> outer
