@@ -703,11 +703,14 @@ hipEvent_t timing_event(Timing &t) {
 int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
   for (const Launch &L : S.launches) {
     const void *items = S.dblob + L.off;
-    hipEvent_t e1 = nullptr;
+    // timed smoother launches carry their events into the dispatch: the
+    // runtime stamps them at the kernel's start and end
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (t && t->on && (L.kind == Launch::SMOOTH || L.kind == Launch::SMOOTH32)) {
-      hipEvent_t e0 = timing_event(*t);
-      e1 = timing_event(*t);
-      if (e0) (void)hipEventRecord(e0, s);
+      e0 = timing_event(*t);
+      e1 = e0 ? timing_event(*t) : nullptr;
+      if (e0 && !e1) --t->used;  // keep the events paired
+      if (!e1) e0 = nullptr;
     }
     switch (L.kind) {
       case Launch::ZERO:
@@ -718,14 +721,14 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         break;
       case Launch::SMOOTH:
         launch_smooth(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq,
-                      L.colour, L.threads);
+                      L.colour, L.threads, e0, e1);
         break;
       case Launch::DEFECT32:
         launch_defect32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq);
         break;
       case Launch::SMOOTH32:
         launch_smooth32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order,
-                        L.colour, L.threads);
+                        L.colour, L.threads, e0, e1);
         break;
       case Launch::UPDATE32:
         launch_update32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order);
@@ -753,7 +756,6 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         break;
     }
     if (e1) {
-      (void)hipEventRecord(e1, s);
       t->bytes += L.bytes;
       t->launch_bytes.push_back(L.bytes);
       t->launch_kind.push_back(L.kind == Launch::SMOOTH32 ? 1 : 0);

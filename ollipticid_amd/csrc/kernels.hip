@@ -17,6 +17,7 @@
 //   k_coarse_solve      elliptic.cc:1782-1816 (coarsest-level loop)
 //   k_coefficients      operator.cc:752-868, 2193-2380 (b, c, d, e)
 //   k_adm_sphere        elliptic.cc:2447-2611 Compute_ADM_Mass
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1668,10 +1669,14 @@ inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1)
 
 template <int ORD, int EQ>
 void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour,
-                 int threads) {
+                 int threads, hipEvent_t e0, hipEvent_t e1) {
   if (nb == 0) return;
-  hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(threads), 0, s, blocks,
-                     unsigned(per_row), 1.0f / float(per_row), colour);
+  if (e0)  // timed (never under graph capture): the runtime stamps e0, e1
+    hipExtLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(threads), 0, s, e0, e1, 0, blocks,
+                          unsigned(per_row), 1.0f / float(per_row), colour);
+  else
+    hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(threads), 0, s, blocks,
+                       unsigned(per_row), 1.0f / float(per_row), colour);
 }
 
 template <int ORD, int EQ>
@@ -1690,10 +1695,14 @@ void defect32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row) {
 
 template <int ORD, int EQ>
 void smooth32_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour,
-                   int threads) {
+                   int threads, hipEvent_t e0, hipEvent_t e1) {
   if (nb == 0) return;
-  hipLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(threads), 0, s, blocks,
-                     unsigned(per_row), 1.0f / float(per_row), colour);
+  if (e0)
+    hipExtLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(threads), 0, s, e0, e1, 0, blocks,
+                          unsigned(per_row), 1.0f / float(per_row), colour);
+  else
+    hipLaunchKernelGGL((k_smooth32<ORD>), dim3(nb), dim3(threads), 0, s, blocks,
+                       unsigned(per_row), 1.0f / float(per_row), colour);
 }
 
 template <int ORD, int EQ>
@@ -1761,8 +1770,8 @@ bool set_sweep_threads(int mixed, int order, int threads) {
 }
 
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                   int eq, int colour, int threads) {
-  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour, threads);
+                   int eq, int colour, int threads, hipEvent_t e0, hipEvent_t e1) {
+  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour, threads, e0, e1);
 }
 
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
@@ -1776,8 +1785,8 @@ void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 }
 
 void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                     int colour, int threads) {
-  OLL_DISPATCH_ORDER(order, 0, smooth32_impl, s, blocks, nblocks, per_row, colour, threads);
+                     int colour, int threads, hipEvent_t e0, hipEvent_t e1) {
+  OLL_DISPATCH_ORDER(order, 0, smooth32_impl, s, blocks, nblocks, per_row, colour, threads, e0, e1);
 }
 
 void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order) {

@@ -291,8 +291,9 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs 
 // Row kernels address fields with 32-bit byte offsets: a box field must stay
 // below 4 GiB (checked when the hierarchy is built).
 constexpr long long kMaxFieldBytes = 1LL << 32;
+// e0, e1 (optional): events the runtime stamps at the kernel's start and end
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                   int eq, int colour, int threads);
+                   int eq, int colour, int threads, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                      int eq, int write_defect);
 // mixed precision: FP64 defect -> FP32 (r, q, e = 0), FP32 colour pass of
@@ -300,7 +301,7 @@ void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                      int eq);
 void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                     int colour, int threads);
+                     int colour, int threads, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order);
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
