@@ -924,6 +924,49 @@ __device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, cons
 #undef V
 }
 
+// the same for a 2:1 nested source (DevXfer::affine): corner nodes and
+// weights (1, 0 at a coincident axis; 1/2, 1/2 at a midpoint) from the index
+// map instead of the tables, the eight corners walked as above
+__device__ __forceinline__ double trilinear_nested(ConstBox &S, ConstXfer &X, const int t[3]) {
+  const void *u = S.f[F_U];
+  const void *ww = S.f[F_WW];
+  int c[3];
+  double wi[3], wf[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int q = t[a] + X.aff[a];
+    c[a] = q >> 1;
+    const bool mid = (q & 1) != 0;
+    wi[a] = mid ? 0.5 : 1.0;
+    wf[a] = mid ? 0.5 : 0.0;
+    // a coincident axis reads node c twice (weight 0 on the second): keep
+    // the second node in range at the source's last node
+    if (!mid && c[a] + 1 >= S.n[a]) c[a] = S.n[a] - 2, wi[a] = 0.0, wf[a] = 1.0;
+  }
+  const Walk wk(S);
+  const unsigned c0 = colourc(S, c[0], c[1], c[2]);
+  unsigned r = unsigned(c[0]) - divc(S, unsigned(c[0])) * wk.nc;
+  const unsigned o000 = boff(S, c[0], c[1], c[2]);
+  unsigned cc = c0;
+  const unsigned o100 = o000 + wk.x(cc, r);
+  unsigned c1 = cc;
+  const unsigned o110 = o100 + wk.y(c1);
+  unsigned c2 = c1;
+  const unsigned o111 = o110 + wk.z(c2);
+  unsigned cb = cc;
+  const unsigned o101 = o100 + wk.z(cb);
+  unsigned cy = c0;
+  const unsigned o010 = o000 + wk.y(cy);
+  unsigned cz = cy;
+  const unsigned o011 = o010 + wk.z(cz);
+  unsigned cw = c0;
+  const unsigned o001 = o000 + wk.z(cw);
+#define V(o) (ld<double>(u, o) - ld<double>(ww, o))
+  return ((V(o000) * wi[0] + V(o100) * wf[0]) * wi[1] + (V(o010) * wi[0] + V(o110) * wf[0]) * wf[1]) * wi[2] +
+         ((V(o001) * wi[0] + V(o101) * wf[0]) * wi[1] + (V(o011) * wi[0] + V(o111) * wf[0]) * wf[1]) * wf[2];
+#undef V
+}
+
 // u restriction: coarse INSIDE points inside a fine box's non-ghost part
 // (all = false), injection at coincident points (Lagrange 3 elsewhere)
 __global__ __launch_bounds__(kPtThreads) void k_restrict_u(const PtBlock *__restrict__ blocks) {
@@ -1041,6 +1084,8 @@ __global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restric
     if (f & 4) {
       const unsigned c = boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
       v = ld<double>(S.f[F_U], c) - ld<double>(S.f[F_WW], c);
+    } else if (X.affine) {
+      v = trilinear_nested(S, X, t);
     } else {
       v = trilinear_corr(S, X, t);
     }

@@ -158,6 +158,11 @@ struct DevAxis {
 struct DevXfer {
   const DevBox *src;
   DevAxis ax[3];
+  // linear tables that are 2:1 nestings on all axes (affine = 1): target
+  // index t maps to q = t + aff[axis], source node q/2 (even q) or the
+  // midpoint of nodes (q-1)/2, (q+1)/2 (odd q)
+  int aff[3];
+  int affine;
 };
 
 // row-parallel launches: one workgroup per RowBlock, which owns `rows`
