@@ -99,24 +99,6 @@ struct Pools {
     int aff[3], affine;
   };
 
-  // A linear (trilinear) table is a 2:1 nesting along its axis when, for an
-  // offset m and every target index t inside the source, q = t + m >= 0 and
-  // an even q coincides with source node q/2 while an odd q sits midway
-  // between nodes (q-1)/2 and (q+1)/2 with weights 1/2 (to 1e-12).  Returns
-  // whether it is, with m.
-  static bool nested_axis(const AxisTable &t, int &m) {
-    bool have = false;
-    for (size_t i = 0; i < t.flags.size(); ++i) {
-      if (!(t.flags[i] & (AX_NB | AX_IN))) continue;
-      const bool co = (t.flags[i] & AX_COINC) != 0;
-      const int mi = co ? 2 * t.coinc[i] - int(i) : 2 * t.start[i] + 1 - int(i);
-      if (!have) m = mi, have = true;
-      if (mi != m || int(i) + m < 0) return false;
-      if (!co && (std::fabs(t.w[2 * i] - 0.5) > 1e-12 || std::fabs(t.w[2 * i + 1] - 0.5) > 1e-12))
-        return false;
-    }
-    return have;
-  }
   std::vector<X> xs;
 
   Hier::XRef add(const Hierarchy &g, int tbox, int slevel, int method, int ord) {

@@ -274,4 +274,27 @@ AxisTable build_axis_table(const Box &T, const Box &S, int a, int method, int or
   return t;
 }
 
+bool nested_axis(const AxisTable &t, int &m) {
+  bool have = false;
+  auto near = [](double w, double v) { return std::fabs(w - v) <= 1e-12; };
+  for (size_t i = 0; i < t.flags.size(); ++i) {
+    if (!(t.flags[i] & (AX_NB | AX_IN))) continue;
+    const double w0 = t.w[2 * i], w1 = t.w[2 * i + 1];
+    int q;  // position in half source spacings
+    if (t.flags[i] & AX_COINC)
+      q = 2 * t.coinc[i];
+    else if (near(w0, 0.5) && near(w1, 0.5))
+      q = 2 * t.start[i] + 1;
+    else if (near(w0, 1.0) && near(w1, 0.0))  // on node start (the cell search clamps)
+      q = 2 * t.start[i];
+    else if (near(w0, 0.0) && near(w1, 1.0))
+      q = 2 * t.start[i] + 2;
+    else
+      return false;
+    if (!have) m = q - int(i), have = true;
+    if (q - int(i) != m || q < 0) return false;
+  }
+  return have;
+}
+
 }  // namespace oll

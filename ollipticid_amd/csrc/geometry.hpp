@@ -93,4 +93,12 @@ struct AxisTable {
 AxisTable build_axis_table(const Box &target, const Box &source, int axis,
                            int method, int ord);
 
+// Whether a linear table is a 2:1 nesting along its axis: for an offset m
+// and every target index t inside the source (AX_NB or AX_IN), q = t + m
+// >= 0, an even q sits on source node q/2 (coincident, or weight 1 on that
+// node) and an odd q midway between nodes (q-1)/2 and (q+1)/2 with weights
+// 1/2 (weights to 1e-12).  Sets m.
+// The prolongation then forms its nodes and weights from q (no tables).
+bool nested_axis(const AxisTable &t, int &m);
+
 }  // namespace oll
