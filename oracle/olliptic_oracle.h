@@ -18,6 +18,11 @@
  *                   order the GPU kernels use.  Within one colour
  *                   no point reads another point of the same colour, so the
  *                   result does not depend on the order inside a colour.
+ * Pinning: LEX reproduces the reference's residual histories bit for bit on
+ * every fixture of tests/golden (every equation); COLOUR with full-weighting
+ * restriction and the over-relaxed coarsest loop converges to the
+ * reference's converged fields.  Parity unpinned: the COLOUR iteration of
+ * Punctures_EM at orders 4-8, whose reference runs stop before converging.
  */
 #ifndef OLLIPTIC_ORACLE_H
 #define OLLIPTIC_ORACLE_H
