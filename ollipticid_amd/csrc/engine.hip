@@ -224,6 +224,20 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.dulap = dulap_of(b.order, d.idx2);
     d.ipn = ipn;
     set_layout(d, colours_of(b.order, p.eq));
+    // two-colour boxes: sweep first the colour of the points that coincide
+    // with the coarser level's (the first box of that level fixes the
+    // lattice offset); in ascending order the V-cycle of some deep nested
+    // hierarchies diverges
+    d.flip = 0;
+    if (d.nc == 2 && b.level > 0) {
+      const Box &C = g.boxes[size_t(g.level_first[size_t(b.level - 1)])];
+      int sum = 0;
+      for (int a = 0; a < 3; ++a) {
+        const long m = std::lround((b.lo[a] - C.lo[a]) / b.d[a]);
+        sum += int(((m % 2) + 2) % 2);
+      }
+      d.flip = sum & 1;
+    }
     for (int f = 0; f < NFIELDS; ++f)
       d.f[f] = f < nf ? H.slab + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     for (int f = 0; f < NFIELDS32; ++f)
@@ -242,9 +256,13 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
       if (X.rl > kRowThreads)
         return err = "a box row exceeds 256 points of one colour (at most 256 x colours points along x)",
                OLL_EINVAL;
+      // points updated by colour pass q (flipped boxes update colour nc-1-q)
       for (int k = X.R; k < X.n[2] - X.R; ++k)
         for (int j = X.R; j < X.n[1] - X.R; ++j)
-          for (int i = X.R; i < X.n[0] - X.R; ++i) cnt[size_t((i + j + k) % nc)] += 1.0;
+          for (int i = X.R; i < X.n[0] - X.R; ++i) {
+            const int c = (i + j + k) % nc;
+            cnt[size_t(X.flip ? nc - 1 - c : c)] += 1.0;
+          }
     }
     // row kernels divide row indices with 24-bit exact float division
     if ((long long)rows * nc >= (1LL << 24))

@@ -413,8 +413,9 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__res
   double rmax = 0.0;
   unsigned p;
   int s, i, j, k;
-  if (colour_slot<R, NC>(Bk, B, lr, t, colour, p, s, i, j, k)) {
-    const Nbr<NC> nb = box_offs<NC, 8>(B, colour, s);
+  const int col = B.flip ? NC - 1 - colour : colour;
+  if (colour_slot<R, NC>(Bk, B, lr, t, col, p, s, i, j, k)) {
+    const Nbr<NC> nb = box_offs<NC, 8>(B, col, s);
     const unsigned o = p * 8u;
     double *u = B.f[F_U];
     const double u0 = ld<double>(u, o);
@@ -503,8 +504,9 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth32(const RowBlock *__r
   const int t = int(threadIdx.x - lr * per_row);
   unsigned p;
   int s, i, j, k;
-  if (!colour_slot<R>(Bk, B, lr, t, colour, p, s, i, j, k)) return;
-  const Nbr<NC> nb = box_offs<NC, 4>(B, colour, s);
+  const int col = B.flip ? NC - 1 - colour : colour;
+  if (!colour_slot<R>(Bk, B, lr, t, col, p, s, i, j, k)) return;
+  const Nbr<NC> nb = box_offs<NC, 4>(B, col, s);
   const unsigned o = p * 4u;
   float *e = B.g[G_E];
   const float le = lapb<ORD, float>(e, o, nb, float(B.idx2[0]), float(B.idx2[1]), float(B.idx2[2]));

@@ -54,6 +54,10 @@ struct DevBox {
   float in0, in01;   // 1 / n0, 1 / (n0 n1): point index -> (i, j, k)
   double *f[NFIELDS];
   float *g[NFIELDS32];  // null unless the hierarchy runs mixed precision
+  // two-colour boxes whose points coinciding with the coarser level's are
+  // of colour 1 sweep their colours in descending order (colour pass q
+  // updates colour 1 - q): the coincident colour is swept first
+  int flip;
 };
 
 constexpr int kMaxAxisPoints = 4096;

@@ -666,6 +666,21 @@ static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega);
 /* elliptic.cc:1126-1242 Smooth; lexicographic or multi-colour order */
 double oracle_smooth(oracle_solver *s, int ibox, int nu) { return smooth_omega(s, ibox, nu, 1.0); }
 
+/* colour order of a two-colour box (the GPU's choice): when the box's
+ * points that coincide with the coarser level's (first box of that level)
+ * are of colour 1, its sweeps take colour 1 first */
+static int colour_flip(const oracle_solver *s, const obox *B) {
+  const int lev = B->D.level;
+  if (lev == 0) return 0;
+  const obox *C = &s->box[s->lvl_first[lev - 1]];
+  int sum = 0;
+  for (int a = 0; a < 3; a++) {
+    const long m = lround((dom_x(&B->D, a, 0) - dom_x(&C->D, a, 0)) / B->D.d[a]);
+    sum += (int)(((m % 2) + 2) % 2);
+  }
+  return sum & 1;
+}
+
 static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega) {
   obox *B = &s->box[ibox];
   double *u = B->f[OF_U], *r = B->f[OF_R], *rho = B->f[OF_RHO];
@@ -678,7 +693,8 @@ static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega) {
   double gLinf = 0;
   for (int kk = 0; kk < nu; kk++) {
     double Linf = 0;
-    for (int col = 0; col < ncol; col++)
+    for (int q = 0; q < ncol; q++)
+      for (int col = (ncol == 2 && colour_flip(s, B)) ? ncol - 1 - q : q; col >= 0; col = -1)
       for (size_t k = R; k < n2 - R; k++)
         for (size_t j = R; j < n1 - R; j++)
           for (size_t i = R; i < n0 - R; i++) {
