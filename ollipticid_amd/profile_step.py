@@ -1,13 +1,17 @@
-"""One batched solve of the equal-mass binary workload (the benchmark's step),
-issued launch by launch so that a profiler can attribute hardware counters
-to each kernel dispatch, plus an optional device-to-device copy of known size
-that calibrates the HBM byte counters.
+"""One batched solve of a named workload, issued launch by launch so that a
+profiler can attribute kernel durations and hardware counters to each
+dispatch, plus an optional device-to-device copy of known size that
+calibrates the HBM byte counters.
 
     rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'k_smooth|copy' \\
         -- python3 -m ollipticid_amd.profile_step --batch 16 --cycles 2 --copy-mib 512
+    rocprofv3 --kernel-trace --stats \\
+        -- python3 -m ollipticid_amd.profile_step --workload scalar_shell --batch 1 --cycles 0
 
-Prints one JSON line: the workload, the number of V-cycles run and, per
-smoother family, the launches and algorithmic bytes of one V-cycle.
+Workloads are those of ollipticid_amd.workloads (default: the equal-mass
+binary, N=65, 9 levels, 4th order).  Prints one JSON line: the workload, the
+number of V-cycles run and the smoother launches, device time and
+algorithmic bytes over the run.
 """
 import argparse
 import json
@@ -17,16 +21,12 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
-PUNC = ["-eq", "Punctures", "-B", "Robin", "-BA", "0", "-Bn", "1"]
-BINARY = PUNC + ["-NumBH", "2", "-bhmp1", "0.487209", "-bhy1", "5.5",
-                 "-bhpx1", "-0.0901099", "-bhpy1", "-0.000703975",
-                 "-bhmp2", "0.487209", "-bhy2", "-5.5",
-                 "-bhpx2", "0.0901099", "-bhpy2", "0.000703975",
-                 "-L", "1000", "-N", "65", "-l", "9", "-O", "4", "-nu2", "20", "-s", "1e-11"]
-
 
 def main():
+    from ollipticid_amd.workloads import WORKLOADS
+
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="binary")
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--cycles", type=int, default=2, help="V-cycles per request (0: to convergence)")
@@ -36,9 +36,10 @@ def main():
 
     import ollipticid_amd as oa
 
-    p = oa.problem_from_argv(BINARY)
+    p = oa.problem_from_argv(WORKLOADS[args.workload])
     p.precision = oa.PREC_MIXED if args.precision == "mixed" else oa.PREC_FP64
-    p.max_cycles = args.cycles
+    if args.cycles:
+        p.max_cycles = args.cycles
     e = oa.Engine(graphs=False)
     for t in range(args.batch):
         assert e.admit(p, t)
@@ -50,9 +51,10 @@ def main():
     launches, ms, nbytes = e.kernel_timing()
     done = e.pop_done()
     e.close()
-    out = {"workload": "equal-mass binary N=65 l=9 O4 nu2=20", "batch": args.batch,
-           "precision": args.precision, "cycles": cycles, "requests_done": len(done),
-           "smoother_launches": launches, "smoother_ms": ms, "smoother_bytes": nbytes}
+    out = {"workload": args.workload, "argv": " ".join(WORKLOADS[args.workload]),
+           "batch": args.batch, "precision": args.precision, "cycles": cycles,
+           "requests_done": len(done), "smoother_launches": launches, "smoother_ms": ms,
+           "smoother_bytes": nbytes}
     if args.copy_mib:
         # the HIP runtime's device-to-device copy kernel on buffers of known size
         import ctypes as C

@@ -39,6 +39,17 @@ CFG2 = PUNC + ["-NumBH", "2", "-bhmp1", "0.487209", "-bhy1", "5.5",
                "-bhpx2", "0.0901099", "-bhpy2", "0.000703975",
                "-L", "1000", "-N", "65", "-l", "9", "-O", "4", "-nu2", "20"]
 
+# the three punctures with the scalar-field shell of the N=161 workload
+# (ollipticid_amd/workloads.py, positions seeded 2026)
+SF3 = ["-eq", "Punctures_scalar_field", "-B", "Robin", "-BA", "0", "-Bn", "1",
+       "-psc", "2", "-phi0", "0.02", "-a2", "1", "-r0", "10", "-sigma", "4", "-NumBH", "3",
+       "-bhmp1", "0.40000000000000002", "-bhx1", "-3.8527822358947659",
+       "-bhy1", "1.6789579885818551", "-bhz1", "-0.39277918627817865",
+       "-bhmp2", "0.29999999999999999", "-bhx2", "-1.553993674703424",
+       "-bhy2", "-1.7409919882841853", "-bhz2", "3.4862189502391807",
+       "-bhmp3", "0.29999999999999999", "-bhx3", "1.6304496497358159",
+       "-bhy3", "3.0327851649490398", "-bhz3", "0.18184435586822101"]
+
 EM2 = ["-eq", "Punctures_EM", "-B", "Robin", "-NumBH", "2", "-bhmp1", "0.5", "-bhqp1", "0.2",
        "-bhx1", "3.13", "-bhy1", "0.17", "-bhz1", "0.07", "-bhmp2", "0.4", "-bhqp2", "-0.1",
        "-bhx2", "-3.29", "-bhy2", "0.41", "-bhz2", "-0.11", "-L", "20", "-N", "25"]
@@ -49,6 +60,17 @@ CASES = {
     "cfg1_conv": (CFG1 + ["-s", "1e-16"], 1),
     "cfg2_tol": (CFG2 + ["-s", "1e-11"], 4),
     "cfg2_conv": (CFG2 + ["-s", "1e-16"], 4),
+    # the binary of cfg2 at 8th order (cfg3's shape, N=129 reduced to 25)
+    # and the three punctures with the scalar-field shell at 6th order
+    # (cfg5's shape, N=161 reduced to 25), both with every level: two
+    # post-sweeps per level keep the finest level's norm from reaching
+    # round-off before the coarser levels have converged, so the stopping
+    # rule ends the runs with every level converged (the discrete solution
+    # does not depend on the sweep counts)
+    "cfg3r_o8": (CFG2[:-10] + ["-L", "1000", "-N", "25", "-l", "9", "-O", "8", "-nu2", "2",
+                               "-s", "1e-16"], 2),
+    "cfg5r_o6": (SF3 + ["-L", "256", "-N", "25", "-l", "10", "-O", "6", "-nu2", "2",
+                        "-s", "1e-16"], 2),
     # small cases covering orders 4/6/8, off-lattice punctures, spins,
     # box merging and the scalar-field equation
     "small_o4_offgrid": (PUNC + [

@@ -3,7 +3,9 @@ outputs (tests/golden, written by tests/golden/make_golden.py from the
 reference binary compiled from its sources).  In the reference's
 lexicographic order the oracle must reproduce the residual history
 (Loo.gp, 18 digits) exactly, the cycle count, the converged u to the
-fixtures' 12 printed digits and the ADM masses to their 9 printed digits."""
+fixtures' 12 printed digits, the puncture ADM masses and every box's
+surface ADM mass (elliptic.cc:2471-2606, "AMDmass=") to their 9 printed
+digits."""
 import numpy as np
 import pytest
 
@@ -33,10 +35,28 @@ def test_oracle_matches_reference(name):
         u = s.field(ib, O.F_U)[::st, ::st, ::st]
         scale = max(1.0, np.abs(ref).max())
         assert np.abs(u - ref).max() <= 1e-11 * scale + 1e-14
-    if meta["adm_mass"]:  # printed for the puncture equations only
-        adm, _ = s.adm_mass()
-        # stdout prints 9 significant digits
-        np.testing.assert_allclose(adm, meta["adm_mass"], rtol=5.01e-9)
+    check_adm(s, meta)
+
+
+def check_adm(s, meta):
+    """Puncture and per-box surface ADM masses against the reference's
+    stdout (9 significant digits, so half a unit of the 9th digit)."""
+    if not meta["adm_mass"]:  # printed for the puncture equations only
+        return
+    adm, surf = s.adm_mass()
+    np.testing.assert_allclose(adm, meta["adm_mass"], rtol=5.01e-9)
+    ref_surf = [b["surface_adm"] for b in meta["boxes"]]
+    assert len(ref_surf) == len(surf) and all(v is not None for v in ref_surf)
+    for ib, (got, ref) in enumerate(zip(surf, ref_surf)):
+        assert abs(got - ref) <= 5.01e-9 * abs(ref) + 1e-300, (ib, got, ref)
+
+
+@pytest.mark.slow
+def test_oracle_matches_reference_cfg2():
+    """The equal-mass binary (N=65, 9 levels, 4th order, tol 1e-11): 17
+    cycles, Loo.gp history bit for bit, u to print precision and the ADM
+    masses; about 90 s on one core."""
+    test_oracle_matches_reference("cfg2_tol")
 
 
 def test_oracle_converged_cfg1_field():
