@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <string>
 
 namespace oll {
 
@@ -185,21 +186,46 @@ std::string build_hierarchy(size_t Nin, double L, int levels, int order,
     }
   }
 
-  // merge boxes of a level that overlap or touch; rescan after a merge
-  // (elliptic.cc:839-896, with the reference's index bookkeeping)
+  // Merge boxes of a level that overlap or touch, rescanning the level
+  // after a merge (elliptic.cc:839-896).  The reference keeps every box of
+  // every level in one list D and addresses box i of level l as the level's
+  // first index plus i, with i running up to the level's count as it stood
+  // when the pair was reached: after a merge of (i, j) with j < i erases box
+  // j, D[first + i] is the next box in the list, possibly the first box of
+  // level l + 1, and that box is what receives the union.  The same list
+  // arithmetic is kept here, so the hierarchy is the reference's one also in
+  // those cases; an index past the end of the list (where the reference
+  // reads past its vector) is refused.
+  std::vector<Box> D;
+  std::vector<int> ndl(size_t(h.levels), 0);
+  for (int l = 0; l < h.levels; ++l) {
+    ndl[size_t(l)] = int(per_level[size_t(l)].size());
+    D.insert(D.end(), per_level[size_t(l)].begin(), per_level[size_t(l)].end());
+  }
   for (int l = h.mgrid + 1; l < h.levels; ++l) {
-    std::vector<Box> &lv = per_level[size_t(l)];
-    bool merged = false;
-    for (size_t i = 0; i < lv.size(); ++i)
-      for (size_t j = 0; j < lv.size(); ++j) {
+    bool restore = false;
+    for (int i = 0; i < ndl[size_t(l)]; ++i)
+      for (int j = 0; j < ndl[size_t(l)]; ++j) {
         if (i == j) continue;
-        if (touches(lv[i], lv[j])) {
-          lv[i] = merge(lv[i], lv[j]);
-          lv.erase(lv.begin() + long(j));
-          merged = true;
+        size_t first = 0;
+        for (int ll = 0; ll < l; ++ll) first += size_t(ndl[size_t(ll)]);
+        const size_t ib = first + size_t(i), jb = first + size_t(j);
+        if (ib >= D.size() || jb >= D.size())
+          return "box merging at level " + std::to_string(l) +
+                 " reaches past the box list (the reference's grid set-up fails here)";
+        if (touches(D[ib], D[jb])) {
+          D[ib] = merge(D[ib], D[jb]);
+          D.erase(D.begin() + long(jb));
+          --ndl[size_t(l)];
+          restore = true;
         }
       }
-    if (merged) --l;
+    if (restore) --l;
+  }
+  size_t q = 0;
+  for (int l = 0; l < h.levels; ++l) {
+    per_level[size_t(l)].assign(D.begin() + long(q), D.begin() + long(q) + ndl[size_t(l)]);
+    q += size_t(ndl[size_t(l)]);
   }
 
   h.boxes.clear();

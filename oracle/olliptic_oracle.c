@@ -331,6 +331,18 @@ static int build_grids(oracle_solver *s) {
           for (int ll = 0; ll < l; ll++) ib += (size_t)ndl[ll];
           jb = ib + (size_t)j;
           ib += (size_t)i;
+          /* box i of level l is addressed in the list of every level's
+           * boxes: after a merge erased an earlier box, D[ib] may be a box
+           * of level l + 1, as in the reference; past the end of the list
+           * (where the reference reads past its vector) the set-up fails */
+          if (ib >= nd || jb >= nd) {
+            set_err("box merging at level %d reaches past the box list", l);
+            free(D);
+            free(ndl);
+            free(nx);
+            free(dx);
+            return -1;
+          }
           if (dom_touch(&D[ib], &D[jb])) {
             odom m;
             dom_add(&m, &D[ib], &D[jb]);
@@ -1051,12 +1063,18 @@ static void set_in(obox *B, const odom *fine) {
       }
 }
 
-/* Defect restriction used with multi-colour sweeps: the same target points
- * and precedence as the reference's Transferir(W, ALL, LAGRANGE 2, all)
- * (elliptic.cc:1660-1685), but where the coarse point coincides with a fine
- * point whose 3x3x3 neighbourhood lies in the fine interior, the value is
- * the full-weighting average (1/4,1/2,1/4)^3 instead of injection.  A
- * converged FAS solution has zero defect, so the fixed point is unchanged. */
+/* Defect restriction used with multi-colour sweeps: the same target points,
+ * precedence and Lagrange-2 nodes as the reference's Transferir(W, ALL,
+ * LAGRANGE 2, all) (elliptic.cc:1660-1685), applied to the filtered defect:
+ * at every fine node whose 3x3x3 neighbourhood lies in the fine interior the
+ * defect is replaced by its full-weighting average (1/4,1/2,1/4)^3, elsewhere
+ * kept.  Where the coarse point coincides with a fine node this is full
+ * weighting instead of injection; where a merged fine box lies off the
+ * coarse lattice, the filter removes the (i+j+k)-parity pattern that the
+ * multi-colour sweeps leave in the defect and that the interpolation would
+ * alias into the coarse correction (without it the V-cycle of such
+ * hierarchies diverges, even for the Laplacian).  A converged FAS solution
+ * has zero interior defect, so the fixed point is the reference's. */
 static int coincident_index(const odom *D, int a, double x, long *idx) {
   size_t i = dom_get_i(D, a, x);
   for (long c = (long)i - 1; c <= (long)i; c++) {
@@ -1071,35 +1089,49 @@ static int coincident_index(const odom *D, int a, double x, long *idx) {
 
 static void restrict_defect_fw(oracle_solver *s, int k, int all) {
   static const double w[3] = {0.25, 0.5, 0.25};
-  for (int ib = 0; ib < s->lvl_count[k]; ib++)
+  for (int ib = 0; ib < s->lvl_count[k]; ib++) {
+    const obox *S = &s->box[s->lvl_first[k] + ib];
+    const double *sf = S->f[OF_W];
+    /* the filtered defect of the fine box */
+    double *flt = malloc(S->npts * sizeof(double));
+    const long n0 = (long)S->D.n[0], n1 = (long)S->D.n[1], n2 = (long)S->D.n[2];
+    const long R0 = (long)S->D.b[0], R1 = (long)S->D.b[1], R2 = (long)S->D.b[2];
+    for (long kk = 0; kk < n2; kk++)
+      for (long j = 0; j < n1; j++)
+        for (long i = 0; i < n0; i++) {
+          double val = sf[IDX(S, i, j, kk)];
+          if (i - 1 >= R0 && i + 1 < n0 - R0 && j - 1 >= R1 && j + 1 < n1 - R1 && kk - 1 >= R2 &&
+              kk + 1 < n2 - R2) {
+            val = 0;
+            for (int c = -1; c <= 1; c++)
+              for (int b = -1; b <= 1; b++)
+                for (int a = -1; a <= 1; a++)
+                  val += w[a + 1] * w[b + 1] * w[c + 1] * sf[IDX(S, i + a, j + b, kk + c)];
+          }
+          flt[IDX(S, i, j, kk)] = val;
+        }
     for (int jb = 0; jb < s->lvl_count[k - 1]; jb++) {
       obox *T = &s->box[s->lvl_first[k - 1] + jb];
-      const obox *S = &s->box[s->lvl_first[k] + ib];
-      const double *sf = S->f[OF_W];
       for (size_t kk = 0; kk < T->D.n[2]; kk++)
         for (size_t j = 0; j < T->D.n[1]; j++)
           for (size_t i = 0; i < T->D.n[0]; i++) {
             double X[3] = {dom_x(&T->D, 0, (long)i), dom_x(&T->D, 1, (long)j), dom_x(&T->D, 2, (long)kk)};
             int take = dom_is_in_nb(&S->D, X) || (all && dom_is_in(&S->D, X));
             if (!take) continue;
+            /* within 1e-6 dx of a node whose neighbourhood is interior: the
+             * full-weighting value of that node */
             long fi[3];
             int fw = 1;
             for (int a = 0; a < 3 && fw; a++) {
               if (!coincident_index(&S->D, a, X[a], &fi[a])) fw = 0;
               else if (fi[a] - 1 < (long)S->D.b[a] || fi[a] + 1 >= (long)(S->D.n[a] - S->D.b[a])) fw = 0;
             }
-            double val;
-            if (fw) {
-              val = 0;
-              for (int c = -1; c <= 1; c++)
-                for (int b = -1; b <= 1; b++)
-                  for (int a = -1; a <= 1; a++)
-                    val += w[a + 1] * w[b + 1] * w[c + 1] * get_val(S, sf, fi[0] + a, fi[1] + b, fi[2] + c);
-            } else
-              val = lagrange_interp(S, sf, 2, X[0], X[1], X[2]);
-            T->f[OF_W][IDX(T, i, j, kk)] = val;
+            T->f[OF_W][IDX(T, i, j, kk)] = fw ? flt[IDX(S, fi[0], fi[1], fi[2])]
+                                              : lagrange_interp(S, flt, 2, X[0], X[1], X[2]);
           }
     }
+    free(flt);
+  }
 }
 
 /* ------------------------------------------------------------------ */
