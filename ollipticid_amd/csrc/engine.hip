@@ -97,6 +97,7 @@ struct Pools {
     int src;
     int start[3], coinc[3], flags[3], w[3], k[3], n[3];
     int aff[3], affine;
+    int filt;  // a Lagrange defect restriction reaching non-coincident nodes
   };
 
   std::vector<X> xs;
@@ -121,6 +122,11 @@ struct Pools {
         x.k[a] = t.k;
         x.n[a] = int(t.start.size());
         if (method == 0 && !nested_axis(t, x.aff[a])) x.affine = -1;
+        // a target coordinate inside the source that is no source node:
+        // the defect restriction interpolates there (k_filter_defect)
+        if (method == 1 && ord == 2)
+          for (uint8_t f : t.flags)
+            if ((f & (AX_NB | AX_IN)) && !(f & AX_COINC)) x.filt = 1;
       }
       x.affine = (method == 0 && x.affine == 0) ? 1 : 0;
       xs.push_back(x);
@@ -294,6 +300,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
   if ((rc = upload(&H.bpool, P.bp, H.bytes, err))) return rc;
   if ((rc = upload(&H.dpool, P.dp, H.bytes, err))) return rc;
   std::vector<DevXfer> xf;
+  H.filt_src.assign(g.boxes.size(), 0);
   for (const Pools::X &x : P.xs) {
     DevXfer d{};
     d.src = H.dbox + x.src;
@@ -307,6 +314,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
       d.aff[a] = x.aff[a];
     }
     d.affine = x.affine;
+    d.filt = x.filt;
+    if (x.filt) H.filt_src[size_t(x.src)] = 1;
     xf.push_back(d);
   }
   if ((rc = upload(&H.dxfer, xf, H.bytes, err))) return rc;
@@ -536,7 +545,8 @@ struct Builder {
     }
   }
 
-  void robin(const std::vector<Hier *> &M, const std::vector<int> &lv, int mode) {
+  // only_box >= 0: that box of the level alone (one member)
+  void robin(const std::vector<Hier *> &M, const std::vector<int> &lv, int mode, int only_box = -1) {
     std::vector<RobinItem> items;
     int total = 0;
     for (size_t h = 0; h < M.size(); ++h) {
@@ -544,6 +554,7 @@ struct Builder {
       const Hier &H = *M[h];
       const int f = H.geo.level_first[size_t(lv[h])], n = H.geo.level_count[size_t(lv[h])];
       for (int b = f; b < f + n; ++b) {
+        if (only_box >= 0 && b != only_box) continue;
         const int nc = face_columns(H.hbox[size_t(b)]);
         items.push_back(RobinItem{H.dbox + b, total, nc});
         total += nc;
@@ -602,6 +613,32 @@ struct Builder {
     }
   }
 
+  // the defect filter of every box of level lv[h] that is the source of a
+  // defect restriction off the coarser lattice
+  void filter(const std::vector<Hier *> &M, const std::vector<int> &lv) {
+    std::vector<PtBlock> items;
+    for (size_t h = 0; h < M.size(); ++h) {
+      const int k = lv[h];
+      if (k < 1) continue;
+      const Hier &H = *M[h];
+      const int f = H.geo.level_first[size_t(k)], n = H.geo.level_count[size_t(k)];
+      for (int b = f; b < f + n; ++b) {
+        if (!H.filt_src[size_t(b)]) continue;
+        const DevBox &B = H.hbox[size_t(b)];
+        const long long np = npts(B), plane = (long long)B.n[0] * B.n[1];
+        for (long long q0 = 0; q0 < np; q0 += kPtThreads)
+          items.push_back(PtBlock{H.dbox + b, nullptr, 0, 0, unsigned(q0),
+                                  unsigned(std::min<long long>(kPtThreads, np - q0)),
+                                  int(q0 % B.n[0]), int((q0 % plane) / B.n[0]), int(q0 / plane)});
+      }
+    }
+    if (items.empty()) return;
+    Launch L{Launch::FILTER};
+    L.off = S.add(items);
+    L.n = int(items.size());
+    S.launches.push_back(L);
+  }
+
   void coarse(const std::vector<Hier *> &M) {
     std::map<int, std::vector<CoarseItem>> groups;  // eq -> items
     std::map<int, std::pair<int, int>> sizes;
@@ -654,9 +691,10 @@ int build_vcycle_from(const std::vector<Hier *> &M, const std::vector<int> &star
     }
     b.robin(M, mglv, ROBIN_UPDATE);
     b.smooth(M, lv, nu1, false);
-    b.residual(M, lv, 1, false);
+    b.residual(M, lv, RES_DEFECT, false);
     b.robin(M, mglv, ROBIN_DEFECT_W);
     b.transfer(Launch::RESTRICT_U, M, lv);
+    b.filter(M, lv);
     b.transfer(Launch::RESTRICT_DEFECT, M, lv);
   }
   b.coarse(M);
@@ -679,22 +717,24 @@ int build_vcycle(const std::vector<Hier *> &M, Schedule &S, std::string &err) {
   return build_vcycle_from(M, start, S, err);
 }
 
-int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err) {
+int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err, int box) {
   (void)err;
   S.clear();
   std::vector<Hier *> M{&H};
   std::vector<int> lv{level};
   Builder b{S};
-  if (stage == STAGE_SMOOTH || stage == STAGE_RESIDUAL) b.zero(M);
+  if (stage == STAGE_SMOOTH || stage == STAGE_RESIDUAL || stage == STAGE_DERIVATIVE) b.zero(M);
   if (stage == STAGE_SMOOTH) {
     // the scratch slot collects the norm of these sweeps
     b.smooth(M, lv, nu, false, H.slot_scratch());
   } else if (stage == STAGE_RESIDUAL) {
-    b.residual(M, lv, 0, true);
+    b.residual(M, lv, RES_OPERATOR, true);
+  } else if (stage == STAGE_DERIVATIVE) {
+    b.residual(M, lv, RES_DERIVATIVE, true);
   } else if (stage == STAGE_ROBIN) {
-    b.robin(M, lv, ROBIN_UPDATE);
+    b.robin(M, lv, ROBIN_UPDATE, box);
   } else {
-    b.robin(M, lv, ROBIN_OPERATOR_R);
+    b.robin(M, lv, ROBIN_OPERATOR_R, box);
   }
   return OLL_OK;
 }
@@ -775,6 +815,9 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
       case Launch::PROLONG:
         launch_prolong(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
+      case Launch::FILTER:
+        launch_filter_defect(s, reinterpret_cast<const PtBlock *>(items), L.n);
+        break;
       case Launch::GHOST:
         launch_ghost_fill(s, reinterpret_cast<const PtBlock *>(items), L.n, L.order);
         break;
@@ -782,7 +825,7 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
     if (e1) {
       t->bytes += L.bytes;
       t->launch_bytes.push_back(L.bytes);
-      t->launch_kind.push_back(L.kind == Launch::SMOOTH32 ? 1 : 0);
+      t->launch_kind.push_back(2 * L.order + (L.kind == Launch::SMOOTH32 ? 1 : 0));
     }
   }
   ENG_CHECK(hipGetLastError());
@@ -890,10 +933,26 @@ struct TempBuffers {
 };
 
 int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::string &err) {
+  const int nb = int(H.hbox.size());
+  const size_t nsph = surface ? size_t(nb) * kAdmNt * kAdmNt : 0;
+  AdmBuffers d;
+  TempBuffers tmp;
+  ENG_CHECK(tmp.alloc(&d.pbox, size_t(H.prob.num_bh)));
+  ENG_CHECK(tmp.alloc(&d.pxyz, size_t(3 * H.prob.num_bh)));
+  ENG_CHECK(tmp.alloc(&d.up, size_t(H.prob.num_bh)));
+  if (surface) {
+    ENG_CHECK(tmp.alloc(&d.ctr, size_t(3 * nb)));
+    ENG_CHECK(tmp.alloc(&d.sph, nsph));
+  }
+  return adm_mass_with(H, s, d, puncture, surface, err);
+}
+
+int adm_mass_with(Hier &H, hipStream_t s, const AdmBuffers &d, double *puncture, double *surface,
+                  std::string &err) {
   const oll_problem &p = H.prob;
   const int top = H.top();
   const int nb = int(H.hbox.size());
-  const int nt = 4 * 25 + 1;
+  const int nt = kAdmNt;
   // u at each puncture from its finest box (Lagrange order 4);
   // multigrid.cc:97-116 falls back to the level's first box
   std::vector<int> pbox(size_t(p.num_bh));
@@ -907,25 +966,17 @@ int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::str
   std::vector<double> ctr(size_t(3 * nb));
   for (int b = 0; b < nb; ++b)
     for (int a = 0; a < 3; ++a) ctr[size_t(3 * b + a)] = H.geo.boxes[size_t(b)].c[a];
-  int *d_pbox = nullptr;
-  double *d_pxyz = nullptr, *d_up = nullptr, *d_ctr = nullptr, *d_sph = nullptr;
   const size_t nsph = surface ? size_t(nb) * nt * nt : 0;
-  TempBuffers tmp;
-  ENG_CHECK(tmp.alloc(&d_pbox, pbox.size()));
-  ENG_CHECK(tmp.alloc(&d_pxyz, pxyz.size()));
-  ENG_CHECK(tmp.alloc(&d_up, pbox.size()));
-  ENG_CHECK(hipMemcpyAsync(d_pbox, pbox.data(), pbox.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  ENG_CHECK(hipMemcpyAsync(d_pxyz, pxyz.data(), pxyz.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  launch_interp_points(s, H.dbox, F_U, d_pbox, d_pxyz, p.num_bh, 4, d_up);
+  ENG_CHECK(hipMemcpyAsync(d.pbox, pbox.data(), pbox.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  ENG_CHECK(hipMemcpyAsync(d.pxyz, pxyz.data(), pxyz.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_interp_points(s, H.dbox, F_U, d.pbox, d.pxyz, p.num_bh, 4, d.up);
   std::vector<double> up(pbox.size()), sph(nsph);
   if (surface) {
-    ENG_CHECK(tmp.alloc(&d_ctr, ctr.size()));
-    ENG_CHECK(tmp.alloc(&d_sph, nsph));
-    ENG_CHECK(hipMemcpyAsync(d_ctr, ctr.data(), ctr.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    launch_adm_sphere(s, H.dbox, nb, d_ctr, d_sph, nt);
-    ENG_CHECK(hipMemcpyAsync(sph.data(), d_sph, nsph * sizeof(double), hipMemcpyDeviceToHost, s));
+    ENG_CHECK(hipMemcpyAsync(d.ctr, ctr.data(), ctr.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_adm_sphere(s, H.dbox, nb, d.ctr, d.sph, nt);
+    ENG_CHECK(hipMemcpyAsync(sph.data(), d.sph, nsph * sizeof(double), hipMemcpyDeviceToHost, s));
   }
-  ENG_CHECK(hipMemcpyAsync(up.data(), d_up, up.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  ENG_CHECK(hipMemcpyAsync(up.data(), d.up, up.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   ENG_CHECK(hipStreamSynchronize(s));
   // elliptic.cc:2448-2469
   for (int i = 0; i < p.num_bh && puncture; ++i) {

@@ -39,6 +39,9 @@ struct Hier {
   };
   // [fine level k][target box within its level]
   std::vector<std::vector<XRef>> ru, rw, pv, gf;
+  // per box: source of a defect restriction off the coarser lattice, whose
+  // defect is filtered before the restriction (k_filter_defect)
+  std::vector<char> filt_src;
   unsigned long long *norms = nullptr;  // nslots * kNormWords
   int nslots = 0;
   int *iters = nullptr;
@@ -78,7 +81,7 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err);
 struct Launch {
   enum Kind {
     ZERO, ROBIN, SMOOTH, RESIDUAL, RESTRICT_U, RESTRICT_DEFECT, COARSE, PROLONG, GHOST,
-    DEFECT32, SMOOTH32, UPDATE32
+    DEFECT32, SMOOTH32, UPDATE32, FILTER
   };
   Kind kind;
   int order = 2, eq = 0, colour = 0, mode = 0;
@@ -118,14 +121,16 @@ struct Timing {
   size_t used = 0;
   double bytes = 0;
   std::vector<double> launch_bytes;  // per timed launch, in issue order
-  std::vector<int> launch_kind;      // 0: FP64 smoother, 1: FP32 correction smoother
+  // 2 * order + (0: FP64 smoother colour pass, 1: FP32 correction colour pass)
+  std::vector<int> launch_kind;
 };
 
 // the full V-cycle of every member; all members must share nu1, nu2
 int build_vcycle(const std::vector<Hier *> &members, Schedule &S, std::string &err);
-// single stages of one hierarchy on one level (tests and tools)
-int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err);
-enum Stage { STAGE_SMOOTH = 0, STAGE_RESIDUAL, STAGE_ROBIN, STAGE_BOUNDARY_OP };
+// single stages of one hierarchy on one level (tests, tools and the class
+// API); box >= 0 restricts the Robin stages to that box of the level
+int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err, int box = -1);
+enum Stage { STAGE_SMOOTH = 0, STAGE_RESIDUAL, STAGE_ROBIN, STAGE_BOUNDARY_OP, STAGE_DERIVATIVE };
 
 int upload_schedule(Schedule &S, hipStream_t s, std::string &err);
 // runs the schedule; with graphs = true it is captured as a graph on its
@@ -145,8 +150,18 @@ int finish_cycle(Hier &H, hipStream_t s, std::string &err);
 // V-cycle schedule whose member h starts at level start[h] (FASN(start))
 int build_vcycle_from(const std::vector<Hier *> &members, const std::vector<int> &start,
                       Schedule &S, std::string &err);
-// ADM masses (elliptic.cc:2447-2611); surface may be null
+// ADM masses (elliptic.cc:2447-2611); surface may be null.  adm_mass
+// allocates its device staging per call; adm_mass_with works in the
+// caller's buffers: pbox (num_bh ints), pxyz (3 num_bh), up (num_bh), and,
+// with surface, ctr (3 nbox) and sph (nbox kAdmNt^2 doubles)
+constexpr int kAdmNt = 4 * 25 + 1;  // sphere samples per angle (elliptic.cc:2471-2480)
+struct AdmBuffers {
+  int *pbox = nullptr;
+  double *pxyz = nullptr, *up = nullptr, *ctr = nullptr, *sph = nullptr;
+};
 int adm_mass(Hier &H, hipStream_t s, double *puncture, double *surface, std::string &err);
+int adm_mass_with(Hier &H, hipStream_t s, const AdmBuffers &d, double *puncture, double *surface,
+                  std::string &err);
 // elliptic.cc:2232-2444 Get_Puncture_App values (host arithmetic)
 void puncture_approx(const Hier &H, double *u_at_p);
 // Lagrange interpolation of `field` at npts host points (box[q], xyz[3q..3q+2])

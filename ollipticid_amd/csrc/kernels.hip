@@ -430,11 +430,12 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__res
   wave_norm(Bk.norm, rmax);
 }
 
-// residual on the interior: r = N(u) (write_defect = 0) or w = rho - N(u)
+// operator on the interior: mode RES_OPERATOR r = N(u), RES_DEFECT
+// w = rho - N(u), RES_DERIVATIVE r = dN/du_p; the norm is max |N(u)|
 template <int ORD, int EQ>
 __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__restrict__ blocks,
                                                           unsigned per_row, float inv_per_row,
-                                                          int write_defect) {
+                                                          int mode) {
   constexpr int R = ORD / 2;
   constexpr int NC = colours_of(ORD, EQ);
   const RowBlock Bk = blocks[xcd_block()];
@@ -448,10 +449,10 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
     const double *u = B.f[F_U];
     double N, dN;
     point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), i, j, k, N, dN);
-    if (write_defect)
+    if (mode == RES_DEFECT)
       st<double>(B.f[F_W], o, ld<double>(B.f[F_RHO], o) - N);
     else
-      st<double>(B.f[F_R], o, N);
+      st<double>(B.f[F_R], o, mode == RES_DERIVATIVE ? dN : N);
     rmax = fabs(N);
   }
   wave_norm(Bk.norm, rmax);
@@ -988,9 +989,61 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_u(const PtBlock *__rest
   }
 }
 
+// full weighting (1/4, 1/2, 1/4)^3 of field w of box S around node
+// (ci, cj, ck), whose 3x3x3 neighbourhood must lie in the box
+__device__ __forceinline__ double fw27(ConstBox &S, const void *w, int ci, int cj, int ck) {
+  // walk the 27 points from (ci-1, cj-1, ck-1)
+  const Walk wk(S);
+  unsigned oz = boff(S, ci - 1, cj - 1, ck - 1);
+  unsigned cz = colourc(S, ci - 1, cj - 1, ck - 1);
+  const unsigned r0 = unsigned(ci - 1) - divc(S, unsigned(ci - 1)) * wk.nc;
+  double acc = 0.0;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const double wz = dz == 0 ? 0.5 : 0.25;
+    unsigned oy = oz, cy = cz;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const double wy = dy == 0 ? 0.5 : 0.25;
+      unsigned cx = cy, r = r0;
+      const unsigned om = oy;
+      const unsigned o0 = om + wk.x(cx, r);
+      const unsigned op = o0 + wk.x(cx, r);
+      const double wm = ld<double>(w, om);
+      const double w0 = ld<double>(w, o0);
+      const double wp = ld<double>(w, op);
+      acc += wz * wy * (0.25 * wm + 0.5 * w0 + 0.25 * wp);
+      oy += wk.y(cy);
+    }
+    oz += wk.z(cz);
+  }
+  return acc;
+}
+
+// Filtered defect of a fine box whose defect is restricted to points of a
+// coarser level that do not coincide with its nodes (a merged box off the
+// coarser lattice): v = the full weighting of w at every node whose 3x3x3
+// neighbourhood is interior, v = w elsewhere.  The restriction then
+// interpolates v instead of w: the filter removes the (i+j+k)-parity
+// pattern that the multi-colour sweeps leave in the defect, which the
+// Lagrange interpolation would otherwise alias into the coarse correction.
+// At a converged u the interior defect is zero, so the fixed point is the
+// one of the unfiltered restriction.
+__global__ __launch_bounds__(kPtThreads) void k_filter_defect(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &S = cbox(Bk.box);
+  int i, j, k;
+  if (!pt_lex(Bk, S, i, j, k)) return;
+  const int R = S.R;
+  const unsigned o = boff(S, i, j, k);
+  const bool in = i - 1 >= R && i + 1 < S.n[0] - R && j - 1 >= R && j + 1 < S.n[1] - R &&
+                  k - 1 >= R && k + 1 < S.n[2] - R;
+  st<double>(S.f[F_V], o, in ? fw27(S, S.f[F_W], i, j, k) : ld<double>(S.f[F_W], o));
+}
+
 // Coarse FAS source: W_c = restriction of the fine defect (full weighting
 // where the coarse point coincides with a fine point whose 3x3x3
-// neighbourhood is fine interior, injection / Lagrange-2 elsewhere), then
+// neighbourhood is fine interior, injection / Lagrange-2 elsewhere, of the
+// filtered defect k_filter_defect leaves in v when the source box is off
+// the coarse lattice), then
 // rho_c = W_c + R_c with R_c = N(u_c) inside, the Robin operator on the
 // shell (multigrid levels).  Refinement levels overwrite rho_c only where
 // the point is inside both boxes' non-ghost parts (ffunction.cc:2808-2849).
@@ -1016,35 +1069,10 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
     const unsigned f = axis_flags(X, t, fl);
     if (!((f & 1) || (Bk.mg && (f & 2)))) continue;
     ConstBox &S = cbox(X.src);
-    const void *w = S.f[F_W];
     if (f & 8) {
-      const int ci = X.ax[0].coinc[i], cj = X.ax[1].coinc[j], ck = X.ax[2].coinc[k];
-      // walk the 27 points from (ci-1, cj-1, ck-1)
-      const Walk wk(S);
-      unsigned oz = boff(S, ci - 1, cj - 1, ck - 1);
-      unsigned cz = colourc(S, ci - 1, cj - 1, ck - 1);
-      const unsigned r0 = unsigned(ci - 1) - divc(S, unsigned(ci - 1)) * wk.nc;
-      double acc = 0.0;
-      for (int dz = -1; dz <= 1; ++dz) {
-        const double wz = dz == 0 ? 0.5 : 0.25;
-        unsigned oy = oz, cy = cz;
-        for (int dy = -1; dy <= 1; ++dy) {
-          const double wy = dy == 0 ? 0.5 : 0.25;
-          unsigned cx = cy, r = r0;
-          const unsigned om = oy;
-          const unsigned o0 = om + wk.x(cx, r);
-          const unsigned op = o0 + wk.x(cx, r);
-          const double wm = ld<double>(w, om);
-          const double w0 = ld<double>(w, o0);
-          const double wp = ld<double>(w, op);
-          acc += wz * wy * (0.25 * wm + 0.5 * w0 + 0.25 * wp);
-          oy += wk.y(cy);
-        }
-        oz += wk.z(cz);
-      }
-      wc = acc;
+      wc = fw27(S, S.f[F_W], X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
     } else {
-      wc = tinterp<3>(S, w, X, t, fl);
+      wc = tinterp<3>(S, X.filt ? S.f[F_V] : S.f[F_W], X, t, fl);
     }
     covered = true;
     break;
@@ -1727,10 +1755,10 @@ void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int
 }
 
 template <int ORD, int EQ>
-void residual_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int write_defect) {
+void residual_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int mode) {
   if (nb == 0) return;
   hipLaunchKernelGGL((k_residual<ORD, EQ>), dim3(nb), dim3(kRowThreads), 0, s, blocks,
-                     unsigned(per_row), 1.0f / float(per_row), write_defect);
+                     unsigned(per_row), 1.0f / float(per_row), mode);
 }
 
 template <int ORD, int EQ>
@@ -1822,8 +1850,8 @@ void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_r
 }
 
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                     int eq, int write_defect) {
-  OLL_DISPATCH_ORDER(order, eq, residual_impl, s, blocks, nblocks, per_row, write_defect);
+                     int eq, int mode) {
+  OLL_DISPATCH_ORDER(order, eq, residual_impl, s, blocks, nblocks, per_row, mode);
 }
 
 void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
@@ -1854,6 +1882,11 @@ void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks) {
 
 void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq) {
   OLL_DISPATCH_ORDER(order, eq, restrict_defect_impl, s, blocks, nblocks);
+}
+
+void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks) {
+  if (nblocks == 0) return;
+  hipLaunchKernelGGL(k_filter_defect, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
 }
 
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks) {

@@ -167,6 +167,9 @@ struct DevXfer {
   // midpoint of nodes (q-1)/2, (q+1)/2 (odd q)
   int aff[3];
   int affine;
+  // defect restrictions from a source off the target's lattice: the
+  // Lagrange nodes read the filtered defect (F_V, k_filter_defect)
+  int filt;
 };
 
 // row-parallel launches: one workgroup per RowBlock, which owns `rows`
@@ -303,8 +306,11 @@ constexpr long long kMaxFieldBytes = 1LL << 32;
 // e0, e1 (optional): events the runtime stamps at the kernel's start and end
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                    int eq, int colour, int threads, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// what k_residual writes on the interior: r = N(u), w = rho - N(u), or
+// r = dN/du_p (the Newton diagonal, elliptic.h:337-345 duOPERATOR)
+enum ResidualMode { RES_OPERATOR = 0, RES_DEFECT = 1, RES_DERIVATIVE = 2 };
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                     int eq, int write_defect);
+                     int eq, int mode);
 // mixed precision: FP64 defect -> FP32 (r, q, e = 0), FP32 colour pass of
 // the linearised correction equation, FP64 update u += e
 void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
@@ -315,6 +321,8 @@ void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq);
+// v = full-weighting filter of the defect w (all points of the listed boxes)
+void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks);
 // order: stencil order of the target (refinement) boxes; Lagrange order+1
 void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order);
