@@ -15,8 +15,6 @@
 #include <string>
 #include <vector>
 
-namespace olliptic {
-
 enum t_equation {
   PUNCTURES = 0, PUNCTURES_SCALAR = 1, POISSON = 2, BRILL_WAVES = 3, PUNCTURES_EM = 4,
   OTHER_EQUATION = 99
@@ -91,5 +89,3 @@ class interface {
 
 // interface.cc:883-949
 void Valid_N(size_t &N);
-
-}  // namespace olliptic

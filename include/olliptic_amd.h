@@ -165,6 +165,9 @@ typedef struct oll_result {
   int cycles;           /* V-cycles run */
   double residual;      /* last top-level post-smoothing norm */
   int hist_len;         /* entries written to history (cycle, norm) pairs */
+  int converged;        /* 1 when the solve ended with residual <= tol (the
+                           reference's stopping rule may end it above tol, at
+                           the first cycle that does not lower the residual) */
 } oll_result;
 
 /* Defaults of the reference CLI (interface.cc:35-355). */
@@ -203,6 +206,14 @@ int oll_smooth(oll_hier *h, int level, int nu, double *linf);
 int oll_residual(oll_hier *h, int level, double *linf);
 /* boundary.cc:447-800 App_Bound (Robin update of the ghost shell). */
 int oll_robin(oll_hier *h, int level);
+/* boundary.cc:447-800 App_Bound(l, ibox): the Robin update of one box's
+ * ghost shell (elliptic.h:301). */
+int oll_robin_box(oll_hier *h, int ibox);
+/* elliptic.h:337-345 OPERATOR / duOPERATOR over the interior of every box of
+ * one level: derivative 0 writes N(u) into OLL_F_R (as oll_residual),
+ * derivative 1 writes dN/du at the point itself (the Newton diagonal of
+ * elliptic.cc:1126-1242).  linf = max |N(u)| either way. */
+int oll_operator(oll_hier *h, int level, int derivative, double *linf);
 /* boundary.cc:1118-1535 App_Boundary_Operator: r = Robin operator on shell. */
 int oll_boundary_operator(oll_hier *h, int level);
 /* elliptic.cc:1469-1968 FASN(k): one FAS step of level k >= 1 (recursive). */
@@ -292,8 +303,9 @@ int oll_engine_profile(const oll_engine *e, double *build_ms, double *run_ms, do
 int oll_engine_set_kernel_timing(oll_engine *e, int on);
 int oll_engine_kernel_timing(oll_engine *e, int *launches, double *total_ms, double *bytes);
 /* The timed smoother launches one by one, in issue order: algorithmic bytes,
- * device ms and kind (0 FP64 sweep colour, 1 FP32 correction colour) of the
- * first cap of them; *n is how many there are. */
+ * device ms and kind (2 * stencil order + 0 for an FP64 sweep colour, + 1
+ * for an FP32 correction colour) of the first cap of them; *n is how many
+ * there are. */
 int oll_engine_kernel_launches(oll_engine *e, int cap, double *bytes, double *ms, int *kind,
                                int *n);
 

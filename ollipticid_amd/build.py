@@ -39,10 +39,18 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    cli = [os.path.join(CSRC, "cli", f) for f in ("interface.cpp", "elliptic.cpp", "main.cpp")]
-    if all(os.path.exists(f) for f in cli):
-        _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + os.path.join(REPO, "include")] + cli +
-             ["-o", EXE, "-L" + HERE, "-lolliptic_amd", "-Wl,-rpath,$ORIGIN"])
+    cli_lib = [os.path.join(CSRC, "cli", f) for f in ("interface.cpp", "elliptic.cpp")]
+    cxx = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + os.path.join(REPO, "include")]
+    _run(cxx + cli_lib + [os.path.join(CSRC, "cli", "main.cpp"), "-o", EXE, "-L" + HERE,
+                          "-lolliptic_amd", "-Wl,-rpath,$ORIGIN"])
+    # the test-suite's host program written against the class API
+    # (tests/dropin_main.cpp), linked like the command line
+    dropin = os.path.join(REPO, "tests", "dropin_main.cpp")
+    if os.path.exists(dropin):
+        outdir = os.path.join(REPO, "tests", "_build")
+        os.makedirs(outdir, exist_ok=True)
+        _run(cxx + cli_lib + [dropin, "-o", os.path.join(outdir, "dropin_main.x"), "-L" + HERE,
+                              "-lolliptic_amd", "-Wl,-rpath,$ORIGIN/../../ollipticid_amd"])
     return LIB
 
 

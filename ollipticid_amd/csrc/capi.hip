@@ -61,9 +61,9 @@ int hier_check(const oll_hier *h, int level) {
   return OLL_OK;
 }
 
-int run_stage(oll_hier *h, int level, int stage, int nu, double *linf) {
+int run_stage(oll_hier *h, int level, int stage, int nu, double *linf, int box = -1) {
   std::string err;
-  int rc = build_stage(h->H, level, stage, nu, h->scratch, err);
+  int rc = build_stage(h->H, level, stage, nu, h->scratch, err, box);
   if (!rc) rc = run_schedule(h->scratch, h->stream, nullptr, false, err);
   if (!rc && linf) {
     rc = read_norms(h->H, h->stream, err);
@@ -257,6 +257,22 @@ int oll_robin(oll_hier *h, int level) {
   return rc ? rc : run_stage(h, level, STAGE_ROBIN, 0, nullptr);
 }
 
+int oll_robin_box(oll_hier *h, int ibox) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  if (ibox < 0 || ibox >= int(h->H.hbox.size())) return fail(OLL_EINVAL, "box out of range");
+  return run_stage(h, h->H.hbox[size_t(ibox)].level, STAGE_ROBIN, 0, nullptr, ibox);
+}
+
+int oll_operator(oll_hier *h, int level, int derivative, double *linf) {
+  int rc = hier_check(h, level);
+  if (rc) return rc;
+  if (derivative != 0 && derivative != 1) return fail(OLL_EINVAL, "derivative must be 0 or 1");
+  double v = 0;
+  rc = run_stage(h, level, derivative ? STAGE_DERIVATIVE : STAGE_RESIDUAL, 0, &v);
+  if (linf) *linf = v;
+  return rc;
+}
+
 int oll_boundary_operator(oll_hier *h, int level) {
   int rc = hier_check(h, level);
   return rc ? rc : run_stage(h, level, STAGE_BOUNDARY_OP, 0, nullptr);
@@ -292,7 +308,7 @@ int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
     float ms = 0;
     CAPI_CHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
-    if (res) *res = oll_result{0, 0.0, 0};
+    if (res) *res = oll_result{0, 0.0, 0, 1};
     return OLL_OK;
   }
   if (h->H.top() < 1) return fail(OLL_EINVAL, "hierarchy has a single level");
@@ -328,6 +344,7 @@ int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap) {
     res->cycles = H.cycle;
     res->residual = H.res;
     res->hist_len = nh;
+    res->converged = H.res <= H.eps_solv ? 1 : 0;
   }
   return OLL_OK;
 }
@@ -591,6 +608,7 @@ int oll_engine_pop_done(oll_engine *e, long long *tag, oll_result *res, double *
     res->cycles = H.cycle;
     res->residual = H.res;
     res->hist_len = int(H.hist.size() / 2);
+    res->converged = H.status == 0 && H.res <= H.eps_solv ? 1 : 0;
   }
   int rc = H.status;
   if (!rc && adm) {
@@ -623,6 +641,7 @@ int oll_engine_pop_hier(oll_engine *e, long long *tag, oll_result *res, oll_hier
     res->cycles = m.H->cycle;
     res->residual = m.H->res;
     res->hist_len = int(m.H->hist.size() / 2);
+    res->converged = m.H->status == 0 && m.H->res <= m.H->eps_solv ? 1 : 0;
   }
   e->in_use -= std::min(e->in_use, m.H->bytes);
   const int status = m.H->status;

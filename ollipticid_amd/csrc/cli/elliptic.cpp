@@ -12,14 +12,57 @@
 #include <limits>
 #include <sstream>
 
-namespace olliptic {
-
 namespace fs = std::filesystem;
 
 // ---------------------------------------------------------------- views
+void ffunction::ensure() {
+  if (!fetched_ || (gen_ && seen_ != *gen_)) Fetch();
+}
+
 double ffunction::Get_val(size_t i, size_t j, size_t k) {
-  if (!fetched_) Fetch();
+  ensure();
   return host_[i + Get_nx() * (j + Get_ny() * k)];
+}
+
+bool ffunction::in_region(size_t i, size_t j, size_t k) const {
+  if (it_ == ALL) return true;
+  const size_t R = d_.Get_buffer();
+  const bool inner = i >= R && i < Get_nx() - R && j >= R && j < Get_ny() - R && k >= R &&
+                     k < Get_nz() - R;
+  return it_ == INSIDE ? inner : !inner;
+}
+
+void ffunction::Set_Iterator(t_iterator it) {
+  it_ = it;
+  ci_ = cj_ = ck_ = 0;
+  done_ = false;
+  if (!in_region(0, 0, 0)) End();
+}
+
+bool ffunction::End() {
+  if (done_) return false;
+  do {
+    if (++ci_ == Get_nx()) {
+      ci_ = 0;
+      if (++cj_ == Get_ny()) {
+        cj_ = 0;
+        if (++ck_ == Get_nz()) {
+          done_ = true;
+          return false;
+        }
+      }
+    }
+  } while (!in_region(ci_, cj_, ck_));
+  return true;
+}
+
+double ffunction::Get_val() { return Get_val(ci_, cj_, ck_); }
+
+void ffunction::Set(double y) { Set(ci_, cj_, ck_, y); }
+
+void ffunction::Set(size_t i, size_t j, size_t k, double y) {
+  ensure();
+  host_[i + Get_nx() * (j + Get_ny() * k)] = y;
 }
 
 void ffunction::Fetch() {
@@ -27,11 +70,14 @@ void ffunction::Fetch() {
   if (oll_get_field(h_, box_, field_, host_.data()) != OLL_OK)
     throw std::runtime_error(oll_last_error());
   fetched_ = true;
+  if (gen_) seen_ = *gen_;
 }
 
 void ffunction::Store() {
   if (oll_set_field(h_, box_, field_, host_.data()) != OLL_OK)
     throw std::runtime_error(oll_last_error());
+  // every other mirror of the hierarchy may now be stale; this one is not
+  if (gen_) seen_ = ++*gen_;
 }
 
 size_t domain::index_of(int a, double x) const {
@@ -102,7 +148,7 @@ const domain &multigrid::Get_domain(size_t level, size_t box) const {
   return dom_[Get_dom_index(level, box)];
 }
 
-ffunction multigrid::Get_ffunction(size_t level, size_t box, const std::string &name) const {
+ffunction *multigrid::Get_ffunction(size_t level, size_t box, const std::string &name) const {
   static const char *names[OLL_NFIELDS] = {"u", "b", "c", "d", "e", "rho0", "r0", "v0", "w0", "ww0"};
   int f = -1;
   for (int i = 0; i < OLL_NFIELDS; ++i)
@@ -110,7 +156,9 @@ ffunction multigrid::Get_ffunction(size_t level, size_t box, const std::string &
   if (name == "rho") f = OLL_F_RHO;
   if (f < 0) throw std::runtime_error("unknown field " + name);
   const size_t ib = Get_dom_index(level, box);
-  return ffunction(h_, int(ib), f, dom_[ib]);
+  std::unique_ptr<ffunction> &p = fields_[{int(ib), f}];
+  if (!p) p.reset(new ffunction(h_, int(ib), f, dom_[ib], gen_));
+  return p.get();
 }
 
 size_t multigrid::Box_Of(double x, double y, double z) const {
@@ -125,10 +173,10 @@ size_t multigrid::Box_Of(double x, double y, double z) const {
 
 std::vector<double> multigrid::Interpol(const std::string &name, const std::vector<int> &box,
                                         const std::vector<double> &xyz, size_t order) const {
-  const ffunction f = Get_ffunction(0, 0, name);
+  const ffunction *f = Get_ffunction(0, 0, name);
   std::vector<double> out(box.size());
   if (box.empty()) return out;
-  if (oll_interp(h_, f.Field(), int(box.size()), box.data(), xyz.data(), int(order),
+  if (oll_interp(h_, f->Field(), int(box.size()), box.data(), xyz.data(), int(order),
                  out.data()) != OLL_OK)
     throw std::runtime_error(oll_last_error());
   return out;
@@ -333,13 +381,65 @@ void elliptic::Make(interface *Oll) {
   fs::create_directories(name, ec);
   std::ofstream(dir_ + name + ".run") << Oll->Command_Line() << "\n";
 
-  if (oll_hier_create(&prob_, &h_) != OLL_OK) {
-    ok_ = false;
-    err_ = oll_last_error();
-    return;
-  }
+  if (oll_hier_create(&prob_, &h_) != OLL_OK) return failed(oll_last_error());
   mg_ = multigrid(h_);
 }
+
+// the reference ends the program from inside on errors (output.cc:20-111
+// error_exit, elliptic.cc:1194-1207): so do the class's void methods
+void elliptic::failed(const std::string &msg, bool print) {
+  ok_ = false;
+  err_ = msg;
+  if (print) std::cerr << "Error: " << err_ << std::endl;
+  std::exit(1);
+}
+
+// boundary.cc:8-84
+void elliptic::Set_Boundary() {
+  if (!ok_) return;
+  if (oll_reset(h_) != OLL_OK) return failed(oll_last_error());
+  mg_.Touch();
+}
+
+// boundary.cc:447-907
+void elliptic::App_Bound(size_t l, size_t ibox) {
+  if (!ok_) return;
+  if (oll_robin_box(h_, int(mg_.Get_dom_index(l, ibox))) != OLL_OK) return failed(oll_last_error());
+  mg_.Touch();
+}
+
+// elliptic.h:337-345 at the iterator point of box (l, ibox)'s u
+double elliptic::op_at(int which, size_t lvar, size_t l, size_t ibox) {
+  if (!ok_) return 0.0;
+  if (lvar != 0) {
+    failed("the puncture equations have one variable (lvar 0)");
+    return 0.0;
+  }
+  ffunction *u = mg_.Get_ffunction(l, ibox, "u");
+  const int ib = u->Box();
+  if (op_level_[which] != int(l) || op_gen_[which] != mg_.Generation()) {
+    double linf = 0;
+    if (oll_operator(h_, int(l), which, &linf) != OLL_OK) {
+      failed(oll_last_error());
+      return 0.0;
+    }
+    op_level_[which] = int(l);
+    op_val_[which].clear();
+    op_gen_[which] = mg_.Generation();
+  }
+  std::vector<double> &v = op_val_[which][ib];
+  if (v.empty()) {
+    v.resize(u->Get_N());
+    if (oll_get_field(h_, ib, OLL_F_R, v.data()) != OLL_OK) {
+      failed(oll_last_error());
+      return 0.0;
+    }
+  }
+  return v[u->Get_Index()];
+}
+
+double elliptic::OPERATOR(size_t lvar, size_t l, size_t ibox) { return op_at(0, lvar, l, ibox); }
+double elliptic::duOPERATOR(size_t lvar, size_t l, size_t ibox) { return op_at(1, lvar, l, ibox); }
 
 // output.cc:1030-1345 (the puncture and box tables)
 void elliptic::Print_Info() {
@@ -407,11 +507,7 @@ void elliptic::print_post(int k, int top, double norm, int cycle) {
 void elliptic::compute_adm(bool print) {
   adm_.assign(size_t(prob_.num_bh), 0.0);
   std::vector<double> surf(size_t(oll_num_boxes(h_)));
-  if (oll_adm_mass(h_, adm_.data(), surf.data()) != OLL_OK) {
-    ok_ = false;
-    err_ = oll_last_error();
-    return;
-  }
+  if (oll_adm_mass(h_, adm_.data(), surf.data()) != OLL_OK) return failed(oll_last_error());
   if (!print || quiet_) return;
   std::cout << std::scientific << std::setprecision(8) << "Puncure method: \n";
   for (size_t i = 0; i < adm_.size(); ++i) std::cout << "ADM mass[" << i << "] = " << adm_[i] << "\n";
@@ -427,7 +523,8 @@ void elliptic::compute_adm(bool print) {
 
 // elliptic.cc:2232-2444 Get_Puncture_App: the approximation at each
 // puncture's grid node, printed
-void elliptic::print_puncture_approx() {
+void elliptic::Get_Puncture_App() {
+  if (!ok_) return;
   std::vector<double> up(size_t(prob_.num_bh), 0.0);
   if (oll_puncture_approx(h_, up.data()) != OLL_OK) return;
   std::cout << std::scientific << std::setprecision(7);
@@ -439,18 +536,15 @@ void elliptic::print_puncture_approx() {
 
 // elliptic.cc:1977-2222 ApproximateID (-m AppPuncture): u is the
 // approximation, no iteration
-void elliptic::approximate_id() {
+void elliptic::ApproximateID() {
+  if (!ok_) return;
   oll_result r{};
-  if (oll_solve(h_, &r, nullptr, 0) != OLL_OK) {
-    ok_ = false;
-    err_ = oll_last_error();
-    std::cerr << "Error: " << err_ << "\n";
-    return;
-  }
+  if (oll_solve(h_, &r, nullptr, 0) != OLL_OK) return failed(oll_last_error());
+  mg_.Touch();
   for (size_t l = 0; l < mg_.Get_levels(); ++l)
     for (size_t b = 0; b < mg_.Get_num_dom(l); ++b)
       std::cout << " == Approximate ID: level -" << l << "- box -" << b << "-\n" << std::endl;
-  print_puncture_approx();
+  Get_Puncture_App();
   cycles_ = 0;
   res_ = 0;
 }
@@ -458,20 +552,22 @@ void elliptic::approximate_id() {
 // elliptic.cc:1251-1342 Vcycle with the reference's per-cycle logging
 void elliptic::Solve() {
   if (!ok_) return;
-  if (prob_.method == OLL_METHOD_APP) return approximate_id();
-  if (fit_approx_) print_puncture_approx();
+  if (prob_.method == OLL_METHOD_APP) return ApproximateID();
+  Vcycle();
+}
+
+void elliptic::Vcycle() {
+  if (!ok_) return;
+  if (fit_approx_) Get_Puncture_App();
   const int top = int(mg_.Get_levels()) - 1;
   std::vector<double> pre(size_t(top) + 1), post(size_t(top) + 1);
   double res = 1e12, res_prev;
   int cycle = 0;
   do {
     res_prev = res;
-    if (oll_fas(h_, top, &res) != OLL_OK) {
-      ok_ = false;
-      err_ = oll_last_error();
-      std::cerr << "Error: " << err_ << "\n";
-      return;
-    }
+    const int rc = oll_fas(h_, top, &res);
+    mg_.Touch();
+    if (rc != OLL_OK) return failed(oll_last_error());
     oll_last_norms(h_, pre.data(), post.data(), top + 1);
     for (int k = top; k >= 0; --k) print_pre(k, top, pre[size_t(k)], cycle);
     for (int k = 1; k <= top; ++k) print_post(k, top, post[size_t(k)], cycle);
@@ -505,7 +601,7 @@ void elliptic::Print_Solution(double Dxyz, double Lx, double Ly, double Lz) {
   if (print_3d_) {
     for (size_t l = 0; l < mg_.Get_levels(); ++l)
       for (size_t b = 0; b < mg_.Get_num_dom(l); ++b) {
-        ffunction u = mg_.Get_ffunction(l, b, "u");
+        ffunction &u = *mg_.Get_ffunction(l, b, "u");
         const domain &d = *u.Get_domain();
         std::ostringstream fn;
         fn << dir_ << "u-l" << l << "b" << b << "-0.xyz";
@@ -679,8 +775,16 @@ bool multigrid::Print_IO_Zcode(const std::string &name, const std::string &var, 
   return ok;
 }
 
-bool elliptic::IO_bam(const std::string &file) {
-  if (!ok_) return false;
+void elliptic::IO_bam(std::string file) {
+  // the reference's message is printed where the point is found
+  if (ok_ && !export_bam(file)) failed("a BAM point lies outside the grid", false);
+}
+
+void elliptic::IO_Zcode(std::string file) {
+  if (ok_ && !export_zcode(file)) failed("a Zcode point lies outside the grid", false);
+}
+
+bool elliptic::export_bam(const std::string &file) {
   static const char *ending[] = {"", "a", "b", "c", "d", "e"};
   std::vector<std::string> names;
   size_t found = 0;
@@ -699,8 +803,7 @@ bool elliptic::IO_bam(const std::string &file) {
   return true;
 }
 
-bool elliptic::IO_Zcode(const std::string &file) {
-  if (!ok_) return false;
+bool elliptic::export_zcode(const std::string &file) {
   std::vector<std::string> names;
   size_t found = 0;
   for (size_t level = 0; level == 0 || found > 0; ++level) {
@@ -725,4 +828,3 @@ bool elliptic::IO_Zcode(const std::string &file) {
   return true;
 }
 
-}  // namespace olliptic

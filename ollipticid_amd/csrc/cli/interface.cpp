@@ -6,9 +6,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <iostream>
 #include <sstream>
-
-namespace olliptic {
 
 // interface.cc:883-949
 void Valid_N(size_t &N) {
@@ -118,6 +117,17 @@ interface::interface(int argc, char *argv[]) {
   bhmp.assign(1, 0);
   read_console(argc, argv);
   proc_parameters();
+  // interface.cc:341-351: a command line the solver cannot run prints the
+  // usage and ends the program with status 0
+  if (!exec) {
+    std::cout << "Olliptic.x (MI355X): " << error << "\n"
+              << "usage: Olliptic.x -eq Punctures|Punctures_scalar_field -B Robin [-BA a -Bn n]\n"
+              << "       -NumBH k -bhmp<i> m -bhx<i> .. -bhsz<i> .. -L L -N N -O 2|4|6|8 -l levels\n"
+              << "       -s tol -nu1 n -nu2 n -o dir [-psc c -phi0 p -r0 r -sigma s -a2 a]\n"
+              << "       [-p1d yes|no] [-pint yes|no] [-int yes|no] [-Dp d] [-p3d yes|no]\n"
+              << "       [-iob yes -ioZ yes -iof file] [-so yes|partial|no] [-prec fp64|mixed]\n";
+    std::exit(0);
+  }
 }
 
 // interface.cc:366-874
@@ -304,5 +314,3 @@ std::string interface::Get_string_par(const std::string &n) const {
   auto it = s_par.find(n);
   return it == s_par.end() ? std::string() : it->second;
 }
-
-}  // namespace olliptic

@@ -58,7 +58,8 @@ class BoxDesc(C.Structure):
 
 
 class Result(C.Structure):
-    _fields_ = [("cycles", C.c_int), ("residual", C.c_double), ("hist_len", C.c_int)]
+    _fields_ = [("cycles", C.c_int), ("residual", C.c_double), ("hist_len", C.c_int),
+                ("converged", C.c_int)]
 
 
 # the exported C ABI: name -> (restype, argtypes)
@@ -84,6 +85,8 @@ SIGNATURES = {
     "oll_residual": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
     "oll_robin": (C.c_int, [_vp, C.c_int]),
     "oll_boundary_operator": (C.c_int, [_vp, C.c_int]),
+    "oll_robin_box": (C.c_int, [_vp, C.c_int]),
+    "oll_operator": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "oll_fas": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
     "oll_solve": (C.c_int, [_vp, C.POINTER(Result), C.POINTER(C.c_double), C.c_int]),
     "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -367,6 +370,16 @@ class Hierarchy:
     def boundary_operator(self, level):
         _check(self.L.oll_boundary_operator(self.h, level))
 
+    def robin_box(self, ibox):
+        _check(self.L.oll_robin_box(self.h, ibox))
+
+    def operator(self, level, derivative=False):
+        """N(u) (or dN/du with derivative=True) into F_R on the level's
+        interior points; returns max |N(u)|."""
+        out = C.c_double()
+        _check(self.L.oll_operator(self.h, level, int(bool(derivative)), C.byref(out)))
+        return out.value
+
     def fas(self, level):
         out = C.c_double()
         _check(self.L.oll_fas(self.h, level, C.byref(out)))
@@ -525,7 +538,8 @@ class Engine:
 
     def kernel_launches(self):
         """per timed smoother launch: (algorithmic bytes, device ms, kind) arrays,
-        kind 0 for FP64 sweep colours and 1 for FP32 correction colours"""
+        kind = 2 * stencil order + (0 for FP64 sweep colours, 1 for FP32
+        correction colours)"""
         n = C.c_int()
         _check(self.L.oll_engine_kernel_launches(self.h, 0, None, None, None, C.byref(n)))
         b, ms = np.zeros(n.value), np.zeros(n.value)
