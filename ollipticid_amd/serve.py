@@ -102,8 +102,17 @@ def serve(probs, arrivals, hbm_budget, graphs=True, max_active=0, log=None, keep
             nxt += 1
         while waiting and (max_active <= 0 or active < max_active):
             i = waiting[0]
-            if not e.admit(probs[i], i):
-                break
+            try:
+                if not e.admit(probs[i], i):
+                    break
+            except oa.OllError as err:
+                # a problem the solver cannot set up (the reference's grid
+                # set-up fails on it too): answered at once with the error
+                waiting.pop(0)
+                now = time.perf_counter() - t0
+                out[i] = {"arrival": arrivals[i], "admitted": now, "done": now, "cycles": 0,
+                          "residual": float("inf"), "error": err.code, "refused": True}
+                continue
             waiting.pop(0)
             active += 1
             out[i] = {"arrival": arrivals[i], "admitted": time.perf_counter() - t0}
@@ -139,12 +148,23 @@ def serve(probs, arrivals, hbm_budget, graphs=True, max_active=0, log=None, keep
 
 
 def summarize(out, wall, extra, tol=1e-10):
-    lat = np.array([r["done"] - r["arrival"] for r in out])
-    return {"requests": len(out), "wall_s": wall, "solves_per_s": len(out) / wall,
-            "p50_ms": 1e3 * float(np.percentile(lat, 50)),
-            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-            "mean_cycles": float(np.mean([r["cycles"] for r in out])),
-            "reached_tol": int(sum(1 for r in out if r["residual"] <= tol)), **extra}
+    """goodput: requests that reached tol per second of wall time, with the
+    latency percentiles over those; the completed figures count every
+    request answered (reached tol, stopped above it, failed or refused)"""
+    good = [r for r in out if "error" not in r and r["residual"] <= tol]
+    lat_good = np.array([r["done"] - r["arrival"] for r in good])
+    lat_all = np.array([r["done"] - r["arrival"] for r in out])
+    pct = lambda a, q: 1e3 * float(np.percentile(a, q)) if len(a) else None  # noqa: E731
+    return {"requests": len(out), "wall_s": wall,
+            "goodput_per_s": len(good) / wall, "reached_tol": len(good),
+            "p50_ms": pct(lat_good, 50), "p99_ms": pct(lat_good, 99),
+            "completed_per_s": len(out) / wall,
+            "completed_p50_ms": pct(lat_all, 50), "completed_p99_ms": pct(lat_all, 99),
+            "above_tol": sum(1 for r in out if "error" not in r and r["residual"] > tol),
+            "failed": sum(1 for r in out if "error" in r and not r.get("refused")),
+            "refused": sum(1 for r in out if r.get("refused")),
+            "mean_cycles": float(np.mean([r["cycles"] for r in out if not r.get("refused")])),
+            **extra}
 
 
 def u_fields(h):

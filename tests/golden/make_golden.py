@@ -62,13 +62,14 @@ CASES = {
     "cfg2_conv": (CFG2 + ["-s", "1e-16"], 4),
     # the binary of cfg2 at 8th order (cfg3's shape, N=129 reduced to 25)
     # and the three punctures with the scalar-field shell at 6th order
-    # (cfg5's shape, N=161 reduced to 25), both with every level: two
-    # post-sweeps per level keep the finest level's norm from reaching
-    # round-off before the coarser levels have converged, so the stopping
-    # rule ends the runs with every level converged (the discrete solution
-    # does not depend on the sweep counts)
-    "cfg3r_o8": (CFG2[:-10] + ["-L", "1000", "-N", "25", "-l", "9", "-O", "8", "-nu2", "2",
-                               "-s", "1e-16"], 2),
+    # (cfg5's shape, N=161 reduced to 25), and the binary itself, each with
+    # every level converged: few sweeps per level keep the finest level's
+    # norm from reaching round-off before the coarser levels have converged,
+    # so the stopping rule ends the runs with every level converged (the
+    # discrete solution does not depend on the sweep counts)
+    "cfg3r_o8": (CFG2[:-10] + ["-L", "1000", "-N", "25", "-l", "9", "-O", "8", "-nu1", "1",
+                               "-nu2", "3", "-s", "1e-16"], 2),
+    "cfg2_allconv": (CFG2[:-2] + ["-nu1", "1", "-nu2", "3", "-s", "1e-16"], 4),
     "cfg5r_o6": (SF3 + ["-L", "256", "-N", "25", "-l", "10", "-O", "6", "-nu2", "2",
                         "-s", "1e-16"], 2),
     # small cases covering orders 4/6/8, off-lattice punctures, spins,
