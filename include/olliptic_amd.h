@@ -234,6 +234,15 @@ int oll_solve(oll_hier *h, oll_result *res, double *hist, int hist_cap);
 int oll_puncture_approx(const oll_hier *h, double *u_at_p);
 /* elliptic.cc:2447-2611 Compute_ADM_Mass: per puncture and per box. */
 int oll_adm_mass(oll_hier *h, double *puncture, double *surface);
+/* The ADM masses of oll_adm_mass without allocating device memory (no
+ * reference counterpart: for callers that cannot allocate, such as code
+ * under stream capture or serving loops that forbid hipMalloc).  The
+ * device staging lives in a fixed 5.2 MB per-GPU scratch array that holds
+ * the surface samples of at most 64 boxes: a hierarchy with more boxes
+ * (and surface != NULL) is refused with OLL_EINVAL.  Every call on one GPU
+ * reuses that array, so calls that overlap on one GPU race; results equal
+ * oll_adm_mass's bit for bit. */
+int oll_adm_mass_noalloc(oll_hier *h, double *puncture, double *surface);
 /* ffunction::Interpol_In with LAGRANGE of order ord (interpolation.cc:189-303,
  * set by ffunction::Set_Interpol, interpolation.cc:32-60): field at npts
  * points, point q = (xyz[3q], xyz[3q+1], xyz[3q+2]) taken from box box[q]

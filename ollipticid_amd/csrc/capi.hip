@@ -9,6 +9,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -40,6 +41,38 @@ int fail(int code, const std::string &msg) {
     if (rc_ != OLL_OK) return fail(rc_, e_); \
   } while (0)
 
+}  // namespace
+
+// Device scratch of oll_adm_mass_noalloc: the sphere samples of up to
+// kScratchBoxes boxes (kAdmNt^2 doubles each), the box centres and the
+// puncture points and values, 5.2 MB in all.  Each GPU the library runs on
+// holds its own copy until the process ends; every call on a GPU reuses it,
+// so calls that overlap on one GPU race (the entry point's contract).  In
+// host code `scratch` only names the symbol: its device address on the
+// current GPU comes from hipGetSymbolAddress (scratch_address), looked up
+// once per device.
+constexpr int kScratchBoxes = 64;
+constexpr size_t kScratchSph = size_t(kScratchBoxes) * kAdmNt * kAdmNt;
+constexpr size_t kScratchDoubles = kScratchSph + 3 * kScratchBoxes + 4 * OLL_MAX_BH + OLL_MAX_BH;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch;  // device id -> address of `scratch` there
+
+int scratch_address(double **out) {
+  int dev = 0;
+  CAPI_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it == g_scratch.end()) {
+    double *p = nullptr;
+    CAPI_CHECK(hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch)));
+    it = g_scratch.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return OLL_OK;
+}
 }  // namespace
 
 struct oll_hier {
@@ -414,6 +447,28 @@ int oll_adm_mass(oll_hier *h, double *puncture, double *surface) {
   if (!h) return fail(OLL_EINVAL, "null handle");
   std::string err;
   const int rc = adm_mass(h->H, h->stream, puncture, surface, err);
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
+int oll_adm_mass_noalloc(oll_hier *h, double *puncture, double *surface) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  const int nb = int(h->H.hbox.size());
+  if (surface && nb > kScratchBoxes)
+    return fail(OLL_EINVAL, "oll_adm_mass_noalloc: " + std::to_string(nb) +
+                                " boxes; the device scratch holds the surface samples of at most " +
+                                std::to_string(kScratchBoxes));
+  if (h->H.prob.num_bh > OLL_MAX_BH) return fail(OLL_EINVAL, "too many punctures");
+  double *base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  AdmBuffers d;
+  d.sph = base;
+  d.ctr = base + kScratchSph;
+  d.pxyz = d.ctr + 3 * kScratchBoxes;
+  d.up = d.pxyz + 3 * OLL_MAX_BH;
+  d.pbox = reinterpret_cast<int *>(d.up + OLL_MAX_BH);
+  std::string err;
+  rc = adm_mass_with(h->H, h->stream, d, puncture, surface, err);
   return rc ? fail(rc, err) : OLL_OK;
 }
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     "oll_fas": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
     "oll_solve": (C.c_int, [_vp, C.POINTER(Result), C.POINTER(C.c_double), C.c_int]),
     "oll_adm_mass": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "oll_adm_mass_noalloc": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oll_puncture_approx": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "oll_interp": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                              C.c_int, C.POINTER(C.c_double)]),
@@ -402,6 +403,15 @@ class Hierarchy:
         p = (C.c_double * nbh)()
         s = (C.c_double * self.nbox)()
         _check(self.L.oll_adm_mass(self.h, p, s))
+        return list(p), list(s)
+
+    def adm_mass_noalloc(self):
+        """adm_mass through oll_adm_mass_noalloc: no device allocation (the
+        per-GPU scratch array of the library)."""
+        nbh = self.problem.num_bh
+        p = (C.c_double * nbh)()
+        s = (C.c_double * self.nbox)()
+        _check(self.L.oll_adm_mass_noalloc(self.h, p, s))
         return list(p), list(s)
 
     def puncture_approx(self):
