@@ -15,6 +15,13 @@ binding applies when the library loads:
 
     python -m ollipticid_amd.autotune [--batch 16] [--cycles 2]
 
+For FP64 it also times the fused double-buffered sweeps (two colour passes
+per launch on tiles of ty x tz interior rows) for each order with every
+candidate tile shape and workgroup size, in the same two regimes, against
+the chosen colour-by-colour size: the best shape per order goes into the
+table, and the fused sweeps are switched on only when they beat the colour
+passes in both regimes by MIN_GAIN at every order measured.
+
 Prints one JSON line per measurement and the chosen table.
 """
 import argparse
@@ -32,6 +39,7 @@ BINARY = PUNC + ["-NumBH", "2", "-bhmp1", "0.487209", "-bhy1", "5.5",
                  "-bhpx2", "0.0901099", "-bhpy2", "0.000703975",
                  "-L", "1000", "-N", "65", "-l", "9", "-nu2", "20", "-s", "1e-11"]
 CANDIDATES = (64, 128, 256, 512)
+FUSED_CANDIDATES = ((4, 4, 256), (4, 8, 256), (8, 8, 256), (8, 8, 512), (16, 16, 512))
 DEFAULT = 256
 MIN_GAIN = 1.02
 
@@ -90,15 +98,48 @@ def main():
             table.setdefault(pname, {})[str(order)] = best
             measured.setdefault(pname, {})[str(order)] = {
                 str(t): {"batch": round(r[0], 1), "alone": round(r[1], 1)} for t, r in rates.items()}
+    fused_shape, fused_measured, fused_gain = {}, {}, {}
+    if "fp64" in args.precisions.split(","):
+        prec = oa.PRECISION_NAMES["fp64"]
+        for order in (int(o) for o in args.orders.split(",")):
+            base = measured["fp64"][str(order)][str(table["fp64"][str(order)])]
+            rates = {}
+            oa.set_fused_sweeps(True)
+            try:
+                for shape in FUSED_CANDIDATES:
+                    oa.set_fused_shape(order, *shape)
+                    rb, _ = measure(oa, order, prec, args.batch, args.cycles)
+                    rs, _ = measure(oa, order, prec, 1, args.cycles)
+                    rates[shape] = (rb, rs)
+                    print(json.dumps({"precision": "fp64", "order": order, "fused_shape": shape,
+                                      "batch_GBs": round(rb, 1), "alone_GBs": round(rs, 1)}),
+                          flush=True)
+            finally:
+                oa.set_fused_sweeps(False)
+            gain = {t: min(r[0] / base["batch"], r[1] / base["alone"]) for t, r in rates.items()}
+            best = max(gain, key=gain.get)
+            oa.set_fused_shape(order, *best)
+            fused_shape[str(order)] = list(best)
+            fused_gain[str(order)] = round(gain[best], 3)
+            fused_measured[str(order)] = {"%dx%d/%d" % t: {"batch": round(r[0], 1),
+                                                             "alone": round(r[1], 1)}
+                                          for t, r in rates.items()}
+    fused_on = bool(fused_gain) and min(fused_gain.values()) >= MIN_GAIN
+    oa.set_fused_sweeps(fused_on)
     out = {"sweep_threads": table,
            "measured_colour_pass_GBs": measured,
+           "fused_sweeps": fused_on,
+           "fused_shape": fused_shape,
+           "fused_gain_vs_colour_passes": fused_gain,
+           "measured_fused_GBs": fused_measured,
            "workload": "equal-mass binary N=65 l=9 nu2=20 at each order, batch of %d and alone, "
                        "%d V-cycles" % (args.batch, args.cycles),
            "rule": "a size replaces %d only when faster in both regimes by %.0f%% or more"
                    % (DEFAULT, 100 * (MIN_GAIN - 1))}
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
-    print(json.dumps({"sweep_threads": table}), flush=True)
+    print(json.dumps({"sweep_threads": table, "fused_sweeps": fused_on, "fused_shape": fused_shape}),
+          flush=True)
 
 
 if __name__ == "__main__":

@@ -430,6 +430,117 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__res
   wave_norm(Bk.norm, rmax);
 }
 
+// ------------------------------------------------------------------
+// fused colour passes.  A sweep reads u from field X and writes every
+// updated point to field Y (X and Y are u and r, swapped from sweep to
+// sweep; the ghost shell is copied into r first, k_copy_shell): within a
+// sweep, a point's neighbours of colours already updated are read from Y,
+// the others and the point itself from X, which no launch of the sweep
+// writes.  One launch does two consecutive colour passes (pass indices Q
+// and Q + 1) on a tile of interior rows: the first pass also on the rows
+// the second pass reads (one row and plane below the tile, R above), the
+// second on the tile alone.  Every launch that computes a first-pass point
+// computes it from the same X and Y values, so tiles that overlap write
+// identical values, and each workgroup reads back only what it wrote itself
+// (after a barrier).  Each point's arithmetic is k_smooth's: the sweeps
+// give the same bits as the colour passes one launch at a time.
+// ------------------------------------------------------------------
+
+template <int ORD, int NC, int PI, int M>
+__device__ __forceinline__ void star_terms(const void *X, const void *Y, unsigned o,
+                                           const Nbr<NC> &n, double *sx, double *sy, double *sz) {
+  if constexpr (M <= ORD / 2) {
+    const void *bp = nbr_in_y(NC, PI, M) ? Y : X;
+    const void *bm = nbr_in_y(NC, PI, -M) ? Y : X;
+    sx[M - 1] = ld<double>(bp, o + n.xp[M - 1]) + ld<double>(bm, o + n.xm[M - 1]);
+    sy[M - 1] = ld<double>(bp, o + n.yp[M - 1]) + ld<double>(bm, o + n.ym[M - 1]);
+    sz[M - 1] = ld<double>(bp, o + n.zp[M - 1]) + ld<double>(bm, o + n.zm[M - 1]);
+    star_terms<ORD, NC, PI, M + 1>(X, Y, o, n, sx, sy, sz);
+  }
+}
+
+// one point of pass PI: reads X and Y as above, writes the new u to Y;
+// the arithmetic of k_smooth / point_op / lapb, operation for operation
+template <int ORD, int EQ, int PI>
+__device__ __forceinline__ double fused_point(ConstBox &B, const void *X, void *Y, int c, int i,
+                                              int j, int k) {
+  constexpr int R = ORD / 2;
+  constexpr int NC = R + 1;
+  const int s = i - (i / NC) * NC;
+  const unsigned p = unsigned(c) * unsigned(B.cs) + unsigned(k) * unsigned(B.ps) +
+                     unsigned(j) * unsigned(B.rs) + unsigned(i / NC);
+  const unsigned o = p * 8u;
+  const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
+  const double u0 = ld<double>(X, o);
+  double sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, sz[4] = {0, 0, 0, 0};
+  star_terms<ORD, NC, PI, 1>(X, Y, o, nb, sx, sy, sz);
+  const double lu = star_sum<ORD, double>(star<ORD, double>(u0, sx[0], sx[1], sx[2], sx[3]),
+                                          star<ORD, double>(u0, sy[0], sy[1], sy[2], sy[3]),
+                                          star<ORD, double>(u0, sz[0], sz[1], sz[2], sz[3]),
+                                          B.idx2[0], B.idx2[1], B.idx2[2]);
+  double N, dN;
+  nonlinear<EQ>(lu, B.dulap, ld<double>(B.f[F_B], o), ld<double>(B.f[F_C], o) + u0,
+                EQ == 1 ? ld<double>(B.f[F_D], o) : 0.0, EQ == 1 ? ld<double>(B.f[F_E], o) : 0.0,
+                N, dN);
+  const double res = N - ld<double>(B.f[F_RHO], o);
+  const double un = u0 - res / dN;
+  st<double>(Y, o, un);
+  double r = fabs(res);
+  if (!(un == un)) r = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
+  return r;
+}
+
+// pass PI over the first nrows rows of the tile's row list, the flattened
+// (row, colour point) index strided over the workgroup; returns the max
+// |residual| over the tile's own rows
+template <int ORD, int EQ, int PI>
+__device__ __forceinline__ double fused_pass(const TileBlock &T, ConstBox &B, const void *X, void *Y,
+                                             int nrows, int yb, int ya, int zb) {
+  constexpr int R = ORD / 2;
+  constexpr int NC = R + 1;
+  const int P = (B.n[0] - 2 * R + NC - 1) / NC;  // colour points per interior row, at most
+  const int nt = T.ty * T.tz;
+  const int c = B.flip ? NC - 1 - PI : PI;
+  double rmax = 0.0;
+  for (int f = int(threadIdx.x); f < nrows * P; f += int(blockDim.x)) {
+    const int r = f / P, t = f - r * P;
+    int j, k;
+    tile_row(T, r, yb, ya, zb, j, k);
+    const int i = colour_first(c, j, k, R, NC) + t * NC;
+    if (i >= B.n[0] - R) continue;
+    const double res = fused_point<ORD, EQ, PI>(B, X, Y, c, i, j, k);
+    if (r < nt) rmax = fmax(rmax, res);
+  }
+  return rmax;
+}
+
+// passes Q and Q + 1 (PAIR) or pass Q alone of one sweep on every listed
+// tile; parity 0 reads X = u and writes Y = r, parity 1 the reverse
+template <int ORD, int EQ, int Q, int PAIR>
+__global__ __launch_bounds__(kFusedThreads) void k_smooth_fused(const TileBlock *__restrict__ blocks,
+                                                              int parity) {
+  constexpr int R = ORD / 2;
+  const TileBlock T = blocks[xcd_block()];
+  ConstBox &B = cbox(T.box);
+  const void *X = parity ? B.f[F_R] : B.f[F_U];
+  void *Y = parity ? B.f[F_U] : B.f[F_R];
+  const int nt = T.ty * T.tz;
+  double rmax;
+  if constexpr (PAIR) {
+    int yb, ya, zb, za;
+    const int nn[3] = {B.n[0], B.n[1], B.n[2]};
+    tile_halo(T, nn, R, yb, ya, zb, za);
+    const int nrows = nt + (yb + ya) * T.tz + (zb + za) * T.ty;
+    rmax = fused_pass<ORD, EQ, Q>(T, B, X, Y, nrows, yb, ya, zb);
+    // the second pass reads the first pass's values this workgroup wrote
+    __syncthreads();
+    rmax = fmax(rmax, fused_pass<ORD, EQ, Q + 1>(T, B, X, Y, nt, 0, 0, 0));
+  } else {
+    rmax = fused_pass<ORD, EQ, Q>(T, B, X, Y, nt, 0, 0, 0);
+  }
+  wave_norm(T.norm, rmax);
+}
+
 // operator on the interior: mode RES_OPERATOR r = N(u), RES_DEFECT
 // w = rho - N(u), RES_DERIVATIVE r = dN/du_p; the norm is max |N(u)|
 template <int ORD, int EQ>
@@ -1181,6 +1292,18 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
   }
 }
 
+// r = u on the ghost shell of the listed boxes (the fused sweeps read
+// ghost values from either field)
+__global__ __launch_bounds__(kPtThreads) void k_copy_shell(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &B = cbox(Bk.box);
+  if (threadIdx.x >= Bk.nq) return;
+  int i, j, k;
+  shell_point(B, Bk.q0 + threadIdx.x, i, j, k);
+  const unsigned o = boff(B, i, j, k);
+  st<double>(B.f[F_R], o, ld<double>(B.f[F_U], o));
+}
+
 // ------------------------------------------------------------------
 // coarsest level: {Robin, one multi-colour sweep} until the sweep norm is
 // <= eps or stops changing (elliptic.cc:1782-1816); one block per
@@ -1847,6 +1970,59 @@ bool set_sweep_threads(int mixed, int order, int threads) {
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                    int eq, int colour, int threads, hipEvent_t e0, hipEvent_t e1) {
   OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour, threads, e0, e1);
+}
+
+// tile shape of the fused passes, [order / 2 - 1]
+static FusedShape g_fused_shape[4] = {{8, 8, 256}, {8, 8, 256}, {8, 8, 256}, {8, 8, 256}};
+
+FusedShape fused_shape(int order) { return g_fused_shape[std::min(3, std::max(0, order / 2 - 1))]; }
+
+bool set_fused_shape(int order, int ty, int tz, int threads) {
+  if (!(order == 2 || order == 4 || order == 6 || order == 8)) return false;
+  if (ty < 1 || tz < 1 || ty > 64 || tz > 64) return false;
+  if (threads < 64 || threads > kFusedThreads || threads % 64 != 0) return false;
+  g_fused_shape[order / 2 - 1] = FusedShape{ty, tz, threads};
+  return true;
+}
+
+template <int ORD, int EQ, int Q, int PAIR>
+void fused_impl(hipStream_t s, const TileBlock *blocks, int nb, int parity, int threads,
+                hipEvent_t e0, hipEvent_t e1) {
+  if (nb == 0) return;
+  if (e0)  // timed (never under graph capture): the runtime stamps e0, e1
+    hipExtLaunchKernelGGL((k_smooth_fused<ORD, EQ, Q, PAIR>), dim3(nb), dim3(threads), 0, s, e0,
+                          e1, 0, blocks, parity);
+  else
+    hipLaunchKernelGGL((k_smooth_fused<ORD, EQ, Q, PAIR>), dim3(nb), dim3(threads), 0, s, blocks,
+                       parity);
+}
+
+template <int ORD, int EQ>
+void fused_order(hipStream_t s, const TileBlock *blocks, int nb, int q, int pair, int parity,
+                 int threads, hipEvent_t e0, hipEvent_t e1) {
+  constexpr int NC = ORD / 2 + 1;
+  // pairs (0, 1), (2, 3); a single pass for the last colour when NC is odd
+  if (pair && q == 0) return fused_impl<ORD, EQ, 0, 1>(s, blocks, nb, parity, threads, e0, e1);
+  if constexpr (NC >= 4)
+    if (pair && q == 2) return fused_impl<ORD, EQ, 2, 1>(s, blocks, nb, parity, threads, e0, e1);
+  if constexpr (NC % 2 == 1)
+    if (!pair && q == NC - 1)
+      return fused_impl<ORD, EQ, NC - 1, 0>(s, blocks, nb, parity, threads, e0, e1);
+}
+
+void launch_smooth_fused(hipStream_t s, const TileBlock *blocks, int nblocks, int order, int eq,
+                         int q, int pair, int parity, int threads, hipEvent_t e0, hipEvent_t e1) {
+  switch (eq) {
+    case 1: OLL_DISPATCH_ORDER_EQ(order, 1, fused_order, s, blocks, nblocks, q, pair, parity, threads, e0, e1) break;
+    case 2: OLL_DISPATCH_ORDER_EQ(order, 2, fused_order, s, blocks, nblocks, q, pair, parity, threads, e0, e1) break;
+    case 3: OLL_DISPATCH_ORDER_EQ(order, 3, fused_order, s, blocks, nblocks, q, pair, parity, threads, e0, e1) break;
+    default: OLL_DISPATCH_ORDER_EQ(order, 0, fused_order, s, blocks, nblocks, q, pair, parity, threads, e0, e1) break;
+  }
+}
+
+void launch_copy_shell(hipStream_t s, const PtBlock *blocks, int nblocks) {
+  if (nblocks == 0) return;
+  hipLaunchKernelGGL(k_copy_shell, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
 }
 
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,

@@ -192,6 +192,83 @@ constexpr int kMaxRowThreads = 512;
 int sweep_threads(int mixed, int order);
 bool set_sweep_threads(int mixed, int order, int threads);
 
+// fused colour passes (k_smooth_fused): one workgroup per tile of ty x tz
+// interior rows (full rows along x) starting at row (j0, k0) of a box
+struct TileBlock {
+  const DevBox *box;
+  unsigned long long *norm;
+  int j0, k0;
+  int ty, tz;
+};
+constexpr int kFusedThreads = 512;  // at most; the tile-shape table picks 64..512
+
+// Fused passes (k_smooth_fused): a neighbour m steps away (m < 0: below)
+// of a point of pass index pi has pass index (pi + m) mod nc (the colours
+// along every axis follow the pass order; two-colour boxes whose order is
+// flipped alike); it was updated earlier in the sweep, and lives in the
+// written field Y, when that index precedes pi.
+__host__ __device__ constexpr bool nbr_in_y(int nc, int pi, int m) {
+  return (((pi + m) % nc) + nc) % nc < pi;
+}
+// i of the first interior point of colour c in row (j, k)
+__host__ __device__ inline int colour_first(int c, int j, int k, int R, int nc) {
+  return R + int(unsigned(c + nc * 3 * kMaxAxisPoints - j - k - R) % unsigned(nc));
+}
+// interior row r of a tile's row list: the tile's rows (y fastest), then
+// the rows its second pass reads outside it: yb rows below (0 or 1) and ya
+// above (up to R) along y for the tile's planes, zb planes below and the
+// rest above along z for the tile's rows, those inside the interior
+__host__ __device__ inline void tile_row(const TileBlock &T, int r, int yb, int ya, int zb, int &j,
+                                         int &k) {
+  const int nt = T.ty * T.tz;
+  if (r < nt) {
+    const int kk = r / T.ty;
+    j = T.j0 + (r - kk * T.ty);
+    k = T.k0 + kk;
+    return;
+  }
+  r -= nt;
+  if (r < yb * T.tz) {
+    j = T.j0 - 1;
+    k = T.k0 + r;
+    return;
+  }
+  r -= yb * T.tz;
+  if (r < ya * T.tz) {
+    const int l = r / T.tz;
+    j = T.j0 + T.ty + l;
+    k = T.k0 + (r - l * T.tz);
+    return;
+  }
+  r -= ya * T.tz;
+  if (r < zb * T.ty) {
+    j = T.j0 + r;
+    k = T.k0 - 1;
+    return;
+  }
+  r -= zb * T.ty;
+  const int l = r / T.ty;
+  k = T.k0 + T.tz + l;
+  j = T.j0 + (r - l * T.ty);
+}
+// halo counts of a tile of box B with ghost width R (rows inside the interior)
+__host__ __device__ inline void tile_halo(const TileBlock &T, const int n[3], int R, int &yb, int &ya,
+                                          int &zb, int &za) {
+  yb = T.j0 - 1 >= R ? 1 : 0;
+  ya = n[1] - R - (T.j0 + T.ty);
+  ya = ya < R ? ya : R;
+  zb = T.k0 - 1 >= R ? 1 : 0;
+  za = n[2] - R - (T.k0 + T.tz);
+  za = za < R ? za : R;
+}
+// tile shape and threads per workgroup of the fused passes per order (the
+// autotuner's knobs); results do not depend on them
+struct FusedShape {
+  int ty, tz, threads;
+};
+FusedShape fused_shape(int order);
+bool set_fused_shape(int order, int ty, int tz, int threads);
+
 // point-parallel launches (transfers): one workgroup per PtBlock, which
 // owns target points q0 .. q0+nq-1 (nq <= kPtThreads) of one box, either
 // its lexicographic points (i fastest) or its ghost-shell points in shell
@@ -311,6 +388,13 @@ void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_r
 enum ResidualMode { RES_OPERATOR = 0, RES_DEFECT = 1, RES_DERIVATIVE = 2 };
 void launch_residual(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                      int eq, int mode);
+// passes q and q+1 (pair) or pass q alone of one sweep, double-buffered:
+// parity 0 reads u and writes r, parity 1 the reverse (eq 0-3)
+void launch_smooth_fused(hipStream_t s, const TileBlock *blocks, int nblocks, int order, int eq,
+                         int q, int pair, int parity, int threads, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr);
+// r = u on the ghost shell (PtBlocks over shell points)
+void launch_copy_shell(hipStream_t s, const PtBlock *blocks, int nblocks);
 // mixed precision: FP64 defect -> FP32 (r, q, e = 0), FP32 colour pass of
 // the linearised correction equation, FP64 update u += e
 void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,

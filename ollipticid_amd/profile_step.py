@@ -32,9 +32,14 @@ def main():
     ap.add_argument("--cycles", type=int, default=2, help="V-cycles per request (0: to convergence)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also copy this many MiB device to device (counter calibration)")
+    ap.add_argument("--fused", action="store_true",
+                    help="fused double-buffered FP64 sweeps instead of colour by colour")
     args = ap.parse_args()
 
     import ollipticid_amd as oa
+    from ollipticid_amd import native
+
+    native.set_fused_sweeps(args.fused)
 
     p = oa.problem_from_argv(WORKLOADS[args.workload])
     p.precision = oa.PREC_MIXED if args.precision == "mixed" else oa.PREC_FP64
@@ -52,7 +57,8 @@ def main():
     done = e.pop_done()
     e.close()
     out = {"workload": args.workload, "argv": " ".join(WORKLOADS[args.workload]),
-           "batch": args.batch, "precision": args.precision, "cycles": cycles,
+           "batch": args.batch, "precision": args.precision, "fused": args.fused,
+           "cycles": cycles,
            "requests_done": len(done), "smoother_launches": launches, "smoother_ms": ms,
            "smoother_bytes": nbytes}
     if args.copy_mib:

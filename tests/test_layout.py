@@ -20,3 +20,18 @@ def test_layout_arithmetic(tmp_path):
                    check=True, capture_output=True, timeout=300)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
+@pytest.mark.skipif(HIPCC is None, reason="hipcc not available")
+def test_fused_sweep_indexing(tmp_path):
+    """tests/fused_check.cpp: the fused double-buffered colour passes'
+    tile rows, halo rows, colour slots and field choice per neighbour,
+    replayed tile by tile in shuffled order, stay in bounds, read nothing
+    another workgroup of the same launch writes, and give the bits of
+    colour-by-colour sweeps."""
+    exe = tmp_path / "fused_check"
+    subprocess.run([HIPCC, "-O1", "-std=c++17", "-I" + os.path.join(os.path.dirname(HERE), "include"),
+                    "-o", str(exe), os.path.join(HERE, "fused_check.cpp")],
+                   check=True, capture_output=True, timeout=300)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
