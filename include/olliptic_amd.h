@@ -159,6 +159,10 @@ typedef struct oll_box_desc {
   double end[3];        /* coordinate of the last point as the reference's domain keeps
                            it (domain.cc:326-351); may differ from origin + (n-1) d in
                            the last bit */
+  int colour_order;     /* two-colour boxes: 1 when the multi-colour sweeps take
+                           colour 1 first (the box's points that coincide with nodes of
+                           its parent, the coarser box holding its centre, have odd
+                           i + j + k), else 0 (also off the parent's lattice) */
 } oll_box_desc;
 
 typedef struct oll_result {
@@ -326,6 +330,20 @@ int oll_engine_kernel_launches(oll_engine *e, int cap, double *bytes, double *ms
  * on it.  The setter returns OLL_EINVAL for other values. */
 int oll_set_sweep_threads(int precision, int order, int threads);
 int oll_get_sweep_threads(int precision, int order);
+/* Fused FP64 sweeps (no reference counterpart): on, FP64 smoothing phases
+ * run their sweeps in pairs as double-buffered launches that each do two
+ * colour passes on a tile of ty x tz interior rows (u and the r field swap
+ * roles from sweep to sweep), an odd last sweep colour by colour; off (the
+ * default), every sweep colour by colour, which measures faster on MI355X.
+ * The results are the same bits either way.  Process-wide; schedules
+ * built afterwards use it. */
+int oll_set_fused_sweeps(int on);
+int oll_get_fused_sweeps(void);
+/* tile of the fused sweeps for one order: ty x tz interior rows (1..64
+ * each) and threads per workgroup (a multiple of 64, 64..512; default
+ * 8 x 8 rows, 256 threads); OLL_EINVAL for other values */
+int oll_set_fused_shape(int order, int ty, int tz, int threads);
+int oll_get_fused_shape(int order, int *ty, int *tz, int *threads);
 
 #ifdef __cplusplus
 }

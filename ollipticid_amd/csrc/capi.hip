@@ -187,7 +187,8 @@ int oll_level_boxes(const oll_hier *h, int level, int *first, int *count) {
   return OLL_OK;
 }
 
-static void fill_desc(const Box &b, oll_box_desc *out) {
+static void fill_desc(const Box &b, int colour_order, oll_box_desc *out) {
+  out->colour_order = colour_order;
   out->level = b.level;
   out->order = b.order;
   out->ghost = b.R;
@@ -203,7 +204,7 @@ static void fill_desc(const Box &b, oll_box_desc *out) {
 int oll_box_info(const oll_hier *h, int ibox, oll_box_desc *out) {
   if (!h || !out) return fail(OLL_EINVAL, "null argument");
   if (ibox < 0 || ibox >= int(h->H.geo.boxes.size())) return fail(OLL_EINVAL, "box out of range");
-  fill_desc(h->H.geo.boxes[size_t(ibox)], out);
+  fill_desc(h->H.geo.boxes[size_t(ibox)], h->H.hbox[size_t(ibox)].flip, out);
   return OLL_OK;
 }
 
@@ -223,7 +224,10 @@ int oll_problem_boxes(const oll_problem *p, oll_box_desc *out, int cap, int *nbo
   if (!err.empty()) return fail(err.find("enough") != std::string::npos ? OLL_EGRID : OLL_EINVAL, err);
   *nbox = int(g.boxes.size());
   if (mgrid) *mgrid = g.mgrid;
-  for (int i = 0; i < *nbox && i < cap && out; ++i) fill_desc(g.boxes[size_t(i)], &out[i]);
+  for (int i = 0; i < *nbox && i < cap && out; ++i) {
+    const Box &b = g.boxes[size_t(i)];
+    fill_desc(b, colour_flip(g, b, colours_of(b.order, p->eq)), &out[i]);
+  }
   return OLL_OK;
 }
 
@@ -760,6 +764,30 @@ int oll_set_sweep_threads(int precision, int order, int threads) {
 
 int oll_get_sweep_threads(int precision, int order) {
   return sweep_threads(precision == OLL_PREC_MIXED, order);
+}
+
+int oll_set_fused_sweeps(int on) {
+  set_fused_sweeps(on != 0);
+  return OLL_OK;
+}
+
+int oll_get_fused_sweeps(void) { return fused_sweeps() ? 1 : 0; }
+
+int oll_set_fused_shape(int order, int ty, int tz, int threads) {
+  if (!set_fused_shape(order, ty, tz, threads))
+    return fail(OLL_EINVAL,
+                "order must be 2, 4, 6 or 8, ty and tz 1..64 and threads a multiple of 64 in 64..512");
+  return OLL_OK;
+}
+
+int oll_get_fused_shape(int order, int *ty, int *tz, int *threads) {
+  if (!(order == 2 || order == 4 || order == 6 || order == 8))
+    return fail(OLL_EINVAL, "order must be 2, 4, 6 or 8");
+  const FusedShape fs = fused_shape(order);
+  if (ty) *ty = fs.ty;
+  if (tz) *tz = fs.tz;
+  if (threads) *threads = fs.threads;
+  return OLL_OK;
 }
 
 int oll_engine_profile(const oll_engine *e, double *build_ms, double *run_ms, double *admit_ms,

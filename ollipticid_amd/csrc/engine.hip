@@ -230,20 +230,7 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.dulap = dulap_of(b.order, d.idx2);
     d.ipn = ipn;
     set_layout(d, colours_of(b.order, p.eq));
-    // two-colour boxes: sweep first the colour of the points that coincide
-    // with the coarser level's (the first box of that level fixes the
-    // lattice offset); in ascending order the V-cycle of some deep nested
-    // hierarchies diverges
-    d.flip = 0;
-    if (d.nc == 2 && b.level > 0) {
-      const Box &C = g.boxes[size_t(g.level_first[size_t(b.level - 1)])];
-      int sum = 0;
-      for (int a = 0; a < 3; ++a) {
-        const long m = std::lround((b.lo[a] - C.lo[a]) / b.d[a]);
-        sum += int(((m % 2) + 2) % 2);
-      }
-      d.flip = sum & 1;
-    }
+    d.flip = colour_flip(g, b, d.nc);
     for (int f = 0; f < NFIELDS; ++f)
       d.f[f] = f < nf ? H.slab + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     for (int f = 0; f < NFIELDS32; ++f)
@@ -393,6 +380,45 @@ void hier_release(Hier &H) {
 // schedules
 // ---------------------------------------------------------------------
 
+// fused double-buffered sweeps (k_smooth_fused): off by default.  On
+// MI355X they run slower than the colour-by-colour passes at every stencil
+// order and tile shape measured (4th order, N=65: 1.1-3x; 6th order,
+// N=161: 1.5x): the first pass's recomputed halo rows (1 + R per tile side)
+// add a third to a half of the work, and the tiles' working sets exceed
+// the L2, so the second pass gains no reuse.
+static bool g_fused_sweeps = false;
+
+// Two-colour boxes sweep first the colour of their points that coincide
+// with nodes of the parent box (the box of the coarser level that holds
+// the box's centre): flip = 1 when those points have odd i + j + k.  A box
+// whose origin lies off the parent's lattice (an offset that is not a
+// whole number of fine spacings along some axis) has no coincident points
+// and sweeps in ascending order.
+int colour_flip(const Hierarchy &g, const Box &b, int nc) {
+  if (nc != 2 || b.level == 0) return 0;
+  const int f = g.level_first[size_t(b.level - 1)], n = g.level_count[size_t(b.level - 1)];
+  const Box *P = &g.boxes[size_t(f)];
+  for (int q = f; q < f + n; ++q) {
+    const Box &C = g.boxes[size_t(q)];
+    bool in = true;
+    for (int a = 0; a < 3; ++a) in = in && b.c[a] >= C.lo[a] && b.c[a] <= C.hi[a];
+    if (in) {
+      P = &C;
+      break;
+    }
+  }
+  int sum = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double t = (b.lo[a] - P->lo[a]) / b.d[a];
+    const double m = std::nearbyint(t);
+    if (std::fabs(t - m) > 1e-6) return 0;
+    sum += int(std::fmod(std::fabs(m), 2.0));
+  }
+  return sum & 1;
+}
+void set_fused_sweeps(bool on) { g_fused_sweeps = on; }
+bool fused_sweeps() { return g_fused_sweeps; }
+
 void Schedule::clear() {
   launches.clear();
   blob.clear();
@@ -439,6 +465,31 @@ struct Builder {
       const int rows = interior_rows(B) * (subrows ? B.nc : 1);
       for (int r0 = 0; r0 < rows; r0 += rpb)
         out.push_back(RowBlock{H.dbox + b, norm, r0, std::min(rpb, rows - r0)});
+    }
+  }
+
+  // tiles of the fused passes over the interior rows of every box of a
+  // level of one member
+  static void tile_blocks(const Hier &H, int level, unsigned long long *norm, int ty, int tz,
+                          std::vector<TileBlock> &out) {
+    const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
+    for (int b = f; b < f + n; ++b) {
+      const DevBox &B = H.hbox[size_t(b)];
+      for (int k0 = B.R; k0 < B.n[2] - B.R; k0 += tz)
+        for (int j0 = B.R; j0 < B.n[1] - B.R; j0 += ty)
+          out.push_back(TileBlock{H.dbox + b, norm, j0, k0, std::min(ty, B.n[1] - B.R - j0),
+                                  std::min(tz, B.n[2] - B.R - k0)});
+    }
+  }
+
+  // point blocks over the ghost shells of every box of a level of one member
+  static void shell_blocks(const Hier &H, int level, std::vector<PtBlock> &out) {
+    const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
+    for (int b = f; b < f + n; ++b) {
+      const long long np = shell_pts(H.hbox[size_t(b)]);
+      for (long long q0 = 0; q0 < np; q0 += kPtThreads)
+        out.push_back(PtBlock{H.dbox + b, nullptr, 0, 0, unsigned(q0),
+                              unsigned(std::min<long long>(kPtThreads, np - q0)), 0, 0, 0});
     }
   }
 
@@ -491,6 +542,41 @@ struct Builder {
                            : (eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : (eq == OLL_EQ_PUNCTURES_EM ? 88.0 : 40.0));
         for (int q = 0; q < nc; ++q) bytes[q] += per * H.colour_pts[size_t(l)][size_t(q)];
       }
+      // FP64 sweeps in pairs as fused double-buffered launches (an even
+      // count, so u ends in its own field); the rest one colour at a time
+      const bool fuse = fused_sweeps() && !mixed && eq != OLL_EQ_PUNCTURES_EM;
+      const int nfused = fuse ? nu - nu % 2 : 0;
+      if (nfused > 0) {
+        std::vector<TileBlock> tb;
+        std::vector<PtBlock> sh;
+        const FusedShape fs = fused_shape(order);
+        for (int h : g.second) {
+          const Hier &H = *M[size_t(h)];
+          const int l = lv[size_t(h)];
+          tile_blocks(H, l, slot(H, l), fs.ty, fs.tz, tb);
+          shell_blocks(H, l, sh);
+        }
+        Launch C{Launch::COPY_SHELL};
+        C.off = S.add(sh);
+        C.n = int(sh.size());
+        S.launches.push_back(C);
+        const size_t toff = S.add(tb);
+        for (int sw = 0; sw < nfused; ++sw)
+          for (int q = 0; q < nc; q += 2) {
+            Launch L{Launch::SMOOTH_FUSED};
+            L.order = order;
+            L.eq = eq;
+            L.colour = q;
+            L.pair = q + 1 < nc ? 1 : 0;
+            L.parity = sw & 1;
+            L.off = toff;
+            L.n = int(tb.size());
+            L.threads = fs.threads;
+            L.bytes = bytes[q] + (L.pair ? bytes[q + 1] : 0.0);
+            S.launches.push_back(L);
+          }
+      }
+      const int nu_single = nu - nfused;
       const size_t coff = S.add(cb);
       const size_t soff = mixed ? S.add(sb) : 0;
       auto whole = [&](Launch::Kind kind) {
@@ -503,7 +589,7 @@ struct Builder {
         S.launches.push_back(L);
       };
       if (mixed) whole(Launch::DEFECT32);
-      for (int s = 0; s < nu; ++s)
+      for (int s = 0; s < nu_single; ++s)
         for (int q = 0; q < nc; ++q) {
           Launch L{mixed ? Launch::SMOOTH32 : Launch::SMOOTH};
           L.order = order;
@@ -770,7 +856,8 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
     // timed smoother launches carry their events into the dispatch: the
     // runtime stamps them at the kernel's start and end
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (t && t->on && (L.kind == Launch::SMOOTH || L.kind == Launch::SMOOTH32)) {
+    if (t && t->on &&
+        (L.kind == Launch::SMOOTH || L.kind == Launch::SMOOTH32 || L.kind == Launch::SMOOTH_FUSED)) {
       e0 = timing_event(*t);
       e1 = e0 ? timing_event(*t) : nullptr;
       if (e0 && !e1) --t->used;  // keep the events paired
@@ -817,6 +904,13 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         break;
       case Launch::FILTER:
         launch_filter_defect(s, reinterpret_cast<const PtBlock *>(items), L.n);
+        break;
+      case Launch::SMOOTH_FUSED:
+        launch_smooth_fused(s, reinterpret_cast<const TileBlock *>(items), L.n, L.order, L.eq,
+                            L.colour, L.pair, L.parity, L.threads, e0, e1);
+        break;
+      case Launch::COPY_SHELL:
+        launch_copy_shell(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
       case Launch::GHOST:
         launch_ghost_fill(s, reinterpret_cast<const PtBlock *>(items), L.n, L.order);

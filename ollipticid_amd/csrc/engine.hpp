@@ -81,7 +81,7 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err);
 struct Launch {
   enum Kind {
     ZERO, ROBIN, SMOOTH, RESIDUAL, RESTRICT_U, RESTRICT_DEFECT, COARSE, PROLONG, GHOST,
-    DEFECT32, SMOOTH32, UPDATE32, FILTER
+    DEFECT32, SMOOTH32, UPDATE32, FILTER, SMOOTH_FUSED, COPY_SHELL
   };
   Kind kind;
   int order = 2, eq = 0, colour = 0, mode = 0;
@@ -91,6 +91,7 @@ struct Launch {
   int max_row = 0;    // row kernels: threads per row
   int threads = kRowThreads;  // colour passes: threads per workgroup
   int max_pts = 0, max_int = 0;
+  int pair = 0, parity = 0;  // SMOOTH_FUSED: two passes or one; 0 reads u, 1 reads r
   double bytes = 0;   // algorithmic bytes (smoother launches, FP64 or FP32)
 };
 
@@ -131,6 +132,14 @@ int build_vcycle(const std::vector<Hier *> &members, Schedule &S, std::string &e
 // API); box >= 0 restricts the Robin stages to that box of the level
 int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err, int box = -1);
 enum Stage { STAGE_SMOOTH = 0, STAGE_RESIDUAL, STAGE_ROBIN, STAGE_BOUNDARY_OP, STAGE_DERIVATIVE };
+
+// colour order of box b (0: ascending, 1: descending) for nc colours
+int colour_flip(const Hierarchy &g, const Box &b, int nc);
+// FP64 sweeps of the puncture, scalar-field, Poisson and Brill equations
+// as fused, double-buffered colour-pass pairs (default off); off issues
+// one launch per colour.  Process-wide; schedules built afterwards use it.
+void set_fused_sweeps(bool on);
+bool fused_sweeps();
 
 int upload_schedule(Schedule &S, hipStream_t s, std::string &err);
 // runs the schedule; with graphs = true it is captured as a graph on its

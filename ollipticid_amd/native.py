@@ -49,12 +49,13 @@ class Problem(C.Structure):
 class BoxDesc(C.Structure):
     _fields_ = [("level", C.c_int), ("order", C.c_int), ("ghost", C.c_int),
                 ("n", C.c_int * 3), ("origin", C.c_double * 3), ("d", C.c_double * 3),
-                ("centre", C.c_double * 3), ("end", C.c_double * 3)]
+                ("centre", C.c_double * 3), ("end", C.c_double * 3), ("colour_order", C.c_int)]
 
     def as_dict(self):
         return {"level": self.level, "order": self.order, "R": self.ghost,
                 "n": list(self.n), "origin": list(self.origin), "d": list(self.d),
-                "centre": list(self.centre), "end": list(self.end)}
+                "centre": list(self.centre), "end": list(self.end),
+                "colour_order": self.colour_order}
 
 
 class Result(C.Structure):
@@ -114,6 +115,11 @@ SIGNATURES = {
                                              C.POINTER(C.c_int)]),
     "oll_set_sweep_threads": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "oll_get_sweep_threads": (C.c_int, [C.c_int, C.c_int]),
+    "oll_set_fused_sweeps": (C.c_int, [C.c_int]),
+    "oll_get_fused_sweeps": (C.c_int, []),
+    "oll_set_fused_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "oll_get_fused_shape": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "oll_engine_profile": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_longlong)]),
@@ -161,16 +167,44 @@ def sweep_threads(precision, order):
     return load().oll_get_sweep_threads(precision, order)
 
 
+def set_fused_sweeps(on):
+    """FP64 sweeps in pairs as fused double-buffered launches, or colour by
+    colour (the default, faster on MI355X); the same bits either way."""
+    _check(load().oll_set_fused_sweeps(1 if on else 0))
+
+
+def fused_sweeps():
+    return bool(load().oll_get_fused_sweeps())
+
+
+def set_fused_shape(order, ty, tz, threads):
+    """Tile (ty x tz interior rows) and workgroup size of the fused sweeps
+    for one order; schedules built afterwards use it."""
+    _check(load().oll_set_fused_shape(order, ty, tz, threads))
+
+
+def fused_shape(order):
+    ty, tz, th = C.c_int(), C.c_int(), C.c_int()
+    _check(load().oll_get_fused_shape(order, C.byref(ty), C.byref(tz), C.byref(th)))
+    return ty.value, tz.value, th.value
+
+
 def apply_launch_shapes(path=SHAPES_PATH):
     """Applies the autotuner's table (ollipticid_amd/autotune.py writes it)
-    when the file exists; returns the applied {precision: {order: threads}}."""
+    when the file exists: colour-pass workgroup sizes, fused-sweep tile
+    shapes and the fused-sweep switch; returns the applied
+    {precision: {order: threads}}."""
     if not os.path.exists(path):
         return {}
     with open(path) as f:
-        table = json.load(f).get("sweep_threads", {})
+        full = json.load(f)
+    table = full.get("sweep_threads", {})
     for pname, per_order in table.items():
         for order, threads in per_order.items():
             set_sweep_threads(PRECISION_NAMES[pname], int(order), int(threads))
+    for order, (ty, tz, threads) in full.get("fused_shape", {}).items():
+        set_fused_shape(int(order), int(ty), int(tz), int(threads))
+    set_fused_sweeps(bool(full.get("fused_sweeps", False)))
     return table
 
 

@@ -679,18 +679,38 @@ static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega);
 double oracle_smooth(oracle_solver *s, int ibox, int nu) { return smooth_omega(s, ibox, nu, 1.0); }
 
 /* colour order of a two-colour box (the GPU's choice): when the box's
- * points that coincide with the coarser level's (first box of that level)
- * are of colour 1, its sweeps take colour 1 first */
+ * points that coincide with nodes of its parent (the coarser box holding
+ * its centre) are of colour 1, its sweeps take colour 1 first; a box off
+ * its parent's lattice sweeps in ascending order */
 static int colour_flip(const oracle_solver *s, const obox *B) {
   const int lev = B->D.level;
   if (lev == 0) return 0;
+  /* the parent: the coarser box holding the box's centre */
   const obox *C = &s->box[s->lvl_first[lev - 1]];
+  for (int q = 0; q < s->lvl_count[lev - 1]; q++) {
+    const obox *P = &s->box[s->lvl_first[lev - 1] + q];
+    int in = 1;
+    for (int a = 0; a < 3; a++) in = in && B->D.c[a] >= P->D.lo[a] && B->D.c[a] <= P->D.hi[a];
+    if (in) {
+      C = P;
+      break;
+    }
+  }
   int sum = 0;
   for (int a = 0; a < 3; a++) {
-    const long m = lround((dom_x(&B->D, a, 0) - dom_x(&C->D, a, 0)) / B->D.d[a]);
-    sum += (int)(((m % 2) + 2) % 2);
+    const double t = (dom_x(&B->D, a, 0) - dom_x(&C->D, a, 0)) / B->D.d[a];
+    const double m = nearbyint(t);
+    if (fabs(t - m) > 1e-6) return 0; /* off the parent's lattice: no coincident points */
+    sum += (int)fmod(fabs(m), 2.0);
   }
   return sum & 1;
+}
+
+int oracle_colour_flip(const oracle_solver *s, int ibox) {
+  if (ibox < 0 || ibox >= s->nbox) return -1;
+  const obox *B = &s->box[ibox];
+  const int ncol = (s->p.eq == ORACLE_PUNCTURES_EM && B->D.order >= 4) ? 5 : (int)B->D.b[0] + 1;
+  return ncol == 2 ? colour_flip(s, B) : 0;
 }
 
 static double smooth_omega(oracle_solver *s, int ibox, int nu, double omega) {

@@ -104,6 +104,9 @@ double *oracle_field(oracle_solver *s, int ibox, int field);
 
 /* Single operations on one box (ibox global index). */
 double oracle_smooth(oracle_solver *s, int ibox, int nu);
+/* 1 when a two-colour box's multi-colour sweeps take colour 1 first (the
+ * GPU's order), 0 otherwise; -1 for a box index out of range */
+int oracle_colour_flip(const oracle_solver *s, int ibox);
 double oracle_residual(oracle_solver *s, int ibox);
 void oracle_robin(oracle_solver *s, int ibox);
 void oracle_boundary_operator(oracle_solver *s, int ibox);

@@ -73,6 +73,8 @@ def lib():
         L.oracle_field.restype = C.POINTER(C.c_double)
         L.oracle_smooth.argtypes = [vp, C.c_int, C.c_int]
         L.oracle_smooth.restype = C.c_double
+        L.oracle_colour_flip.argtypes = [vp, C.c_int]
+        L.oracle_colour_flip.restype = C.c_int
         L.oracle_residual.argtypes = [vp, C.c_int]
         L.oracle_residual.restype = C.c_double
         L.oracle_robin.argtypes = [vp, C.c_int]
@@ -210,6 +212,10 @@ class Solver:
         if not ptr:
             raise ValueError("field %d is not held for this equation" % fid)
         return np.ctypeslib.as_array(ptr, shape=(n[2], n[1], n[0]))
+
+    def colour_flip(self, ib):
+        """1 when box ib's two-colour sweeps take colour 1 first, else 0."""
+        return self.L.oracle_colour_flip(self.h, ib)
 
     def smooth(self, ib, nu):
         return self.L.oracle_smooth(self.h, ib, nu)

@@ -8,6 +8,7 @@ import json
 import os
 import re
 
+import numpy as np
 import pytest
 
 import ollipticid_amd as oa
@@ -121,3 +122,28 @@ def test_sweep_threads_table():
                 oa.set_sweep_threads(prec, order, threads)
     finally:
         oa.set_sweep_threads(oa.PREC_MIXED, 6, before)
+
+
+def test_colour_order_matches_oracle():
+    """The host geometry's per-box colour order (oll_problem_boxes) is the
+    oracle's: a 2nd-order case with a box swept colour 1 first, and 24
+    requests of the served stream (2nd, 4th and 6th order; two-colour boxes
+    on and off their parent's lattice)."""
+    from oracle import oracle as O
+    from ollipticid_amd import serve
+    from tests.test_gpu_parity import O2_FLIP
+
+    rng = np.random.default_rng(2026)
+    argvs = [O2_FLIP] + [serve.request_argv(rng) for _ in range(24)]
+    flipped = 0
+    for argv in argvs:
+        try:
+            boxes, _ = oa.problem_boxes(oa.problem_from_argv(argv))
+        except oa.OllError:  # a box list the reference cannot set up either
+            continue
+        got = [b["colour_order"] for b in boxes]
+        s = O.Solver.from_argv(argv, sweep_order=O.COLOUR)
+        assert got == [s.colour_flip(i) for i in range(s.nbox)], argv
+        flipped += sum(got)
+    assert 1 in [b["colour_order"] for b in oa.problem_boxes(oa.problem_from_argv(O2_FLIP))[0]]
+    assert flipped > 1

@@ -30,7 +30,12 @@ SCALAR_O6 = ["-eq", "Punctures_scalar_field", "-B", "Robin", "-psc", "2", "-phi0
              "-nu2", "4", "-s", "1e-12"]
 O8 = PUNC + ["-NumBH", "1", "-bhmp1", "0.8", "-bhx1", "0.5", "-bhpy1", "0.15", "-bhsz1", "0.1",
              "-L", "50", "-N", "25", "-l", "2", "-O", "8", "-nu2", "4", "-s", "1e-12"]
-CASES = {"o4_merge": SMALL_O4, "sf_o6": SCALAR_O6, "o8": O8}
+# 2nd order, two punctures; the finest box's points that coincide with its
+# parent's nodes have odd i + j + k, so its sweeps take colour 1 first
+O2_FLIP = PUNC + ["-NumBH", "2", "-bhmp1", "0.5", "-bhx1", "0.5", "-bhpy1", "0.1",
+                  "-bhmp2", "0.4", "-bhx2", "-3", "-bhy2", "2", "-bhsz2", "0.1",
+                  "-L", "32", "-N", "25", "-l", "4", "-O", "2", "-nu2", "4", "-s", "1e-12"]
+CASES = {"o4_merge": SMALL_O4, "sf_o6": SCALAR_O6, "o8": O8, "o2_flip": O2_FLIP}
 
 
 def pair(argv):
@@ -133,6 +138,46 @@ def test_fas_cycles_match_oracle(case):
     for ib, b in enumerate(s.boxes()):
         g, o = h.get_field(ib, oa.F_U), s.field(ib, O.F_U)
         assert np.abs(g - o).max() <= 1e-11 * max(1.0, np.abs(o).max()), (case, ib)
+
+
+def test_colour_order_matches_oracle():
+    """The GPU's per-box colour order is the oracle's, and the 2nd-order case
+    has a box swept colour 1 first."""
+    h, s = pair(O2_FLIP)
+    got = [b["colour_order"] for b in h.boxes()]
+    assert got == [s.colour_flip(ib) for ib in range(s.nbox)]
+    assert 1 in got
+
+
+# the linear Brill-waves equation (Lap u + Lq u = 0), 2nd order, refined
+# around x = 0.5: two boxes are swept colour 1 first
+BRILL_FLIP = ["-eq", "Brill-waves", "-B", "Robin", "-NumBH", "1", "-bhx1", "0.5", "-L", "20",
+              "-N", "25", "-l", "4", "-O", "2", "-nu2", "4", "-s", "1e-12"]
+
+
+def test_mixed_smooth_two_colour_order():
+    """Mixed-precision (FP32 correction) sweeps on every level of a
+    2nd-order case with descending-order boxes, against the oracle's FP64
+    sweeps in the same colour order.  The equation is linear, so the FP32
+    sweeps on the correction equation are the FP64 sweeps up to FP32
+    rounding: the change of u agrees to 1e-5 relative; a wrong colour order
+    changes it at O(1)."""
+    p = oa.problem_from_argv(BRILL_FLIP)
+    p.precision = oa.PREC_MIXED
+    h = oa.Hierarchy(p)
+    s = O.Solver.from_argv(BRILL_FLIP, sweep_order=O.COLOUR)
+    assert sum(b["colour_order"] for b in h.boxes()) >= 2
+    seed_state(h, s, 4)
+    for lv in range(1, s.levels):
+        u0 = {ib: s.field(ib, O.F_U).copy() for ib in s.level_boxes(lv)}
+        h.smooth(lv, 2)
+        for ib in s.level_boxes(lv):
+            s.smooth(ib, 2)
+            dg = h.get_field(ib, oa.F_U) - u0[ib]
+            do = s.field(ib, O.F_U) - u0[ib]
+            print("level", lv, "box", ib, "colour order", s.colour_flip(ib),
+                  "rel diff %.2e" % rel_err(dg, do), flush=True)
+            assert rel_err(dg, do) < 1e-5, (lv, ib, s.colour_flip(ib), rel_err(dg, do))
 
 
 def test_solve_cfg1_matches_reference():
