@@ -69,7 +69,7 @@ CASES = {
     # discrete solution does not depend on the sweep counts)
     "cfg3r_o8": (CFG2[:-10] + ["-L", "1000", "-N", "25", "-l", "9", "-O", "8", "-nu1", "1",
                                "-nu2", "3", "-s", "1e-16"], 2),
-    "cfg2_allconv": (CFG2[:-2] + ["-nu1", "1", "-nu2", "3", "-s", "1e-16"], 4),
+    "cfg2_allconv": (CFG2[:-2] + ["-nu1", "1", "-nu2", "2", "-s", "1e-16"], 4),
     "cfg5r_o6": (SF3 + ["-L", "256", "-N", "25", "-l", "10", "-O", "6", "-nu2", "2",
                         "-s", "1e-16"], 2),
     # small cases covering orders 4/6/8, off-lattice punctures, spins,

@@ -3,11 +3,13 @@
 * the equal-mass binary (N=65, 9 levels, 4th order), the same binary at
   8th order and the three punctures with the scalar-field shell at 6th
   order (the N=129 and N=161 workloads reduced to N=25), against reference
-  runs converged on every level (tests/golden/cfg2_allconv, cfg3r_o8,
-  cfg5r_o6): every determined point, the puncture ADM masses and the
-  surface ADM masses, in FP64 and in mixed precision, solved with the
-  workloads' own sweep counts (the discrete solution does not depend on
-  them);
+  runs converged on every level (tests/golden/cfg3r_o8, cfg5r_o6; for the
+  binary, whose base box the reference's stopping rule leaves 2e-9 short
+  of convergence, its run continued past that rule by the oracle in the
+  reference's order, tests/golden/cfg2_continued): every determined point,
+  the puncture ADM masses and the surface ADM masses, in FP64 and in mixed
+  precision, solved with the workloads' own sweep counts (the discrete
+  solution does not depend on them);
 * the two full-size workloads (N=129, 31 M points; N=161, 139 M points),
   where no reference run exists: FP64 and mixed precision solved to the
   workload's tolerance, and the mixed solution within 1e-8 of the FP64 one
@@ -83,7 +85,7 @@ def compare_to_reference(h, meta, arrays, label):
             assert abs(g - r) <= ADM_RTOL * abs(r), (label, "surface ADM", ib, g, r)
 
 
-@pytest.mark.parametrize("name", ["cfg2_conv", "cfg3r_o8", "cfg5r_o6"])
+@pytest.mark.parametrize("name", ["cfg2_continued", "cfg3r_o8", "cfg5r_o6"])
 @pytest.mark.parametrize("prec", [oa.PREC_FP64, oa.PREC_MIXED], ids=["fp64", "mixed"])
 def test_workload_shapes_match_reference(name, prec):
     """The binary (N=65, 4th order), the 8th-order binary and the 6th-order

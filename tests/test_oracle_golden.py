@@ -67,3 +67,25 @@ def test_oracle_converged_cfg1_field():
     assert cycles == meta["cycles"] == 30
     for ib, ref in enumerate(golden_u(meta, arrays, s.boxes())):
         assert np.abs(s.field(ib, O.F_U) - ref).max() < 1e-14
+
+
+def test_continued_binary_fixture():
+    """tests/golden/cfg2_continued: the reference's binary run (N=65, 9
+    levels, cfg2_allconv) replayed by the oracle to 1e-14 and continued 150
+    V-cycles until no stored value moves (< 1e-14 over the last 20).  It
+    agrees with the reference's own stopped run to 1e-10 on every level but
+    the base box, which the reference's stopping rule leaves about 2e-9
+    short of convergence."""
+    cont, carr = load_golden("cfg2_continued")
+    ref, rarr = load_golden("cfg2_allconv")
+    assert cont["reference_cycles"] == ref["cycles"] == 73
+    assert cont["replay_max_diff"] <= 1e-14
+    assert cont["moved_over_last_20_cycles"] <= 1e-14
+    per_level = {}
+    for b in cont["boxes"]:
+        key = "u_l%d_b%d" % (b["level"], b["box"])
+        d = float(np.abs(carr[key] - rarr[key]).max())
+        per_level[b["level"]] = max(per_level.get(b["level"], 0.0), d)
+    mgrid = 2
+    assert all(d <= 1e-10 for lv, d in per_level.items() if lv > mgrid), per_level
+    assert 1e-9 < per_level[mgrid] < 5e-9, per_level
