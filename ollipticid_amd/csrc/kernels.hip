@@ -984,18 +984,31 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
           uj += uk * wy[J];
         }
     } else {
-      // wider stencils: one row of nodes at a time (register budget)
-      for (int J = 0; J < kk[1]; ++J) {
-        unsigned off = oJ, c = cJ;
-        double uk = 0.0;
+      // wider stencils: CJ rows of nodes at a time, every load of them
+      // issued before their sums (register budget); the sums keep the
+      // order of the row-by-row form
+      constexpr int CJ = KP / 2;
+      for (int J0 = 0; J0 < kk[1]; J0 += CJ) {
+        double v[CJ][KP];
 #pragma unroll
-        for (int K = 0; K < KP; ++K)
-          if (K < kk[2]) {
-            uk += ld<double>(f, off) * wz[K];
+        for (int J = 0; J < CJ; ++J) {
+          unsigned off = oJ, c = cJ;
+#pragma unroll
+          for (int K = 0; K < KP; ++K) {
+            v[J][K] = (J0 + J < kk[1] && K < kk[2]) ? ld<double>(f, off) : 0.0;
             off += wk.z(c);
           }
-        uj += uk * wy[J];
-        oJ += wk.y(cJ);
+          if (J0 + J < kk[1]) oJ += wk.y(cJ);
+        }
+#pragma unroll
+        for (int J = 0; J < CJ; ++J)
+          if (J0 + J < kk[1]) {
+            double uk = 0.0;
+#pragma unroll
+            for (int K = 0; K < KP; ++K)
+              if (K < kk[2]) uk += v[J][K] * wz[K];
+            uj += uk * wy[J0 + J];
+          }
       }
     }
     acc += uj * (w[0] ? w[0][I] : 1.0);
@@ -1372,7 +1385,11 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
 constexpr int kCoarseThreads = 512;
 
 // OWN: interior points per colour per thread (colour c's points are
-// numbered row by row, ceil(ni / 2) slots per interior row)
+// numbered row by row, ceil(ni / 2) slots per interior row).  With up to
+// 4 points per colour the coefficients of a thread's points stay in
+// registers for the whole loop; with more they would not fit the register
+// file (the kernel spilled), so each colour's coefficients are loaded from
+// the (cache-resident) fields at the start of the colour's pass.
 template <int EQ, int OWN>
 __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseItem *__restrict__ items,
                                                                  double eps, int max_iter) {
@@ -1412,8 +1429,21 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
   const int ni = n0 - 2, nj = n1 - 2, nk = n2 - 2;
   const int per_row = (ni + 1) / 2;
   const int slots = nj * nk * per_row;
+  constexpr bool kCached = OWN <= 4;
+  constexpr int NCACHE = kCached ? 2 * OWN : 1;
   int own_p[2 * OWN];
-  double own_b[2 * OWN], own_cc[2 * OWN], own_d[2 * OWN], own_e[2 * OWN];
+  double own_b[NCACHE], own_cc[NCACHE], own_d[NCACHE], own_e[NCACHE];
+  // the coefficients of owned point q (colour col, slot m)
+  auto coeffs = [&](int q, double &b, double &cc, double &d, double &e) {
+    const unsigned o = unsigned(own_p[q] & 0x3fffffff) * 8u;
+    b = ld<double>(B.f[F_B], o);
+    cc = ld<double>(B.f[F_C], o);
+    d = e = 0.0;
+    if (EQ == 1 || EQ == kEqPuncturesEm) {
+      d = ld<double>(B.f[F_D], o);
+      e = ld<double>(B.f[F_E], o);
+    }
+  };
 #pragma unroll
   for (int col = 0; col < 2; ++col)
 #pragma unroll
@@ -1421,21 +1451,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
       const int q = col * OWN + m;
       const int t = threadIdx.x + m * kCoarseThreads;
       own_p[q] = -1;
-      own_b[q] = own_cc[q] = own_d[q] = own_e[q] = 0.0;
+      if constexpr (kCached) own_b[q] = own_cc[q] = own_d[q] = own_e[q] = 0.0;
       if (t >= slots) continue;
       const int row = t / per_row, tt = t - row * per_row;
       const int j = 1 + row % nj, k = 1 + row / nj;
       const int i = 1 + ((col - j - k - 1) & 1) + 2 * tt;
       if (i >= n0 - 1) continue;
-      const int p = coarse_p(G, i, j, k) | ((i & 1) << 30);
-      own_p[q] = p;
-      const unsigned o = unsigned(p & 0x3fffffff) * 8u;
-      own_b[q] = ld<double>(B.f[F_B], o);
-      own_cc[q] = ld<double>(B.f[F_C], o);
-      if (EQ == 1 || EQ == kEqPuncturesEm) {
-        own_d[q] = ld<double>(B.f[F_D], o);
-        own_e[q] = ld<double>(B.f[F_E], o);
-      }
+      own_p[q] = coarse_p(G, i, j, k) | ((i & 1) << 30);
+      if constexpr (kCached) coeffs(q, own_b[q], own_cc[q], own_d[q], own_e[q]);
     }
   // shell points: faces (x, y, z pairs), edges (XY, XZ, YZ), corners;
   // decoded each sweep (one Robin body keeps the register budget)
@@ -1492,10 +1515,24 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
     double mx = 0.0;
 #pragma unroll
     for (int col = 0; col < 2; ++col) {
+      // uncached: this colour's coefficients, all loads issued together
+      double cb[kCached ? 1 : OWN], ccc[kCached ? 1 : OWN], cd[kCached ? 1 : OWN],
+          ce[kCached ? 1 : OWN];
+      if constexpr (!kCached) {
+#pragma unroll
+        for (int m = 0; m < OWN; ++m) {
+          cb[m] = ccc[m] = cd[m] = ce[m] = 0.0;
+          if (own_p[col * OWN + m] >= 0) coeffs(col * OWN + m, cb[m], ccc[m], cd[m], ce[m]);
+        }
+      }
 #pragma unroll
       for (int m = 0; m < OWN; ++m) {
         const int q = col * OWN + m;
         if (own_p[q] < 0) continue;
+        const double qb = kCached ? own_b[kCached ? q : 0] : cb[kCached ? 0 : m];
+        const double qc = kCached ? own_cc[kCached ? q : 0] : ccc[kCached ? 0 : m];
+        const double qd = kCached ? own_d[kCached ? q : 0] : cd[kCached ? 0 : m];
+        const double qe = kCached ? own_e[kCached ? q : 0] : ce[kCached ? 0 : m];
         const int p = own_p[q] & 0x3fffffff;
         const Nbr<2> nb = stencil_offs<2, 1>(col, own_p[q] >> 30, unsigned(G.cs), unsigned(G.rs),
                                              unsigned(G.ps));
@@ -1509,9 +1546,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
           const double ge[3] = {ld<double>(B.f[F_X0], o), ld<double>(B.f[F_X1], o), ld<double>(B.f[F_X2], o)};
           const double gf[3] = {ld<double>(B.f[F_X3], o), ld<double>(B.f[F_X4], o), ld<double>(B.f[F_X5], o)};
           const double du[3] = {0.0, 0.0, 0.0};  // order 2: centred derivatives
-          em_nonlinear(lu, G.dulap, su[p], g, du, own_b[q], own_d[q], ge, gf, N, dN);
+          em_nonlinear(lu, G.dulap, su[p], g, du, qb, qd, ge, gf, N, dN);
         } else {
-          nonlinear<EQ>(lu, G.dulap, own_b[q], own_cc[q] + su[p], own_d[q], own_e[q], N, dN);
+          nonlinear<EQ>(lu, G.dulap, qb, qc + su[p], qd, qe, N, dN);
         }
         const double res = N - srho[p];
         const double un = su[p] - omega * (res / dN);

@@ -34,14 +34,17 @@ def main():
                     help="also copy this many MiB device to device (counter calibration)")
     ap.add_argument("--fused", action="store_true",
                     help="fused double-buffered FP64 sweeps instead of colour by colour")
+    ap.add_argument("--fused-shape", default=None,
+                    help="ty,tz,threads of the fused sweeps' tiles (with --fused)")
     args = ap.parse_args()
 
     import ollipticid_amd as oa
     from ollipticid_amd import native
 
-    native.set_fused_sweeps(args.fused)
-
     p = oa.problem_from_argv(WORKLOADS[args.workload])
+    native.set_fused_sweeps(args.fused)
+    if args.fused_shape:
+        native.set_fused_shape(p.order, *[int(v) for v in args.fused_shape.split(",")])
     p.precision = oa.PREC_MIXED if args.precision == "mixed" else oa.PREC_FP64
     if args.cycles:
         p.max_cycles = args.cycles
@@ -58,6 +61,7 @@ def main():
     e.close()
     out = {"workload": args.workload, "argv": " ".join(WORKLOADS[args.workload]),
            "batch": args.batch, "precision": args.precision, "fused": args.fused,
+           "fused_shape": native.fused_shape(p.order) if args.fused else None,
            "cycles": cycles,
            "requests_done": len(done), "smoother_launches": launches, "smoother_ms": ms,
            "smoother_bytes": nbytes}
