@@ -5,7 +5,7 @@ single-GPU workload, three punctures with a scalar-field shell
 (Punctures_scalar_field; phi = A exp(-(r-r0)^2/sigma^2), A = 0.01, r0 = 10,
 sigma = 2; bare masses 0.4, 0.3, 0.3 at positions seeded 2026 within
 |x| < 6; L = 256, N = 161, 10 refinement levels, 6th-order FD, nu2 = 20,
-tol 1e-10; 139 M points, 12 GiB).  A step resets the resident hierarchy to
+tol 1e-10; 139 M points, 10.5 GiB of fields).  A step resets the resident hierarchy to
 u = 0, solves it under the reference's stopping rule and computes the ADM
 masses (allocation-free entry point).  Geometry, transfer tables and the
 field slab are built before the timed region; no host buffers cross PCIe

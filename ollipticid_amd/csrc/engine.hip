@@ -417,6 +417,15 @@ int colour_flip(const Hierarchy &g, const Box &b, int nc) {
   return sum & 1;
 }
 void set_fused_sweeps(bool on) { g_fused_sweeps = on; }
+static bool g_ghost_columns = true;
+void set_ghost_columns(bool on) { g_ghost_columns = on; }
+bool ghost_columns() { return g_ghost_columns; }
+// columns pay at orders 6 and 8 (3 and 4 points per column share 8^3 and
+// 10^3 node stencils: ghost fill of the N=161 6th-order workload 766 ->
+// 502 us per launch); at orders 2 and 4 the halved thread count and lower
+// occupancy cost more than the saved loads (4th-order binary: 10% slower
+// alone), so those fill point by point
+bool ghost_columns_for(int order) { return g_ghost_columns && order >= 6; }
 bool fused_sweeps() { return g_fused_sweeps; }
 
 void Schedule::clear() {
@@ -657,13 +666,15 @@ struct Builder {
 
   // point blocks of the target boxes of level tl with transfer refs
   // X[k][t] (k = the member's fine level of the pair)
+  // (shell: the ghost shell's points, or with columns its columns of R
+  // points across the face slabs)
   static void points(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
-                     bool shell, int mg, std::vector<PtBlock> &out) {
+                     bool shell, int mg, std::vector<PtBlock> &out, bool columns = false) {
     const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
     for (int t = 0; t < n; ++t) {
       const DevBox &B = H.hbox[size_t(f + t)];
       const Hier::XRef r = X[size_t(k)][size_t(t)];
-      const long long np = shell ? shell_pts(B) : npts(B);
+      const long long np = shell ? (columns ? shell_pts(B) / B.R : shell_pts(B)) : npts(B);
       const long long plane = (long long)B.n[0] * B.n[1];
       for (long long q0 = 0; q0 < np; q0 += kPtThreads)
         out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, mg, unsigned(q0),
@@ -686,13 +697,16 @@ struct Builder {
       if (kind == Launch::RESTRICT_DEFECT)
         points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}]);
       if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, groups[{0, 0}]);
-      if (kind == Launch::GHOST && !mg) points(H, k, k, H.gf, true, 0, groups[{H.level_order(k), 0}]);
+      if (kind == Launch::GHOST && !mg)
+        points(H, k, k, H.gf, true, 0, groups[{H.level_order(k), 0}],
+               ghost_columns_for(H.level_order(k)));
     }
     for (auto &g : groups) {
       if (g.second.empty()) continue;
       Launch L{kind};
       L.order = g.first.first;
       L.eq = g.first.second;
+      L.mode = kind == Launch::GHOST && ghost_columns_for(L.order) ? 1 : 0;
       L.off = S.add(g.second);
       L.n = int(g.second.size());
       S.launches.push_back(L);
@@ -913,7 +927,7 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         launch_copy_shell(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
       case Launch::GHOST:
-        launch_ghost_fill(s, reinterpret_cast<const PtBlock *>(items), L.n, L.order);
+        launch_ghost_fill(s, reinterpret_cast<const PtBlock *>(items), L.n, L.order, L.mode);
         break;
     }
     if (e1) {

@@ -139,6 +139,12 @@ int colour_flip(const Hierarchy &g, const Box &b, int nc);
 // as fused, double-buffered colour-pass pairs (default off); off issues
 // one launch per colour.  Process-wide; schedules built afterwards use it.
 void set_fused_sweeps(bool on);
+// ghost fills one thread per shell column (default) or per shell point;
+// the same bits either way.  Process-wide; schedules built afterwards use it.
+void set_ghost_columns(bool on);
+bool ghost_columns();
+// whether ghost fills of boxes of this stencil order go by columns
+bool ghost_columns_for(int order);
 bool fused_sweeps();
 
 int upload_schedule(Schedule &S, hipStream_t s, std::string &err);

@@ -1305,6 +1305,276 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
   }
 }
 
+// column c of the ghost shell: the R shell points that differ along the
+// slab normal `axis` only (z slabs: columns (i, j); y slabs: (i, k) with k
+// interior; x slabs: (j, k) with j, k interior; low side first); returns
+// the column's first point.  The columns of a box are shell_pts / R.
+__device__ __forceinline__ void shell_column(ConstBox &B, unsigned q, int &i, int &j, int &k,
+                                             int &axis) {
+  const unsigned R = unsigned(B.R);
+  const unsigned n0 = unsigned(B.n[0]), n1 = unsigned(B.n[1]), n2 = unsigned(B.n[2]);
+  const unsigned zc = n0 * n1;
+  if (q < 2 * zc) {
+    const unsigned s = q >= zc, r = q - s * zc;
+    j = int(fast_div(r, n0, B.in0));
+    i = int(r - unsigned(j) * n0);
+    k = s ? int(n2 - R) : 0;
+    axis = 2;
+    return;
+  }
+  q -= 2 * zc;
+  const unsigned yc = n0 * (n2 - 2 * R);
+  if (q < 2 * yc) {
+    const unsigned s = q >= yc, r = q - s * yc;
+    const unsigned kk = fast_div(r, n0, B.in0);
+    i = int(r - kk * n0);
+    k = int(R + kk);
+    j = s ? int(n1 - R) : 0;
+    axis = 1;
+    return;
+  }
+  q -= 2 * yc;
+  const unsigned xc = (n1 - 2 * R) * (n2 - 2 * R);
+  const unsigned s = q >= xc, r = q - s * xc;
+  const unsigned kk = fast_div(r, n1 - 2 * R, B.inyi);
+  j = int(R + (r - kk * (n1 - 2 * R)));
+  k = int(R + kk);
+  i = s ? int(n0 - R) : 0;
+  axis = 0;
+}
+
+// Ghost shell of refinement boxes, one thread per shell column (above).
+// The column's points share the node ranges and weights of the two axes
+// across the slab, so the node values are loaded once for all of them over
+// the union of their node ranges along the normal (at most KP + 3 nodes);
+// each point's sums run over its own nodes in tinterp's order (z innermost,
+// then y, then x, ascending), so every value is tinterp's to the bit.
+// Columns whose points take different sources fall back to tinterp point
+// by point.
+template <int KP, int ax>
+__device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, const int p0[3]) {
+  constexpr int KU = KP + 3;       // union of the column's node ranges, at most
+  constexpr int RG = (KP - 2) / 2;  // ghost width of the boxes of this stencil
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  int src[RG];
+  unsigned fn[RG];  // flags of the normal axis per point
+  unsigned fl0[3] = {0, 0, 0};
+  bool same = true;
+#pragma unroll
+  for (int l = 0; l < RG; ++l) {
+    src[l] = -1;
+    fn[l] = 0;
+    int t[3] = {p0[0], p0[1], p0[2]};
+    t[ax] += l;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      unsigned fl[3];
+      const unsigned f = axis_flags(xs[x], t, fl);
+      if (!(f & 3)) continue;
+      src[l] = x;
+      fn[l] = fl[ax];
+      if (l == 0) {
+        fl0[0] = fl[0];
+        fl0[1] = fl[1];
+        fl0[2] = fl[2];
+      }
+      break;
+    }
+    same = same && src[l] == src[0];
+  }
+  if (src[0] < 0 || !same) {
+#pragma unroll
+    for (int l = 0; l < RG; ++l) {
+      if (src[l] < 0) continue;
+      int t[3] = {p0[0], p0[1], p0[2]};
+      t[ax] += l;
+      ConstXfer &X = xs[src[l]];
+      unsigned fl[3];
+      axis_flags(X, t, fl);
+      ConstBox &S = cbox(X.src);
+      st<double>(T.f[F_U], boff(T, t[0], t[1], t[2]), tinterp<KP>(S, S.f[F_U], X, t, fl));
+    }
+    return;
+  }
+  ConstXfer &X = xs[src[0]];
+  ConstBox &S = cbox(X.src);
+  const void *f = S.f[F_U];
+  // the two axes across the slab: shared nodes and weights (tinterp's)
+  int s0[3], kk[3];
+  const double *w[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (fl0[a] & 4) {
+      s0[a] = X.ax[a].coinc[p0[a]];
+      kk[a] = 1;
+      w[a] = nullptr;
+    } else {
+      s0[a] = X.ax[a].start[p0[a]];
+      kk[a] = KP;
+      w[a] = X.ax[a].w + p0[a] * KP;
+    }
+  }
+  // the normal: each point's first node and count, their union from lo
+  int first[RG], cnt[RG];
+  int lo = 1 << 30, hi = -1;
+#pragma unroll
+  for (int l = 0; l < RG; ++l) {
+    const int t = p0[ax] + l;
+    if (fn[l] & 4) {
+      first[l] = X.ax[ax].coinc[t];
+      cnt[l] = 1;
+    } else {
+      first[l] = X.ax[ax].start[t];
+      cnt[l] = KP;
+    }
+    lo = min(lo, first[l]);
+    hi = max(hi, first[l] + cnt[l] - 1);
+  }
+  const int U = hi - lo + 1;
+  if (U > KU) {  // not a refinement by two along the normal: point by point
+#pragma unroll
+    for (int l = 0; l < RG; ++l) {
+      int t[3] = {p0[0], p0[1], p0[2]};
+      t[ax] += l;
+      unsigned fl[3];
+      axis_flags(X, t, fl);
+      st<double>(T.f[F_U], boff(T, t[0], t[1], t[2]), tinterp<KP>(S, f, X, t, fl));
+    }
+    return;
+  }
+  // each point's weights along the normal at union positions (zero outside
+  // its range, where no term is added)
+  double wn[RG][KU];
+#pragma unroll
+  for (int l = 0; l < RG; ++l) {
+    const int d = first[l] - lo;
+    const double *wl = (fn[l] & 4) ? nullptr : X.ax[ax].w + (p0[ax] + l) * KP;
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      double v = 0.0;
+      if (u >= d && u < d + cnt[l]) v = wl ? wl[u - d] : 1.0;
+      wn[l][u] = v;
+    }
+  }
+  s0[ax] = lo;
+  kk[ax] = U;
+  const Walk wk(S);
+  unsigned oI = boff(S, s0[0], s0[1], s0[2]);
+  unsigned cI = colourc(S, s0[0], s0[1], s0[2]);
+  unsigned rI = unsigned(s0[0]) - divc(S, unsigned(s0[0])) * wk.nc;
+  double acc[RG];
+#pragma unroll
+  for (int l = 0; l < RG; ++l) acc[l] = 0.0;
+  // a node of the union lies in point l's range
+#define OLL_IN(l, u) ((u) >= first[l] - lo && (u) < first[l] - lo + cnt[l])
+  for (int I = 0; I < kk[0]; ++I) {
+    double uj[RG];
+#pragma unroll
+    for (int l = 0; l < RG; ++l) uj[l] = 0.0;
+    double ujx = 0.0;  // normal along x: the shared (y, z) sum
+    unsigned oJ = oI, cJ = cI;
+    if (ax == 1) {
+      // normal along y: z sums of every union row, then per point over y
+      double uk[KU];
+#pragma unroll
+      for (int J = 0; J < KU; ++J) {
+        unsigned off = oJ, c = cJ;
+        double v[KP];
+#pragma unroll
+        for (int K = 0; K < KP; ++K) {
+          v[K] = (J < U && K < kk[2]) ? ld<double>(f, off) : 0.0;
+          off += wk.z(c);
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int K = 0; K < KP; ++K)
+          if (K < kk[2]) s += v[K] * (w[2] ? w[2][K] : 1.0);
+        uk[J] = s;
+        if (J < U) oJ += wk.y(cJ);
+      }
+#pragma unroll
+      for (int l = 0; l < RG; ++l)
+#pragma unroll
+        for (int J = 0; J < KU; ++J)
+          if (OLL_IN(l, J)) uj[l] += uk[J] * wn[l][J];
+    } else {
+      for (int J = 0; J < kk[1]; ++J) {
+        unsigned off = oJ, c = cJ;
+        const double wy = w[1] ? w[1][J] : 1.0;
+        if (ax == 2) {
+          // normal along z: one union row, then per point over z
+          double v[KU];
+#pragma unroll
+          for (int K = 0; K < KU; ++K) {
+            v[K] = K < U ? ld<double>(f, off) : 0.0;
+            off += wk.z(c);
+          }
+#pragma unroll
+          for (int l = 0; l < RG; ++l) {
+            double s = 0.0;
+#pragma unroll
+            for (int K = 0; K < KU; ++K)
+              if (OLL_IN(l, K)) s += v[K] * wn[l][K];
+            uj[l] += s * wy;
+          }
+        } else {
+          // normal along x: the shared row sum
+          double v[KP];
+#pragma unroll
+          for (int K = 0; K < KP; ++K) {
+            v[K] = K < kk[2] ? ld<double>(f, off) : 0.0;
+            off += wk.z(c);
+          }
+          double s = 0.0;
+#pragma unroll
+          for (int K = 0; K < KP; ++K)
+            if (K < kk[2]) s += v[K] * (w[2] ? w[2][K] : 1.0);
+          ujx += s * wy;
+        }
+        oJ += wk.y(cJ);
+      }
+    }
+    if (ax == 0) {
+#pragma unroll
+      for (int l = 0; l < RG; ++l)
+        if (OLL_IN(l, I)) {
+          double wI = 0.0;  // wn[l][I] without a dynamic register index
+#pragma unroll
+          for (int u = 0; u < KU; ++u)
+            if (u == I) wI = wn[l][u];
+          acc[l] += ujx * wI;
+        }
+    } else {
+      const double wx = w[0] ? w[0][I] : 1.0;
+#pragma unroll
+      for (int l = 0; l < RG; ++l) acc[l] += uj[l] * wx;
+    }
+    oI += wk.x(cI, rI);
+  }
+#undef OLL_IN
+#pragma unroll
+  for (int l = 0; l < RG; ++l) {
+    int t[3] = {p0[0], p0[1], p0[2]};
+    t[ax] += l;
+    st<double>(T.f[F_U], boff(T, t[0], t[1], t[2]), acc[l]);
+  }
+}
+
+template <int KP>
+__global__ __launch_bounds__(kPtThreads) void k_ghost_cols(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
+  if (threadIdx.x >= Bk.nq) return;
+  int i, j, k, ax;
+  shell_column(T, Bk.q0 + threadIdx.x, i, j, k, ax);
+  const int p0[3] = {i, j, k};
+  if (ax == 0)
+    ghost_column<KP, 0>(Bk, T, p0);
+  else if (ax == 1)
+    ghost_column<KP, 1>(Bk, T, p0);
+  else
+    ghost_column<KP, 2>(Bk, T, p0);
+}
+
 // r = u on the ghost shell of the listed boxes (the fused sweeps read
 // ghost values from either field)
 __global__ __launch_bounds__(kPtThreads) void k_copy_shell(const PtBlock *__restrict__ blocks) {
@@ -2107,13 +2377,21 @@ void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks) {
   hipLaunchKernelGGL(k_prolong, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
 }
 
-void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order) {
+template <int KP>
+void ghost_impl(hipStream_t s, const PtBlock *blocks, int nblocks, int columns) {
+  if (columns)
+    hipLaunchKernelGGL(k_ghost_cols<KP>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
+  else
+    hipLaunchKernelGGL(k_ghost_fill<KP>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
+}
+
+void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int columns) {
   if (nblocks == 0) return;
   switch (order) {
-    case 2: hipLaunchKernelGGL(k_ghost_fill<4>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
-    case 4: hipLaunchKernelGGL(k_ghost_fill<6>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
-    case 6: hipLaunchKernelGGL(k_ghost_fill<8>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
-    default: hipLaunchKernelGGL(k_ghost_fill<10>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks); break;
+    case 2: ghost_impl<4>(s, blocks, nblocks, columns); break;
+    case 4: ghost_impl<6>(s, blocks, nblocks, columns); break;
+    case 6: ghost_impl<8>(s, blocks, nblocks, columns); break;
+    default: ghost_impl<10>(s, blocks, nblocks, columns); break;
   }
 }
 

@@ -409,7 +409,9 @@ void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, i
 void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks);
 // order: stencil order of the target (refinement) boxes; Lagrange order+1
-void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order);
+// columns = 1: blocks over shell columns (R points across a face slab, one
+// thread each); 0: blocks over shell points
+void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int columns);
 // false when the largest box does not fit the LDS-resident kernel
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
                          int eq, double eps, int max_iter);

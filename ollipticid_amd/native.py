@@ -117,6 +117,8 @@ SIGNATURES = {
     "oll_get_sweep_threads": (C.c_int, [C.c_int, C.c_int]),
     "oll_set_fused_sweeps": (C.c_int, [C.c_int]),
     "oll_get_fused_sweeps": (C.c_int, []),
+    "oll_set_ghost_columns": (C.c_int, [C.c_int]),
+    "oll_get_ghost_columns": (C.c_int, []),
     "oll_set_fused_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "oll_get_fused_shape": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
@@ -175,6 +177,16 @@ def set_fused_sweeps(on):
 
 def fused_sweeps():
     return bool(load().oll_get_fused_sweeps())
+
+
+def set_ghost_columns(on):
+    """Ghost fills of 6th- and 8th-order boxes one thread per shell column
+    (the default) or every box per shell point; the same bits either way."""
+    _check(load().oll_set_ghost_columns(1 if on else 0))
+
+
+def ghost_columns():
+    return bool(load().oll_get_ghost_columns())
 
 
 def set_fused_shape(order, ty, tz, threads):

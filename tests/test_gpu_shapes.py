@@ -1,10 +1,13 @@
 """Launch shapes change speed only: the colour passes with 64 and with 512
 threads per workgroup give bit for bit the solve of the 256-thread default
-(FP64 and mixed precision, 4th and 2nd order)."""
+(FP64 and mixed precision, 4th and 2nd order), and the ghost-shell fill by
+columns (6th and 8th order) gives bit for bit the fill point by point
+(merged and off-lattice boxes, the scalar-field equation)."""
 import numpy as np
 import pytest
 
 import ollipticid_amd as oa
+from ollipticid_amd import native
 from tests.util import load_golden
 
 pytestmark = pytest.mark.gpu
@@ -39,3 +42,19 @@ def test_sweep_threads_do_not_change_results(name, precision):
                 assert np.array_equal(a, b), (threads, ib)
     finally:
         oa.set_sweep_threads(precision, order, before)
+
+
+@pytest.mark.parametrize("name", ["small_o6_merge", "small_o8", "cfg3r_o8", "cfg5r_o6"])
+def test_ghost_columns_equal_points(name):
+    meta, _ = load_golden(name)
+    print("ghost fill", name, flush=True)
+    try:
+        native.set_ghost_columns(False)
+        ref = solve_fields(meta["argv"], oa.PREC_FP64, cycles=4)
+        native.set_ghost_columns(True)
+        got = solve_fields(meta["argv"], oa.PREC_FP64, cycles=4)
+    finally:
+        native.set_ghost_columns(True)
+    assert got[:2] == ref[:2], (got[:2], ref[:2])
+    for ib, (a, b) in enumerate(zip(got[2], ref[2])):
+        assert np.array_equal(a, b), (name, ib, float(np.abs(a - b).max()))
