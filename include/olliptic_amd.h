@@ -339,15 +339,17 @@ int oll_get_sweep_threads(int precision, int order);
  * built afterwards use it. */
 int oll_set_fused_sweeps(int on);
 int oll_get_fused_sweeps(void);
-/* Ghost-shell fill (no reference counterpart; the transfer of
- * elliptic.cc:1905-1908): on (the default), boxes of order 6 and 8 are
- * filled one thread per column of R shell points across a face slab, which
- * loads the interpolation nodes once for the whole column (orders 2 and 4
- * stay point by point, which measures faster there); off, every box point
- * by point.  The same bits either way.  Process-wide; schedules built
- * afterwards use it. */
-int oll_set_ghost_columns(int on);
-int oll_get_ghost_columns(void);
+/* Shared-load transfer kernels (no reference counterpart): on (the
+ * default), the ghost-shell fill (elliptic.cc:1905-1908) of boxes of order
+ * 6 and 8 runs one thread per column of R shell points across a face slab,
+ * loading the interpolation nodes once for the whole column (orders 2 and
+ * 4 stay point by point, which measures faster there), and the defect
+ * filter ahead of an off-lattice restriction runs one thread per run of 4
+ * points along x, loading the neighbour rows once for the run; off, both
+ * go point by point.  The same bits either way.  Process-wide; schedules
+ * built afterwards use it. */
+int oll_set_shared_loads(int on);
+int oll_get_shared_loads(void);
 /* tile of the fused sweeps for one order: ty x tz interior rows (1..64
  * each) and threads per workgroup (a multiple of 64, 64..512; default
  * 8 x 8 rows, 256 threads); OLL_EINVAL for other values */

@@ -773,12 +773,12 @@ int oll_set_fused_sweeps(int on) {
 
 int oll_get_fused_sweeps(void) { return fused_sweeps() ? 1 : 0; }
 
-int oll_set_ghost_columns(int on) {
-  set_ghost_columns(on != 0);
+int oll_set_shared_loads(int on) {
+  set_shared_loads(on != 0);
   return OLL_OK;
 }
 
-int oll_get_ghost_columns(void) { return ghost_columns() ? 1 : 0; }
+int oll_get_shared_loads(void) { return shared_loads() ? 1 : 0; }
 
 int oll_set_fused_shape(int order, int ty, int tz, int threads) {
   if (!set_fused_shape(order, ty, tz, threads))

@@ -417,15 +417,15 @@ int colour_flip(const Hierarchy &g, const Box &b, int nc) {
   return sum & 1;
 }
 void set_fused_sweeps(bool on) { g_fused_sweeps = on; }
-static bool g_ghost_columns = true;
-void set_ghost_columns(bool on) { g_ghost_columns = on; }
-bool ghost_columns() { return g_ghost_columns; }
+static bool g_shared_loads = true;
+void set_shared_loads(bool on) { g_shared_loads = on; }
+bool shared_loads() { return g_shared_loads; }
 // columns pay at orders 6 and 8 (3 and 4 points per column share 8^3 and
 // 10^3 node stencils: ghost fill of the N=161 6th-order workload 766 ->
 // 502 us per launch); at orders 2 and 4 the halved thread count and lower
 // occupancy cost more than the saved loads (4th-order binary: 10% slower
 // alone), so those fill point by point
-bool ghost_columns_for(int order) { return g_ghost_columns && order >= 6; }
+bool ghost_columns_for(int order) { return g_shared_loads && order >= 6; }
 bool fused_sweeps() { return g_fused_sweeps; }
 
 void Schedule::clear() {
@@ -725,7 +725,10 @@ struct Builder {
       for (int b = f; b < f + n; ++b) {
         if (!H.filt_src[size_t(b)]) continue;
         const DevBox &B = H.hbox[size_t(b)];
-        const long long np = npts(B), plane = (long long)B.n[0] * B.n[1];
+        const int run = filter_run_length();
+        const long long np = shared_loads() ? (long long)((B.n[0] + run - 1) / run) * B.n[1] * B.n[2]
+                                            : npts(B);
+        const long long plane = (long long)B.n[0] * B.n[1];
         for (long long q0 = 0; q0 < np; q0 += kPtThreads)
           items.push_back(PtBlock{H.dbox + b, nullptr, 0, 0, unsigned(q0),
                                   unsigned(std::min<long long>(kPtThreads, np - q0)),
@@ -734,6 +737,7 @@ struct Builder {
     }
     if (items.empty()) return;
     Launch L{Launch::FILTER};
+    L.mode = shared_loads() ? 1 : 0;
     L.off = S.add(items);
     L.n = int(items.size());
     S.launches.push_back(L);
@@ -917,7 +921,7 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         launch_prolong(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
       case Launch::FILTER:
-        launch_filter_defect(s, reinterpret_cast<const PtBlock *>(items), L.n);
+        launch_filter_defect(s, reinterpret_cast<const PtBlock *>(items), L.n, L.mode);
         break;
       case Launch::SMOOTH_FUSED:
         launch_smooth_fused(s, reinterpret_cast<const TileBlock *>(items), L.n, L.order, L.eq,

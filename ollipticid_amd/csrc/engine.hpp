@@ -139,10 +139,11 @@ int colour_flip(const Hierarchy &g, const Box &b, int nc);
 // as fused, double-buffered colour-pass pairs (default off); off issues
 // one launch per colour.  Process-wide; schedules built afterwards use it.
 void set_fused_sweeps(bool on);
-// ghost fills one thread per shell column (default) or per shell point;
-// the same bits either way.  Process-wide; schedules built afterwards use it.
-void set_ghost_columns(bool on);
-bool ghost_columns();
+// ghost fills one thread per shell column and defect filters one thread
+// per run of points along x (default), or both one thread per point; the
+// same bits either way.  Process-wide; schedules built afterwards use it.
+void set_shared_loads(bool on);
+bool shared_loads();
 // whether ghost fills of boxes of this stencil order go by columns
 bool ghost_columns_for(int order);
 bool fused_sweeps();

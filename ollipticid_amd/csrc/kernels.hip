@@ -1163,6 +1163,83 @@ __global__ __launch_bounds__(kPtThreads) void k_filter_defect(const PtBlock *__r
   st<double>(S.f[F_V], o, in ? fw27(S, S.f[F_W], i, j, k) : ld<double>(S.f[F_W], o));
 }
 
+// The same filter, one thread per run of kFilterCx points along x of one
+// row: the 9 neighbour rows' kFilterCx + 2 values are loaded once for the
+// run (the lanes of a wave then read consecutive slots of one colour block
+// per load), and each point sums its 27 terms in fw27's order (same bits).
+constexpr int kFilterCx = 4;
+__global__ __launch_bounds__(kPtThreads) void k_filter_cols(const PtBlock *__restrict__ blocks) {
+  constexpr int NX = kFilterCx + 2;
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &S = cbox(Bk.box);
+  if (threadIdx.x >= Bk.nq) return;
+  const int n0 = S.n[0], n1 = S.n[1], n2 = S.n[2], R = S.R;
+  const int nxc = (n0 + kFilterCx - 1) / kFilterCx;
+  const int q = int(Bk.q0 + threadIdx.x);
+  const int row = q / nxc;
+  const int i0 = (q - row * nxc) * kFilterCx;
+  const int k = row / n1, j = row - k * n1;
+  const void *w = S.f[F_W];
+  const bool in_jk = j - 1 >= R && j + 1 < n1 - R && k - 1 >= R && k + 1 < n2 - R;
+  bool in[kFilterCx];
+  bool any = false;
+#pragma unroll
+  for (int p = 0; p < kFilterCx; ++p) {
+    const int i = i0 + p;
+    in[p] = in_jk && i - 1 >= R && i + 1 < n0 - R;
+    any = any || in[p];
+  }
+  double acc[kFilterCx];
+#pragma unroll
+  for (int p = 0; p < kFilterCx; ++p) acc[p] = 0.0;
+  if (any) {
+    // x nodes i0 - 1 .. i0 + kFilterCx of rows j - 1 .. j + 1, planes
+    // k - 1 .. k + 1 (all inside the box: in_jk); nodes past the box ends
+    // along x are not loaded and feed no filtered point
+    const Walk wk(S);
+    unsigned ox[NX], cx[NX];
+    bool okx[NX];
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+      const int xi = i0 - 1 + x;
+      okx[x] = xi >= 0 && xi < n0;
+      ox[x] = okx[x] ? boff(S, xi, j - 1, k - 1) : 0u;
+      cx[x] = okx[x] ? colourc(S, xi, j - 1, k - 1) : 0u;
+    }
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz) {
+      const double wz = dz == 0 ? 0.5 : 0.25;
+      unsigned oy[NX], cy[NX];
+#pragma unroll
+      for (int x = 0; x < NX; ++x) {
+        oy[x] = ox[x];
+        cy[x] = cx[x];
+      }
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy) {
+        const double wy = dy == 0 ? 0.5 : 0.25;
+        double val[NX];
+#pragma unroll
+        for (int x = 0; x < NX; ++x) val[x] = okx[x] ? ld<double>(w, oy[x]) : 0.0;
+#pragma unroll
+        for (int p = 0; p < kFilterCx; ++p)
+          if (in[p]) acc[p] += wz * wy * (0.25 * val[p] + 0.5 * val[p + 1] + 0.25 * val[p + 2]);
+#pragma unroll
+        for (int x = 0; x < NX; ++x) oy[x] += wk.y(cy[x]);
+      }
+#pragma unroll
+      for (int x = 0; x < NX; ++x) ox[x] += wk.z(cx[x]);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kFilterCx; ++p) {
+    const int i = i0 + p;
+    if (i >= n0) continue;
+    const unsigned o = boff(S, i, j, k);
+    st<double>(S.f[F_V], o, in[p] ? acc[p] : ld<double>(w, o));
+  }
+}
+
 // Coarse FAS source: W_c = restriction of the fine defect (full weighting
 // where the coarse point coincides with a fine point whose 3x3x3
 // neighbourhood is fine interior, injection / Lagrange-2 elsewhere, of the
@@ -2367,10 +2444,15 @@ void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, i
   OLL_DISPATCH_ORDER(order, eq, restrict_defect_impl, s, blocks, nblocks);
 }
 
-void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks) {
+void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int runs) {
   if (nblocks == 0) return;
-  hipLaunchKernelGGL(k_filter_defect, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
+  if (runs)
+    hipLaunchKernelGGL(k_filter_cols, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
+  else
+    hipLaunchKernelGGL(k_filter_defect, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
 }
+
+int filter_run_length() { return kFilterCx; }
 
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks) {
   if (nblocks == 0) return;

@@ -405,8 +405,11 @@ void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq);
-// v = full-weighting filter of the defect w (all points of the listed boxes)
-void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks);
+// v = full-weighting filter of the defect w (all points of the listed
+// boxes); runs = 1: blocks over runs of filter_run_length() points along x
+// (ceil(n0 / length) runs per row), 0: blocks over points
+void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int runs);
+int filter_run_length();
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks);
 // order: stencil order of the target (refinement) boxes; Lagrange order+1
 // columns = 1: blocks over shell columns (R points across a face slab, one

@@ -117,8 +117,8 @@ SIGNATURES = {
     "oll_get_sweep_threads": (C.c_int, [C.c_int, C.c_int]),
     "oll_set_fused_sweeps": (C.c_int, [C.c_int]),
     "oll_get_fused_sweeps": (C.c_int, []),
-    "oll_set_ghost_columns": (C.c_int, [C.c_int]),
-    "oll_get_ghost_columns": (C.c_int, []),
+    "oll_set_shared_loads": (C.c_int, [C.c_int]),
+    "oll_get_shared_loads": (C.c_int, []),
     "oll_set_fused_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "oll_get_fused_shape": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
@@ -179,14 +179,15 @@ def fused_sweeps():
     return bool(load().oll_get_fused_sweeps())
 
 
-def set_ghost_columns(on):
+def set_shared_loads(on):
     """Ghost fills of 6th- and 8th-order boxes one thread per shell column
-    (the default) or every box per shell point; the same bits either way."""
-    _check(load().oll_set_ghost_columns(1 if on else 0))
+    and defect filters one thread per run of points along x (the default),
+    or both point by point; the same bits either way."""
+    _check(load().oll_set_shared_loads(1 if on else 0))
 
 
-def ghost_columns():
-    return bool(load().oll_get_ghost_columns())
+def shared_loads():
+    return bool(load().oll_get_shared_loads())
 
 
 def set_fused_shape(order, ty, tz, threads):

@@ -1,8 +1,9 @@
 """Launch shapes change speed only: the colour passes with 64 and with 512
 threads per workgroup give bit for bit the solve of the 256-thread default
 (FP64 and mixed precision, 4th and 2nd order), and the ghost-shell fill by
-columns (6th and 8th order) gives bit for bit the fill point by point
-(merged and off-lattice boxes, the scalar-field equation)."""
+columns (6th and 8th order) and the defect filter by runs of points give
+bit for bit the point-by-point kernels (merged and off-lattice boxes, the
+scalar-field equation, 2nd to 8th order)."""
 import numpy as np
 import pytest
 
@@ -44,17 +45,18 @@ def test_sweep_threads_do_not_change_results(name, precision):
         oa.set_sweep_threads(precision, order, before)
 
 
-@pytest.mark.parametrize("name", ["small_o6_merge", "small_o8", "cfg3r_o8", "cfg5r_o6"])
-def test_ghost_columns_equal_points(name):
+@pytest.mark.parametrize("name", ["small_o2_4bh", "small_o4_offgrid", "small_o6_merge", "small_o8",
+                                  "small_sf_3bh", "cfg3r_o8", "cfg5r_o6"])
+def test_shared_loads_equal_points(name):
     meta, _ = load_golden(name)
     print("ghost fill", name, flush=True)
     try:
-        native.set_ghost_columns(False)
+        native.set_shared_loads(False)
         ref = solve_fields(meta["argv"], oa.PREC_FP64, cycles=4)
-        native.set_ghost_columns(True)
+        native.set_shared_loads(True)
         got = solve_fields(meta["argv"], oa.PREC_FP64, cycles=4)
     finally:
-        native.set_ghost_columns(True)
+        native.set_shared_loads(True)
     assert got[:2] == ref[:2], (got[:2], ref[:2])
     for ib, (a, b) in enumerate(zip(got[2], ref[2])):
         assert np.array_equal(a, b), (name, ib, float(np.abs(a - b).max()))
