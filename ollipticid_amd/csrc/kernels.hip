@@ -1737,11 +1737,11 @@ constexpr int kCoarseThreads = 512;
 // registers for the whole loop; with more they would not fit the register
 // file (the kernel spilled), so each colour's coefficients are loaded from
 // the (cache-resident) fields at the start of the colour's pass.
-template <int EQ, int OWN>
-__global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseItem *__restrict__ items,
+template <int EQ, int OWN, int NT = kCoarseThreads>
+__global__ __launch_bounds__(NT) void k_coarse_solve(const CoarseItem *__restrict__ items,
                                                                  double eps, int max_iter) {
   extern __shared__ double lds[];
-  __shared__ double sh[kCoarseThreads / 64];
+  __shared__ double sh[NT / 64];
   const CoarseItem It = items[blockIdx.x];
   ConstBox &B = cbox(It.box);
   CoarseGeo G;
@@ -1768,7 +1768,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
   double *srho = lds + np;
   double *ug = B.f[F_U];
   const double *rg = B.f[F_RHO];
-  for (int q = threadIdx.x; q < np; q += kCoarseThreads) {
+  for (int q = threadIdx.x; q < np; q += NT) {
     su[q] = ld<double>(ug, unsigned(q) * 8u);
     srho[q] = ld<double>(rg, unsigned(q) * 8u);
   }
@@ -1776,7 +1776,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
   const int ni = n0 - 2, nj = n1 - 2, nk = n2 - 2;
   const int per_row = (ni + 1) / 2;
   const int slots = nj * nk * per_row;
-  constexpr bool kCached = OWN <= 4;
+  constexpr bool kCached = OWN <= 4 && NT <= 512;
   constexpr int NCACHE = kCached ? 2 * OWN : 1;
   int own_p[2 * OWN];
   double own_b[NCACHE], own_cc[NCACHE], own_d[NCACHE], own_e[NCACHE];
@@ -1796,7 +1796,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
 #pragma unroll
     for (int m = 0; m < OWN; ++m) {
       const int q = col * OWN + m;
-      const int t = threadIdx.x + m * kCoarseThreads;
+      const int t = threadIdx.x + m * NT;
       own_p[q] = -1;
       if constexpr (kCached) own_b[q] = own_cc[q] = own_d[q] = own_e[q] = 0.0;
       if (t >= slots) continue;
@@ -1817,7 +1817,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
   int it = 0;
   do {
     prev = cur;
-    for (int t = threadIdx.x; t < nface; t += kCoarseThreads) {
+    for (int t = threadIdx.x; t < nface; t += NT) {
       int i, j, k, mask, r = t;
       if (r < 2 * fx) {
         const int hi = r >= fx;
@@ -1913,10 +1913,10 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_solve(const CoarseIte
     __syncthreads();
     cur = sh[0];
 #pragma unroll
-    for (int w = 1; w < kCoarseThreads / 64; ++w) cur = fmax(cur, sh[w]);
+    for (int w = 1; w < NT / 64; ++w) cur = fmax(cur, sh[w]);
     ++it;
   } while (cur > eps && prev != cur && it < max_iter && !isinf(cur));
-  for (int q = threadIdx.x; q < np; q += kCoarseThreads) st<double>(ug, unsigned(q) * 8u, su[q]);
+  for (int q = threadIdx.x; q < np; q += NT) st<double>(ug, unsigned(q) * 8u, su[q]);
   if (threadIdx.x == 0) {
     *It.norm = (unsigned long long)__double_as_longlong(cur);
     *It.iters = it;
@@ -2481,14 +2481,16 @@ bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_
                          int eq, double eps, int max_iter) {
   const size_t lds = 2 * sizeof(double) * size_t(max_pts);  // max_pts: largest B.size
   // max_int: largest number of points of one colour slot list, ceil(ni/2) nj nk
-  if (lds + 2048 > size_t(kLdsLimit) || max_int > kCoarseThreads * 8) return false;
+  // up to 4 points of a colour per thread: 512 threads, or 1024 for
+  // larger boxes (the 21^3 coarsest box of N=161)
+  if (lds + 2048 > size_t(kLdsLimit) || max_int > 2 * kCoarseThreads * 4) return false;
   if (n == 0) return true;
   const bool small = max_int <= kCoarseThreads * 4;
 #define OLL_COARSE(E)                                                                              \
   if (small)                                                                                       \
     hipLaunchKernelGGL((k_coarse_solve<E, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter); \
   else                                                                                             \
-    hipLaunchKernelGGL((k_coarse_solve<E, 8>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter);
+    hipLaunchKernelGGL((k_coarse_solve<E, 4, 2 * kCoarseThreads>), dim3(n), dim3(2 * kCoarseThreads), lds, s, items, eps, max_iter);
   switch (eq) {
     case 1: OLL_COARSE(1) break;
     case 2: OLL_COARSE(2) break;
@@ -2509,15 +2511,15 @@ void kernels_init() {
   static bool done = false;
   if (done) return;
   const void *fns[] = {reinterpret_cast<const void *>(&k_coarse_solve<0, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<0, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<0, 4, 2 * kCoarseThreads>),
                        reinterpret_cast<const void *>(&k_coarse_solve<1, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<1, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<1, 4, 2 * kCoarseThreads>),
                        reinterpret_cast<const void *>(&k_coarse_solve<2, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<2, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<2, 4, 2 * kCoarseThreads>),
                        reinterpret_cast<const void *>(&k_coarse_solve<3, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<3, 8>),
+                       reinterpret_cast<const void *>(&k_coarse_solve<3, 4, 2 * kCoarseThreads>),
                        reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 8>)};
+                       reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 4, 2 * kCoarseThreads>)};
   for (const void *f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
   done = true;
