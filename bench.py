@@ -301,8 +301,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the served stream shards across "
-              "GPUs with no exchange (see DESIGN.md), so no multi-GPU figure is produced",
-              file=sys.stderr)
+              "GPUs with no exchange, so no multi-GPU figure is produced", file=sys.stderr)
         sys.exit(2)
 
     import ollipticid_amd as oa
