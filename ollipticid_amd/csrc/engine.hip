@@ -105,12 +105,24 @@ struct Pools {
   Hier::XRef add(const Hierarchy &g, int tbox, int slevel, int method, int ord) {
     Hier::XRef r;
     r.off = int(xs.size());
+    for (int a = 0; a < 3; ++a) {
+      r.lo[a] = 1 << 30;
+      r.hi[a] = 0;
+    }
     const int fs = g.level_first[size_t(slevel)], ns = g.level_count[size_t(slevel)];
     for (int s = 0; s < ns; ++s) {
       X x{};
       x.src = fs + s;
+      int slo[3], shi[3];
       for (int a = 0; a < 3; ++a) {
         const AxisTable t = build_axis_table(g.boxes[size_t(tbox)], g.boxes[size_t(fs + s)], a, method, ord);
+        slo[a] = int(t.flags.size());
+        shi[a] = 0;
+        for (int q = 0; q < int(t.flags.size()); ++q)
+          if (t.flags[size_t(q)] & AX_NB) {
+            slo[a] = std::min(slo[a], q);
+            shi[a] = q + 1;
+          }
         x.start[a] = int(ip.size());
         ip.insert(ip.end(), t.start.begin(), t.start.end());
         x.coinc[a] = int(ip.size());
@@ -130,6 +142,11 @@ struct Pools {
       }
       x.affine = (method == 0 && x.affine == 0) ? 1 : 0;
       xs.push_back(x);
+      if (slo[0] < shi[0] && slo[1] < shi[1] && slo[2] < shi[2])
+        for (int a = 0; a < 3; ++a) {
+          r.lo[a] = std::min(r.lo[a], slo[a]);
+          r.hi[a] = std::max(r.hi[a], shi[a]);
+        }
     }
     r.cnt = int(xs.size()) - r.off;
     return r;
@@ -683,6 +700,27 @@ struct Builder {
     }
   }
 
+  // point blocks over the rows of the region of target points inside some
+  // source's non-ghost part (XRef lo, hi): one block per row segment
+  static void segments(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
+                       std::vector<PtBlock> &out) {
+    const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
+    for (int t = 0; t < n; ++t) {
+      const DevBox &B = H.hbox[size_t(f + t)];
+      const Hier::XRef r = X[size_t(k)][size_t(t)];
+      const int ilo = std::max(r.lo[0], B.R), ihi = std::min(r.hi[0], B.n[0] - B.R);
+      if (ilo >= ihi) continue;
+      const long long plane = (long long)B.n[0] * B.n[1];
+      for (int kk = std::max(r.lo[2], B.R); kk < std::min(r.hi[2], B.n[2] - B.R); ++kk)
+        for (int j = std::max(r.lo[1], B.R); j < std::min(r.hi[1], B.n[1] - B.R); ++j)
+          for (int i0 = ilo; i0 < ihi; i0 += kPtThreads) {
+            const long long q0 = kk * plane + (long long)j * B.n[0] + i0;
+            out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, 0, unsigned(q0),
+                                  unsigned(std::min(kPtThreads, ihi - i0)), i0, j, kk});
+          }
+    }
+  }
+
   void transfer(Launch::Kind kind, const std::vector<Hier *> &M, const std::vector<int> &lv) {
     // lv[h] = the member's fine level k of the pair (k-1, k); restrictions
     // are grouped by the coarse level's (order, eq), ghost fills by the
@@ -693,7 +731,12 @@ struct Builder {
       if (k < 1) continue;
       const Hier &H = *M[h];
       const int mg = k <= H.geo.mgrid;
-      if (kind == Launch::RESTRICT_U) points(H, k - 1, k, H.ru, false, mg, groups[{0, 0}]);
+      if (kind == Launch::RESTRICT_U) {
+        if (mg)
+          points(H, k - 1, k, H.ru, false, mg, groups[{0, 0}]);
+        else
+          segments(H, k - 1, k, H.ru, groups[{0, 0}]);
+      }
       if (kind == Launch::RESTRICT_DEFECT)
         points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}]);
       if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, groups[{0, 0}]);

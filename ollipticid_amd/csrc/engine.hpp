@@ -36,6 +36,9 @@ struct Hier {
   size_t bytes = 0;              // HBM held
   struct XRef {
     int off = 0, cnt = 0;        // into dxfer
+    // index bounding box [lo, hi) of the target points inside some
+    // source's non-ghost part (empty when lo >= hi on an axis)
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   };
   // [fine level k][target box within its level]
   std::vector<std::vector<XRef>> ru, rw, pv, gf;
