@@ -101,6 +101,9 @@ struct Pools {
   };
 
   std::vector<X> xs;
+  // per source box: bounding box [lo, hi) of the nodes linear transfers
+  // (prolongations) read from it, with one node of margin
+  std::map<int, std::array<int, 6>> src_nodes;
 
   Hier::XRef add(const Hierarchy &g, int tbox, int slevel, int method, int ord) {
     Hier::XRef r;
@@ -123,6 +126,24 @@ struct Pools {
             slo[a] = std::min(slo[a], q);
             shi[a] = q + 1;
           }
+        if (method == 0) {
+          const Box &S = g.boxes[size_t(fs + s)];
+          int lo = 1 << 30, hi = -1;
+          for (int q = 0; q < int(t.flags.size()); ++q) {
+            const uint8_t fq = t.flags[size_t(q)];
+            if (!(fq & (AX_NB | AX_IN))) continue;
+            const int n0 = (fq & AX_COINC) ? t.coinc[size_t(q)] : t.start[size_t(q)];
+            lo = std::min(lo, n0 - 1);
+            hi = std::max(hi, n0 + 3);
+          }
+          if (hi > lo) {
+            auto it = src_nodes.find(fs + s);
+            if (it == src_nodes.end())
+              it = src_nodes.emplace(fs + s, std::array<int, 6>{1 << 30, 1 << 30, 1 << 30, 0, 0, 0}).first;
+            it->second[size_t(a)] = std::max(0, std::min(it->second[size_t(a)], lo));
+            it->second[size_t(3 + a)] = std::min(int(S.n[a]), std::max(it->second[size_t(3 + a)], hi));
+          }
+        }
         x.start[a] = int(ip.size());
         ip.insert(ip.end(), t.start.begin(), t.start.end());
         x.coinc[a] = int(ip.size());
@@ -300,6 +321,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
       H.gf[size_t(k)].push_back(P.add(g, tb, k - 1, 1, p.order + 1));
     }
   }
+  H.prolong_nodes.assign(g.boxes.size(), std::array<int, 6>{1 << 30, 1 << 30, 1 << 30, 0, 0, 0});
+  for (const auto &kv : P.src_nodes) H.prolong_nodes[size_t(kv.first)] = kv.second;
   if ((rc = upload(&H.ipool, P.ip, H.bytes, err))) return rc;
   if ((rc = upload(&H.bpool, P.bp, H.bytes, err))) return rc;
   if ((rc = upload(&H.dpool, P.dp, H.bytes, err))) return rc;
@@ -700,19 +723,40 @@ struct Builder {
     }
   }
 
-  // point blocks over the rows of the region of target points inside some
-  // source's non-ghost part (XRef lo, hi): one block per row segment
+  // point blocks, one per row segment, over the region of a refinement
+  // pair's coarse box where a restriction acts: the interior points inside
+  // some fine box's non-ghost part (XRef lo, hi), and with `ww` also the
+  // nodes the pair's prolongation reads back (u - ww: k_restrict_defect
+  // saves ww there; elsewhere no kernel reads ww)
   static void segments(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
-                       std::vector<PtBlock> &out) {
+                       std::vector<PtBlock> &out, bool ww) {
     const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
     for (int t = 0; t < n; ++t) {
       const DevBox &B = H.hbox[size_t(f + t)];
       const Hier::XRef r = X[size_t(k)][size_t(t)];
-      const int ilo = std::max(r.lo[0], B.R), ihi = std::min(r.hi[0], B.n[0] - B.R);
-      if (ilo >= ihi) continue;
+      int lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::max(r.lo[a], B.R);
+        hi[a] = std::min(r.hi[a], B.n[a] - B.R);
+      }
+      if (ww) {
+        const std::array<int, 6> &pn = H.prolong_nodes[size_t(f + t)];
+        if (pn[0] < pn[3] && pn[1] < pn[4] && pn[2] < pn[5])
+          for (int a = 0; a < 3; ++a) {
+            if (lo[a] >= hi[a]) {
+              lo[a] = pn[size_t(a)];
+              hi[a] = pn[size_t(3 + a)];
+            } else {
+              lo[a] = std::min(lo[a], pn[size_t(a)]);
+              hi[a] = std::max(hi[a], pn[size_t(3 + a)]);
+            }
+          }
+      }
+      const int ilo = lo[0], ihi = hi[0];
+      if (ilo >= ihi || lo[1] >= hi[1] || lo[2] >= hi[2]) continue;
       const long long plane = (long long)B.n[0] * B.n[1];
-      for (int kk = std::max(r.lo[2], B.R); kk < std::min(r.hi[2], B.n[2] - B.R); ++kk)
-        for (int j = std::max(r.lo[1], B.R); j < std::min(r.hi[1], B.n[1] - B.R); ++j)
+      for (int kk = lo[2]; kk < hi[2]; ++kk)
+        for (int j = lo[1]; j < hi[1]; ++j)
           for (int i0 = ilo; i0 < ihi; i0 += kPtThreads) {
             const long long q0 = kk * plane + (long long)j * B.n[0] + i0;
             out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, 0, unsigned(q0),
@@ -735,10 +779,14 @@ struct Builder {
         if (mg)
           points(H, k - 1, k, H.ru, false, mg, groups[{0, 0}]);
         else
-          segments(H, k - 1, k, H.ru, groups[{0, 0}]);
+          segments(H, k - 1, k, H.ru, groups[{0, 0}], false);
       }
-      if (kind == Launch::RESTRICT_DEFECT)
-        points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}]);
+      if (kind == Launch::RESTRICT_DEFECT) {
+        if (mg)
+          points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}]);
+        else
+          segments(H, k - 1, k, H.rw, groups[{H.level_order(k - 1), H.prob.eq}], true);
+      }
       if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, groups[{0, 0}]);
       if (kind == Launch::GHOST && !mg)
         points(H, k, k, H.gf, true, 0, groups[{H.level_order(k), 0}],

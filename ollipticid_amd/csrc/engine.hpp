@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -42,6 +43,9 @@ struct Hier {
   };
   // [fine level k][target box within its level]
   std::vector<std::vector<XRef>> ru, rw, pv, gf;
+  // per box: bounding box {lo[3], hi[3]} of the nodes prolongations read
+  // from it (u and ww; one node of margin), empty when lo >= hi
+  std::vector<std::array<int, 6>> prolong_nodes;
   // per box: source of a defect restriction off the coarser lattice, whose
   // defect is filtered before the restriction (k_filter_defect)
   std::vector<char> filt_src;
