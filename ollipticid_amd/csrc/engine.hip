@@ -282,11 +282,9 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     int rows = 0;
     for (int b = 0; b < g.level_count[size_t(l)]; ++b) {
       const DevBox &X = H.hbox[size_t(g.level_first[size_t(l)] + b)];
-      rows += interior_rows(X);
-      // a workgroup of the row kernels holds at least one whole row
-      if (X.rl > kRowThreads)
-        return err = "a box row exceeds 256 points of one colour (at most 256 x colours points along x)",
-               OLL_EINVAL;
+      // rows longer than a launch's threads are split into segments (64
+      // slots at the smallest launch shape)
+      rows += interior_rows(X) * ((X.rl + 63) / 64);
       // points updated by colour pass q (flipped boxes update colour nc-1-q)
       for (int k = X.R; k < X.n[2] - X.R; ++k)
         for (int j = X.R; j < X.n[1] - X.R; ++j)
@@ -505,15 +503,20 @@ struct Builder {
 
   // row blocks of one level of one member appended to `out`: the interior
   // rows (or colour rows) of each box in runs of kRowThreads / pr
+  // (pr: the level's longest row in threads; a launch runs min(pr,
+  // threads) threads per row, splitting longer rows into segments)
   static void row_blocks(const Hier &H, int level, unsigned long long *norm, bool subrows, int pr,
                          std::vector<RowBlock> &out, int threads = kRowThreads) {
-    const int rpb = std::max(1, threads / pr);
+    const int seg = std::min(pr, threads);
+    const int rpb = std::max(1, threads / seg);
     const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
     for (int b = f; b < f + n; ++b) {
       const DevBox &B = H.hbox[size_t(b)];
-      const int rows = interior_rows(B) * (subrows ? B.nc : 1);
+      const int len = subrows ? B.rl : (B.n[0] - 2 * B.R + B.nc - 1) / B.nc;
+      const int nseg = (len + seg - 1) / seg;
+      const int rows = interior_rows(B) * (subrows ? B.nc : 1) * nseg;
       for (int r0 = 0; r0 < rows; r0 += rpb)
-        out.push_back(RowBlock{H.dbox + b, norm, r0, std::min(rpb, rows - r0)});
+        out.push_back(RowBlock{H.dbox + b, norm, r0, std::min(rpb, rows - r0), nseg});
     }
   }
 
@@ -634,7 +637,7 @@ struct Builder {
         L.eq = eq;
         L.off = soff;
         L.n = int(sb.size());
-        L.max_row = ps;
+        L.max_row = std::min(ps, kRowThreads);
         S.launches.push_back(L);
       };
       if (mixed) whole(Launch::DEFECT32);
@@ -646,7 +649,7 @@ struct Builder {
           L.colour = q;
           L.off = coff;
           L.n = int(cb.size());
-          L.max_row = pc;
+          L.max_row = std::min(pc, threads);
           L.threads = threads;
           L.bytes = bytes[q];
           S.launches.push_back(L);
@@ -675,7 +678,7 @@ struct Builder {
       L.mode = write_defect;
       L.off = S.add(blocks);
       L.n = int(blocks.size());
-      L.max_row = ps;
+      L.max_row = std::min(ps, kRowThreads);
       S.launches.push_back(L);
     }
   }

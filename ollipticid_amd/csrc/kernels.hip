@@ -352,9 +352,16 @@ __device__ __forceinline__ void row_jk(ConstBox &B, unsigned r, int &j, int &k) 
 // interior colour points
 template <int R, int NC = R + 1>
 __device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, unsigned lr, int t,
-                                            int colour, unsigned &p, int &s, int &i, int &j, int &k) {
+                                            unsigned per_row, int colour, unsigned &p, int &s,
+                                            int &i, int &j, int &k) {
   if (lr >= unsigned(Bk.rows)) return false;
-  row_jk<R>(B, unsigned(Bk.row0) + lr, j, k);
+  unsigned row = unsigned(Bk.row0) + lr;
+  if (Bk.nseg > 1) {  // segment `seg` of the row: colour points seg * per_row on
+    const unsigned seg = row % unsigned(Bk.nseg);
+    row /= unsigned(Bk.nseg);
+    t += int(seg * per_row);
+  }
+  row_jk<R>(B, row, j, k);
   // i of the row's first interior point of this colour
   const int first = R + int(unsigned(colour + NC * 3 * kMaxAxisPoints - j - k - R) % NC);
   i = first + t * NC;
@@ -366,16 +373,22 @@ __device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, uns
 }
 
 // whole-row kernels: sub-row sr = NC * row + c is colour c of interior row
-// `row`; the thread owns slot t of it.  Element index p, colour c and
-// i mod NC; false for ghost and padding slots
+// `row` (with segments, sr * nseg + segment); the thread owns slot t of it
+// (of the segment).  Element index p, colour c and i mod NC; false for
+// ghost and padding slots
 template <int R, int NC = R + 1>
 __device__ __forceinline__ bool row_slot(const RowBlock &Bk, ConstBox &B, unsigned per_row,
                                          float inv_per_row, unsigned &p, int &c, int &s, int &i,
                                          int &j, int &k) {
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
-  const int t = int(threadIdx.x - lr * per_row);
+  int t = int(threadIdx.x - lr * per_row);
   if (lr >= unsigned(Bk.rows)) return false;
-  const unsigned sr = unsigned(Bk.row0) + lr;
+  unsigned sr = unsigned(Bk.row0) + lr;
+  if (Bk.nseg > 1) {  // segment `seg` of the colour row: slots seg * per_row on
+    const unsigned seg = sr % unsigned(Bk.nseg);
+    sr /= unsigned(Bk.nseg);
+    t += int(seg * per_row);
+  }
   const unsigned row = sr / NC;
   c = int(sr - row * NC);
   row_jk<R>(B, row, j, k);
@@ -414,7 +427,7 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__res
   unsigned p;
   int s, i, j, k;
   const int col = B.flip ? NC - 1 - colour : colour;
-  if (colour_slot<R, NC>(Bk, B, lr, t, col, p, s, i, j, k)) {
+  if (colour_slot<R, NC>(Bk, B, lr, t, per_row, col, p, s, i, j, k)) {
     const Nbr<NC> nb = box_offs<NC, 8>(B, col, s);
     const unsigned o = p * 8u;
     double *u = B.f[F_U];
@@ -617,7 +630,7 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth32(const RowBlock *__r
   unsigned p;
   int s, i, j, k;
   const int col = B.flip ? NC - 1 - colour : colour;
-  if (!colour_slot<R>(Bk, B, lr, t, col, p, s, i, j, k)) return;
+  if (!colour_slot<R>(Bk, B, lr, t, per_row, col, p, s, i, j, k)) return;
   const Nbr<NC> nb = box_offs<NC, 4>(B, col, s);
   const unsigned o = p * 4u;
   float *e = B.g[G_E];

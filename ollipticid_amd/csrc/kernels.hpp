@@ -183,6 +183,10 @@ struct RowBlock {
   unsigned long long *norm;  // null: no norm
   int row0;
   int rows;
+  // segments per row: a row longer than the launch's threads per row is
+  // split into nseg segments of that many slots, each a row of its own in
+  // the block numbering (row index = row * nseg + segment)
+  int nseg;
 };
 constexpr int kRowThreads = 256;  // threads per row workgroup
 // the colour passes (k_smooth, k_smooth32) take 64..kMaxRowThreads threads

@@ -60,3 +60,52 @@ def test_shared_loads_equal_points(name):
     assert got[:2] == ref[:2], (got[:2], ref[:2])
     for ib, (a, b) in enumerate(zip(got[2], ref[2])):
         assert np.array_equal(a, b), (name, ib, float(np.abs(a - b).max()))
+
+
+LONG_ROWS = ["-eq", "Punctures", "-B", "Robin", "-BA", "0", "-Bn", "1", "-NumBH", "1",
+             "-bhmp1", "0.5", "-bhx1", "0.3", "-bhpy1", "0.1", "-L", "64", "-N", "513", "-l", "1",
+             "-O", "2", "-nu2", "2", "-s", "1e-10"]
+
+
+def test_long_rows_match_oracle():
+    """A 513^3 2nd-order base box: its colour rows hold 257 points, more than
+    a whole-row workgroup's 256 threads (2 segments per row), and with 64
+    threads per colour-pass workgroup its sweeps run 4 segments per row.
+    The residual on every point and two sweeps against the oracle's
+    multi-colour order (1e-12 relative), the segmented sweeps against the
+    unsegmented ones bit for bit."""
+    from oracle import oracle as O
+    from tests.test_gpu_parity import rel_err, seed_state
+
+    before = oa.sweep_threads(oa.PREC_FP64, 2)
+    print("case: long rows", " ".join(LONG_ROWS), flush=True)
+    h = oa.Hierarchy(oa.problem_from_argv(LONG_ROWS))
+    s = O.Solver.from_argv(LONG_ROWS, sweep_order=O.COLOUR)
+    try:
+        top = s.levels - 1
+        ib = s.level_boxes(top)[0]
+        assert h.boxes()[ib]["n"] == [513, 513, 513]
+        seed_state(h, s, 6)
+        ng = h.residual(top)
+        no = s.residual(ib)
+        assert abs(ng - no) <= 1e-12 * no, (ng, no)
+        R = 1
+        g = h.get_field(ib, oa.F_R)[R:-R, R:-R, R:-R]
+        o = s.field(ib, O.F_R)[R:-R, R:-R, R:-R]
+        assert rel_err(g, o) < 1e-12
+        u0 = h.get_field(ib, oa.F_U)
+        oa.set_sweep_threads(oa.PREC_FP64, 2, 256)
+        h.smooth(top, 2)
+        u256 = h.get_field(ib, oa.F_U)
+        h.set_field(ib, oa.F_U, u0)
+        oa.set_sweep_threads(oa.PREC_FP64, 2, 64)
+        ng = h.smooth(top, 2)
+        no = s.smooth(ib, 2)
+        assert abs(ng - no) <= 1e-11 * no, (ng, no)
+        u64 = h.get_field(ib, oa.F_U)
+        assert np.array_equal(u64, u256)
+        assert rel_err(u64, s.field(ib, O.F_U)) < 1e-12
+    finally:
+        oa.set_sweep_threads(oa.PREC_FP64, 2, before)
+        h.close()
+        s.close()
