@@ -32,6 +32,9 @@
 // reference also names each corner, edge and face block of its permuted
 // storage; the solve path uses these three)
 enum t_iterator { ALL, INSIDE, BOUNDARY };
+// interpolation methods (ffunction.h:82-83); transfers take LINEAR and
+// LAGRANGE
+enum m_interpol { LINEAR, LAGRANGE, NEWTON, HERMITE, AVERAGE };
 
 class domain {
  public:
@@ -147,6 +150,12 @@ class multigrid {
   // multigrid.h:98-101: the named field of box `box` of `level` (u, b, c,
   // d, e, rho0, r0, v0, w0, ww0); the multigrid owns it
   ffunction *Get_ffunction(size_t level, size_t box, const std::string &name) const;
+  // multigrid.cc:122-141: field from_var of the boxes of level `from` into
+  // to_var of the boxes of level `to` over `iter`'s points (oll_transfer);
+  // NEWTON, HERMITE and AVERAGE end the program with a message
+  void Transferir(size_t from, std::string from_var, size_t to, std::string to_var,
+                  t_iterator iter = ALL, m_interpol method = LINEAR, size_t ord = 1,
+                  bool all = true, bool to_zero = false);
   // the device state changed: host mirrors are fetched again when read
   void Touch() { ++*gen_; }
   unsigned long Generation() const { return *gen_; }

@@ -203,6 +203,23 @@ int oll_problem_footprint(const oll_problem *p, size_t *bytes);
 int oll_get_field(oll_hier *h, int ibox, int field, double *host);
 int oll_set_field(oll_hier *h, int ibox, int field, const double *host);
 
+/* multigrid::Transferir (multigrid.cc:122-141, ffunction.cc:2436-2475):
+ * field from_field of the boxes of level from_level interpolated into
+ * field to_field of the boxes of level to_level, at the points of `region`
+ * (OLL_REGION_*: all, interior, ghost shell); OLL_INTERP_LINEAR (trilinear)
+ * or OLL_INTERP_LAGRANGE of `order` (2..9, order + 1 nodes per axis; a point
+ * on a source node takes its value).  A point takes the last source box
+ * whose non-ghost part holds it (with `all`, whose extent holds it); with
+ * `to_zero` the last source box alone decides and points it does not hold
+ * are set to 0, as the reference's loop over source boxes leaves them.
+ * Levels beyond the last are clamped to it; a field onto itself on one
+ * level is refused (OLL_EINVAL).  Transfer tables are built for
+ * the call (device memory is allocated and freed). */
+enum { OLL_REGION_ALL = 0, OLL_REGION_INSIDE = 1, OLL_REGION_BOUNDARY = 2 };
+enum { OLL_INTERP_LINEAR = 0, OLL_INTERP_LAGRANGE = 1 };
+int oll_transfer(oll_hier *h, int from_level, int from_field, int to_level, int to_field,
+                 int region, int method, int order, int all, int to_zero);
+
 /* Single stages, all boxes of one level (for tests and tools). */
 /* elliptic.cc:1126-1242 Smooth (multi-colour order); linf = max |N(u)-rho|. */
 int oll_smooth(oll_hier *h, int level, int nu, double *linf);

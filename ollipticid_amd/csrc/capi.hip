@@ -271,6 +271,23 @@ int oll_set_field(oll_hier *h, int ibox, int field, const double *host) {
   return OLL_OK;
 }
 
+int oll_transfer(oll_hier *h, int from_level, int from_field, int to_level, int to_field,
+                 int region, int method, int order, int all, int to_zero) {
+  if (!h) return fail(OLL_EINVAL, "null handle");
+  const int nf = fields_of(h->H.prob.eq);
+  if (from_field < 0 || from_field >= nf || to_field < 0 || to_field >= nf)
+    return fail(OLL_EINVAL, "field out of range");
+  if (from_level < 0 || to_level < 0) return fail(OLL_EINVAL, "level out of range");
+  if (region < OLL_REGION_ALL || region > OLL_REGION_BOUNDARY)
+    return fail(OLL_EINVAL, "region must be OLL_REGION_ALL, _INSIDE or _BOUNDARY");
+  if (method != OLL_INTERP_LINEAR && method != OLL_INTERP_LAGRANGE)
+    return fail(OLL_EINVAL, "method must be OLL_INTERP_LINEAR or OLL_INTERP_LAGRANGE");
+  std::string err;
+  const int rc = transfer_fields(h->H, h->stream, from_level, from_field, to_level, to_field, region,
+                                 method, order, all, to_zero, err);
+  return rc ? fail(rc, err) : OLL_OK;
+}
+
 int oll_smooth(oll_hier *h, int level, int nu, double *linf) {
   int rc = hier_check(h, level);
   if (rc) return rc;

@@ -148,13 +148,37 @@ const domain &multigrid::Get_domain(size_t level, size_t box) const {
   return dom_[Get_dom_index(level, box)];
 }
 
-ffunction *multigrid::Get_ffunction(size_t level, size_t box, const std::string &name) const {
+// field id of a variable name (operator.cc:103-124 names)
+static int field_id(const std::string &name) {
   static const char *names[OLL_NFIELDS] = {"u", "b", "c", "d", "e", "rho0", "r0", "v0", "w0", "ww0"};
   int f = -1;
   for (int i = 0; i < OLL_NFIELDS; ++i)
     if (name == names[i]) f = i;
   if (name == "rho") f = OLL_F_RHO;
   if (f < 0) throw std::runtime_error("unknown field " + name);
+  return f;
+}
+
+void multigrid::Transferir(size_t from, std::string from_var, size_t to, std::string to_var,
+                           t_iterator iter, m_interpol method, size_t ord, bool all,
+                           bool to_zero) {
+  if (method != LINEAR && method != LAGRANGE) {
+    std::cerr << "Transferir: only LINEAR and LAGRANGE interpolation are available" << std::endl;
+    std::exit(1);
+  }
+  const int region = iter == ALL ? OLL_REGION_ALL : iter == INSIDE ? OLL_REGION_INSIDE
+                                                                   : OLL_REGION_BOUNDARY;
+  if (oll_transfer(h_, int(from), field_id(from_var), int(to), field_id(to_var), region,
+                   method == LINEAR ? OLL_INTERP_LINEAR : OLL_INTERP_LAGRANGE, int(ord),
+                   all ? 1 : 0, to_zero ? 1 : 0) != OLL_OK) {
+    std::cerr << "Transferir: " << oll_last_error() << std::endl;
+    std::exit(1);
+  }
+  Touch();
+}
+
+ffunction *multigrid::Get_ffunction(size_t level, size_t box, const std::string &name) const {
+  const int f = field_id(name);
   const size_t ib = Get_dom_index(level, box);
   std::unique_ptr<ffunction> &p = fields_[{int(ib), f}];
   if (!p) p.reset(new ffunction(h_, int(ib), f, dom_[ib], gen_));

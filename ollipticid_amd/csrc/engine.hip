@@ -418,6 +418,71 @@ void hier_release(Hier &H) {
 // schedules
 // ---------------------------------------------------------------------
 
+int transfer_fields(Hier &H, hipStream_t s, int from, int ffield, int to, int tfield, int region,
+                    int method, int ord, int all, int to_zero, std::string &err) {
+  const Hierarchy &g = H.geo;
+  const int L = g.levels;
+  from = std::max(0, std::min(from, L - 1));
+  to = std::max(0, std::min(to, L - 1));
+  // the kernel reads the source while it writes the target: a field onto
+  // itself on one level would depend on the order of the points
+  if (from == to && ffield == tfield)
+    return err = "a field cannot be transferred onto itself on one level", OLL_EINVAL;
+  if (method == 1 && (ord < 2 || ord > 9))
+    return err = "Lagrange transfers take order 2..9", OLL_EINVAL;
+  const int nodes = method == 0 ? 2 : ord + 1;
+  Pools P;
+  std::vector<Hier::XRef> refs;
+  const int ft = g.level_first[size_t(to)], nt = g.level_count[size_t(to)];
+  for (int t = 0; t < nt; ++t) refs.push_back(P.add(g, ft + t, from, method, method == 0 ? 1 : ord));
+  int *ip = nullptr;
+  unsigned char *bp = nullptr;
+  double *dp = nullptr;
+  DevXfer *dx = nullptr;
+  PtBlock *db = nullptr;
+  size_t bytes = 0;
+  int rc = upload(&ip, P.ip, bytes, err);
+  if (!rc) rc = upload(&bp, P.bp, bytes, err);
+  if (!rc) rc = upload(&dp, P.dp, bytes, err);
+  std::vector<DevXfer> xf;
+  for (const Pools::X &x : P.xs) {
+    DevXfer d{};
+    d.src = H.dbox + x.src;
+    for (int a = 0; a < 3; ++a) {
+      d.ax[a].start = ip + x.start[a];
+      d.ax[a].coinc = ip + x.coinc[a];
+      d.ax[a].flags = bp + x.flags[a];
+      d.ax[a].w = dp + x.w[a];
+      d.ax[a].k = x.k[a];
+      d.ax[a].n = x.n[a];
+      d.aff[a] = x.aff[a];
+    }
+    d.affine = x.affine;
+    xf.push_back(d);
+  }
+  if (!rc) rc = upload(&dx, xf, bytes, err);
+  std::vector<PtBlock> blocks;
+  const int mode = (region & 3) | (all ? 4 : 0) | (to_zero ? 8 : 0);
+  for (int t = 0; t < nt; ++t) {
+    const DevBox &B = H.hbox[size_t(ft + t)];
+    const long long np = npts(B), plane = (long long)B.n[0] * B.n[1];
+    for (long long q0 = 0; q0 < np; q0 += kPtThreads)
+      blocks.push_back(PtBlock{H.dbox + ft + t, dx + refs[size_t(t)].off, refs[size_t(t)].cnt, mode,
+                               unsigned(q0), unsigned(std::min<long long>(kPtThreads, np - q0)),
+                               int(q0 % B.n[0]), int((q0 % plane) / B.n[0]), int(q0 / plane)});
+  }
+  if (!rc) rc = upload(&db, blocks, bytes, err);
+  if (!rc && !launch_transfer_fields(s, db, int(blocks.size()), ffield, tfield, nodes))
+    rc = (err = "unsupported interpolation", OLL_EINVAL);
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = (err = "transfer failed", OLL_EDEVICE);
+  (void)hipFree(ip);
+  (void)hipFree(bp);
+  (void)hipFree(dp);
+  (void)hipFree(dx);
+  (void)hipFree(db);
+  return rc;
+}
+
 // fused double-buffered sweeps (k_smooth_fused): off by default.  On
 // MI355X they run slower than the colour-by-colour passes at every stencil
 // order and tile shape measured (4th order, N=65: 1.1-3x; 6th order,

@@ -140,6 +140,13 @@ int build_vcycle(const std::vector<Hier *> &members, Schedule &S, std::string &e
 int build_stage(Hier &H, int level, int stage, int nu, Schedule &S, std::string &err, int box = -1);
 enum Stage { STAGE_SMOOTH = 0, STAGE_RESIDUAL, STAGE_ROBIN, STAGE_BOUNDARY_OP, STAGE_DERIVATIVE };
 
+// multigrid::Transferir (multigrid.cc:122-141): field ffield of the boxes
+// of level `from` into field tfield of the boxes of level `to` over region
+// (0 all, 1 interior, 2 ghost shell), linear (method 0) or Lagrange of
+// order ord (method 1, ord + 1 nodes per axis, ord 2..9); transfer tables
+// are built for the call and freed after it
+int transfer_fields(Hier &H, hipStream_t s, int from, int ffield, int to, int tfield, int region,
+                    int method, int ord, int all, int to_zero, std::string &err);
 // colour order of box b (0: ascending, 1: descending) for nc colours
 int colour_flip(const Hierarchy &g, const Box &b, int nc);
 // FP64 sweeps of the puncture, scalar-field, Poisson and Brill equations

@@ -1107,6 +1107,85 @@ __device__ __forceinline__ double trilinear_nested(ConstBox &S, ConstXfer &X, co
 #undef V
 }
 
+// trilinear interpolation of field f of box S (interpolation.cc:125-182),
+// the corners walked as in trilinear_corr
+__device__ __forceinline__ double trilinear_field(ConstBox &S, const void *f, ConstXfer &X,
+                                                  const int t[3]) {
+  const int x0 = X.ax[0].start[t[0]], y0 = X.ax[1].start[t[1]], z0 = X.ax[2].start[t[2]];
+  const double *wx = X.ax[0].w + 2 * t[0];
+  const double *wy = X.ax[1].w + 2 * t[1];
+  const double *wzp = X.ax[2].w + 2 * t[2];
+  const double dxi = wx[0], dxf = wx[1], dyi = wy[0], dyf = wy[1], dzi = wzp[0], dzf = wzp[1];
+  const Walk wk(S);
+  const unsigned c0 = colourc(S, x0, y0, z0);
+  unsigned r = unsigned(x0) - divc(S, unsigned(x0)) * wk.nc;
+  const unsigned o000 = boff(S, x0, y0, z0);
+  unsigned c = c0;
+  const unsigned o100 = o000 + wk.x(c, r);
+  unsigned c1 = c;
+  const unsigned o110 = o100 + wk.y(c1);
+  unsigned c2 = c1;
+  const unsigned o111 = o110 + wk.z(c2);
+  unsigned cb = c;
+  const unsigned o101 = o100 + wk.z(cb);
+  unsigned cy = c0;
+  const unsigned o010 = o000 + wk.y(cy);
+  unsigned cz = cy;
+  const unsigned o011 = o010 + wk.z(cz);
+  unsigned cw = c0;
+  const unsigned o001 = o000 + wk.z(cw);
+#define V(o) ld<double>(f, o)
+  return ((V(o000) * dxi + V(o100) * dxf) * dyi + (V(o010) * dxi + V(o110) * dxf) * dyf) * dzi +
+         ((V(o001) * dxi + V(o101) * dxf) * dyi + (V(o011) * dxi + V(o111) * dxf) * dyf) * dzf;
+#undef V
+}
+
+// multigrid::Transferir (multigrid.cc:122-141, ffunction.cc:2436-2475):
+// every point of the region (PtBlock::mg bits 0-1: 0 all, 1 interior, 2
+// ghost shell) of the listed target boxes takes field ff of the source
+// boxes interpolated at its coordinates, linear (KP = 2) or Lagrange with
+// KP nodes per axis: the last source whose non-ghost part holds the point
+// (or, with `all` (bit 2), whose extent does); with `to_zero` (bit 3) the
+// last source alone decides and a point it does not hold is set to 0 (the
+// reference's source loop overwrites every point on each pass)
+template <int KP>
+__global__ __launch_bounds__(kPtThreads) void k_transfer_fields(const PtBlock *__restrict__ blocks,
+                                                                int ff, int tf) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
+  int i, j, k;
+  if (!pt_lex(Bk, T, i, j, k)) return;
+  const int region = Bk.mg & 3, all = (Bk.mg >> 2) & 1, to_zero = (Bk.mg >> 3) & 1;
+  const bool inside = is_interior(T, i, j, k);
+  if ((region == 1 && !inside) || (region == 2 && inside)) return;
+  const int t[3] = {i, j, k};
+  const unsigned o = boff(T, i, j, k);
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  for (int x = Bk.nsrc - 1; x >= 0; --x) {
+    ConstXfer &X = xs[x];
+    unsigned fl[3];
+    const unsigned f = axis_flags(X, t, fl);
+    if ((f & 1) || (all && (f & 2))) {
+      ConstBox &S = cbox(X.src);
+      double v;
+      if constexpr (KP == 2) {
+        if (f & 4)  // a source node: its value (interpolation.cc:140-147)
+          v = ld<double>(S.f[ff], boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]));
+        else
+          v = trilinear_field(S, S.f[ff], X, t);
+      } else {
+        v = tinterp<KP>(S, S.f[ff], X, t, fl);
+      }
+      st<double>(T.f[tf], o, v);
+      return;
+    }
+    if (to_zero) {
+      st<double>(T.f[tf], o, 0.0);
+      return;
+    }
+  }
+}
+
 // u restriction: coarse INSIDE points inside a fine box's non-ghost part
 // (all = false), injection at coincident points (Lagrange 3 elsewhere)
 __global__ __launch_bounds__(kPtThreads) void k_restrict_u(const PtBlock *__restrict__ blocks) {
@@ -2512,6 +2591,26 @@ bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_
     default: OLL_COARSE(0) break;
   }
 #undef OLL_COARSE
+  return true;
+}
+
+bool launch_transfer_fields(hipStream_t s, const PtBlock *blocks, int nblocks, int ff, int tf,
+                            int nodes) {
+  if (nblocks == 0) return true;
+#define OLL_TF(KP) hipLaunchKernelGGL(k_transfer_fields<KP>, dim3(nblocks), dim3(kPtThreads), 0, s, blocks, ff, tf)
+  switch (nodes) {
+    case 2: OLL_TF(2); break;
+    case 3: OLL_TF(3); break;
+    case 4: OLL_TF(4); break;
+    case 5: OLL_TF(5); break;
+    case 6: OLL_TF(6); break;
+    case 7: OLL_TF(7); break;
+    case 8: OLL_TF(8); break;
+    case 9: OLL_TF(9); break;
+    case 10: OLL_TF(10); break;
+    default: return false;
+  }
+#undef OLL_TF
   return true;
 }
 

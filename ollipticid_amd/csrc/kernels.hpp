@@ -409,6 +409,13 @@ void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq);
+// generic transfer of field ff of the listed blocks' sources into field
+// tf of their targets (multigrid::Transferir): PtBlock::mg packs the
+// region (bits 0-1), `all` (bit 2) and `to_zero` (bit 3); nodes = 2 for
+// linear interpolation, else the Lagrange nodes per axis (3..10); false
+// for other node counts
+bool launch_transfer_fields(hipStream_t s, const PtBlock *blocks, int nblocks, int ff, int tf,
+                            int nodes);
 // v = full-weighting filter of the defect w (all points of the listed
 // boxes); runs = 1: blocks over runs of filter_run_length() points along x
 // (ceil(n0 / length) runs per row), 0: blocks over points

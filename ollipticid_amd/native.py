@@ -117,6 +117,7 @@ SIGNATURES = {
     "oll_get_sweep_threads": (C.c_int, [C.c_int, C.c_int]),
     "oll_set_fused_sweeps": (C.c_int, [C.c_int]),
     "oll_get_fused_sweeps": (C.c_int, []),
+    "oll_transfer": (C.c_int, [_vp] + [C.c_int] * 9),
     "oll_set_shared_loads": (C.c_int, [C.c_int]),
     "oll_get_shared_loads": (C.c_int, []),
     "oll_set_fused_shape": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -427,6 +428,14 @@ class Hierarchy:
         out = C.c_double()
         _check(self.L.oll_operator(self.h, level, int(bool(derivative)), C.byref(out)))
         return out.value
+
+    def transfer(self, from_level, from_field, to_level, to_field, region=0, method=0, order=1,
+                 all_=True, to_zero=False):
+        """multigrid::Transferir: field from_field of level from_level into
+        to_field of level to_level (region 0 all, 1 interior, 2 ghost shell;
+        method 0 linear, 1 Lagrange of `order`)."""
+        _check(self.L.oll_transfer(self.h, from_level, from_field, to_level, to_field, region,
+                                   method, order, 1 if all_ else 0, 1 if to_zero else 0))
 
     def fas(self, level):
         out = C.c_double()

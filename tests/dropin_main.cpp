@@ -3,7 +3,8 @@
 // signatures: first main.cc's call sequence (src/main.cc:101-141), then the
 // rest of the elliptic class's hot-path methods (include/elliptic.h:288-345):
 // the operator and its Newton derivative at the solution through the
-// ffunction iterators, Set_Boundary, App_Bound and Vcycle again from u = 0.
+// ffunction iterators, Set_Boundary, App_Bound and Vcycle again from u = 0,
+// then multigrid::Transferir on polynomials its interpolation reproduces.
 // Prints one "dropin:" line per check for the test-suite to read.
 #include <math.h>
 
@@ -61,5 +62,57 @@ int main(int argc, char *argv[]) {
   E.Vcycle();
   E.FullMG();
   cout << scientific << setprecision(17) << "dropin: again " << E.Get_cycles() << " " << E.Get_residual() << endl;
+
+  // multigrid::Transferir with the V-cycle's own patterns: the restriction
+  // of u (INSIDE, LAGRANGE 3, all = false; elliptic.cc:1545) carries a
+  // polynomial of degree 3 per axis over exactly, and the prolongation
+  // (ALL, LINEAR, to_zero; elliptic.cc:1868) a multilinear one
+  auto cubic = [](double x, double y, double z) {
+    return 0.3 + 0.1 * x - 0.2 * y + 0.05 * z + 0.01 * x * y * z + 0.002 * x * x * x -
+           0.001 * y * y * z + 1e-4 * z * z * z;
+  };
+  auto bilinear = [](double x, double y, double z) {
+    return -0.7 + 0.03 * x + 0.02 * y - 0.01 * z + 0.004 * x * y - 0.002 * y * z + 1e-3 * x * y * z;
+  };
+  auto fill = [&](size_t l, const char *name, auto f, double shift) {
+    for (size_t b = 0; b < mg.Get_num_dom(l); ++b) {
+      ffunction *F = mg.Get_ffunction(l, b, name);
+      F->Set_Iterator(ALL);
+      do F->Set(f(F->Get_x(), F->Get_y(), F->Get_z()) + shift); while (F->End());
+      F->Store();
+    }
+  };
+  fill(top, "u", cubic, 0.0);
+  fill(top - 1, "v0", cubic, 1.0);
+  mg.Transferir(top, "u", top - 1, "v0", INSIDE, LAGRANGE, 3, false);
+  size_t moved = 0;
+  double err3 = 0;
+  for (size_t b = 0; b < mg.Get_num_dom(top - 1); ++b) {
+    ffunction *V = mg.Get_ffunction(top - 1, b, "v0");
+    V->Set_Iterator(ALL);
+    do {
+      const double p = cubic(V->Get_x(), V->Get_y(), V->Get_z());
+      const double d = fabs(V->Get_val() - p);
+      if (d < 0.5) {
+        ++moved;
+        err3 = fmax(err3, d);
+      }
+    } while (V->End());
+  }
+  fill(top - 1, "u", bilinear, 0.0);
+  fill(top, "v0", bilinear, 1.0);
+  mg.Transferir(top - 1, "u", top, "v0", ALL, LINEAR, 1, true, true);
+  double err1 = 0;
+  size_t pts1 = 0;
+  for (size_t b = 0; b < mg.Get_num_dom(top); ++b) {
+    ffunction *V = mg.Get_ffunction(top, b, "v0");
+    V->Set_Iterator(ALL);
+    do {
+      err1 = fmax(err1, fabs(V->Get_val() - bilinear(V->Get_x(), V->Get_y(), V->Get_z())));
+      ++pts1;
+    } while (V->End());
+  }
+  cout << scientific << setprecision(17) << "dropin: transfer " << moved << " " << err3 << " "
+       << pts1 << " " << err1 << endl;
   return 0;
 }

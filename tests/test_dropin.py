@@ -5,7 +5,8 @@ and tests/dropin_main.cpp, a host program using the reference's names and
 signatures (interface, elliptic and its Solve, Vcycle, Set_Boundary,
 App_Bound, OPERATOR, duOPERATOR; multigrid::Get_ffunction; the ffunction
 iterators), builds against them; on the GPU it solves, evaluates the
-operator at the solution and solves again from u = 0 to the same result."""
+operator at the solution and solves again from u = 0 to the same result,
+then moves polynomials between levels with multigrid::Transferir."""
 import os
 import subprocess
 
@@ -53,4 +54,10 @@ def test_dropin_program_runs(tmp_path):
     assert float(dn_max) < 0 and float(dn_min) < float(dn_max)
     # Set_Boundary restores u = 0: the second solve is the first, bit for bit
     assert lines["again"] == lines["solve"]
+    # Transferir: restriction (Lagrange 3) of a cubic onto the coarser
+    # level's covered interior points, prolongation (linear, to_zero) of a
+    # multilinear function onto every point of the finest level
+    moved, err3, pts1, err1 = lines["transfer"]
+    assert int(moved) > 0 and float(err3) < 1e-11
+    assert int(pts1) > 0 and float(err1) < 1e-12
     assert (tmp_path / "run" / "Loo.gp").exists()

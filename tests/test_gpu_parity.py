@@ -140,6 +140,46 @@ def test_fas_cycles_match_oracle(case):
         assert np.abs(g - o).max() <= 1e-11 * max(1.0, np.abs(o).max()), (case, ib)
 
 
+TRANSFERS = [
+    # (from, to) relative to the top level, region, method, order, all, to_zero
+    ("restrict u", 0, -1, 1, 1, 3, False, False),
+    ("restrict w", 0, -1, 0, 1, 2, True, False),
+    ("prolong", -1, 0, 0, 0, 1, True, True),
+    ("ghost fill", -1, 0, 2, 1, 5, True, False),
+    ("two levels up", -2, 0, 1, 1, 4, False, True),
+]
+
+
+@pytest.mark.parametrize("case", ["o4_merge", "o2_flip"])
+def test_transfer_matches_oracle(case):
+    """oll_transfer (multigrid::Transferir) against the oracle's transfer
+    for the restriction, prolongation and ghost-fill patterns of the
+    V-cycle and one more: every point of every target box, from a seeded
+    u into a v that starts equal on both sides."""
+    h, s = pair(CASES[case])
+    try:
+        seed_state(h, s, 7)
+        top = s.levels - 1
+        for name, df, dt, region, method, order, all_, to_zero in TRANSFERS:
+            fl, tl = top + df, top + dt
+            if min(fl, tl) < 0:
+                continue
+            print("transfer", case, name, flush=True)
+            for ib in s.level_boxes(tl):
+                shape = s.field(ib, O.F_V).shape
+                v0 = 0.25 + 0.01 * (np.arange(np.prod(shape)) % 7).reshape(shape)
+                s.field(ib, O.F_V)[:] = v0
+                h.set_field(ib, oa.F_V, v0)
+            h.transfer(fl, oa.F_U, tl, oa.F_V, region, method, order, all_, to_zero)
+            s.transfer(fl, O.F_U, tl, O.F_V, region, method, order, int(all_), int(to_zero))
+            for ib in s.level_boxes(tl):
+                g, o = h.get_field(ib, oa.F_V), s.field(ib, O.F_V)
+                assert rel_err(g, o) < 1e-12, (case, name, ib, rel_err(g, o))
+    finally:
+        h.close()
+        s.close()
+
+
 def test_colour_order_matches_oracle():
     """The GPU's per-box colour order is the oracle's, and the 2nd-order case
     has a box swept colour 1 first."""
