@@ -939,6 +939,8 @@ int build_vcycle_from(const std::vector<Hier *> &M, const std::vector<int> &star
     const DevBox &B = H->hbox[size_t(H->geo.level_first[0])];
     if (!coarse_box_fits(B.n, B.size))
       return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;
+    // k_coarse_solve's Robin shell and sweeps are those of a 2nd-order box
+    if (B.nc != 2 || B.R != 1) return err = "coarsest box must be 2nd order", OLL_EINVAL;
   }
   Builder b{S};
   b.zero(M);

@@ -781,8 +781,67 @@ __device__ __forceinline__ int shell_index(int q, int hi, int n, int R) {
   return hi ? n - R + q : R - 1 - q;
 }
 
-// edges then corners of one box, by one block (faces must be done)
-__device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode) {
+// The corners' Robin updates form one chain per corner (each point reads
+// the ones updated before it), so they run from LDS: the whole block stages
+// each corner's cube of kCornerW^3 points (R <= 4 shell layers plus the 4
+// inward points the one-sided derivatives read) and rho of its R^3 points,
+// eight threads walk the chains, and the updated points are written back.
+constexpr int kCornerR = 4;
+constexpr int kCornerW = kCornerR + 4;
+constexpr int kCornerCube = kCornerW * kCornerW * kCornerW;
+constexpr int kCornerR3 = kCornerR * kCornerR * kCornerR;
+
+// box index of local coordinate l (counted inwards from side h) along an axis
+__device__ __forceinline__ int corner_index(int l, int h, int n) { return h ? n - 1 - l : l; }
+
+// one Robin update of corner point (li, lj, lk) (local, counted inwards)
+// of the cube cu of corner (hx, hy, hz): robin_point's arithmetic, with the
+// derivatives' inward neighbours read from the cube
+__device__ __forceinline__ void robin_corner_update(const DevBox &B, double *cu, double rho, int hx,
+                                                    int hy, int hz, int li, int lj, int lk) {
+  const int l[3] = {li, lj, lk}, h[3] = {hx, hy, hz};
+  const int st[3] = {1, kCornerW, kCornerW * kCornerW};
+  const int q = (lk * kCornerW + lj) * kCornerW + li;
+  double X[3];
+  for (int a = 0; a < 3; ++a) X[a] = coord(B, a, corner_index(l[a], h[a], B.n[a]));
+  const double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+  double sF = 0.0, sdF = 0.0;
+  const double u0 = cu[q];
+  for (int a = 0; a < 3; ++a) {
+    const double id = B.idx[a];
+    const double *w = cu + q;
+    const int m = st[a];
+    double cf, dv;
+    // deriv(): dx2 at the box faces for order 2, the one-sided dx4 otherwise
+    if (B.order == 2) {
+      if (!h[a]) {
+        cf = -3.0 * 0.5 * id;
+        dv = (-3.0 * u0 + 4.0 * w[m] - w[2 * m]) * 0.5 * id;
+      } else {
+        cf = 3.0 * 0.5 * id;
+        dv = (3.0 * u0 - 4.0 * w[m] + w[2 * m]) * 0.5 * id;
+      }
+    } else if (!h[a]) {
+      cf = -25.0 * id / 12.0;
+      dv = -(25.0 * u0 - 48.0 * w[m] + 36.0 * w[2 * m] - 16.0 * w[3 * m] + 3.0 * w[4 * m]) * id / 12.0;
+    } else {
+      cf = 25.0 * id / 12.0;
+      dv = (25.0 * u0 - 48.0 * w[m] + 36.0 * w[2 * m] - 16.0 * w[3 * m] + 3.0 * w[4 * m]) * id / 12.0;
+    }
+    sF += dv / X[a];
+    sdF += cf / X[a];
+  }
+  const double op = r2 * B.ipn * sF + u0;
+  const double Fv = op - rho;
+  const double dF = r2 * B.ipn * sdF + 1.0;
+  cu[q] = u0 - Fv / dF;
+}
+
+// edges then corners of one box, by one block (faces must be done);
+// cube and crho: 8 kCornerCube and 8 kCornerR3 doubles of LDS, for the
+// corner chains (R <= kCornerR)
+__device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode, double *cube,
+                                    double *crho) {
   const int R = B.R;
   const int nint[3] = {B.n[0] - 2 * R, B.n[1] - 2 * R, B.n[2] - 2 * R};
   // edges XY(4) run along z, XZ(4) along y, YZ(4) along x; within an edge
@@ -815,15 +874,57 @@ __device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode) {
       }
   }
   __syncthreads();
-  // corners: k outer, j, i inner (domain.cc:592-695)
+  // corners: k outer, j, i inner (domain.cc:592-695); corner c has
+  // hz = bit 0, hx = bit 1, hy = bit 2
+  if (mode != ROBIN_UPDATE) {
+    // the operator modes leave u alone: every corner point at once
+    const int R3 = R * R * R;
+    for (int t = threadIdx.x; t < 8 * R3; t += blockDim.x) {
+      const int c = t / R3, q = t - c * R3;
+      const int hz = c & 1, hx = (c >> 1) & 1, hy = (c >> 2) & 1;
+      const int qk = q / (R * R), qj = (q / R) % R, qi = q % R;
+      robin_point(B, D, shell_index(qi, hx, B.n[0], R), shell_index(qj, hy, B.n[1], R),
+                  shell_index(qk, hz, B.n[2], R), 7, mode);
+    }
+    return;
+  }
+  // stage the eight cubes (points beyond a small box are never read) and
+  // the corner points' rho
+  for (int t = threadIdx.x; t < 8 * kCornerCube; t += blockDim.x) {
+    const int c = t / kCornerCube, q = t - c * kCornerCube;
+    const int hz = c & 1, hx = (c >> 1) & 1, hy = (c >> 2) & 1;
+    const int li = q % kCornerW, lj = (q / kCornerW) % kCornerW, lk = q / (kCornerW * kCornerW);
+    double v = 0.0;
+    if (li < B.n[0] && lj < B.n[1] && lk < B.n[2]) {
+      const long long p = pidx(B, corner_index(li, hx, B.n[0]), corner_index(lj, hy, B.n[1]),
+                               corner_index(lk, hz, B.n[2]));
+      v = D.u[p];
+      if (li < R && lj < R && lk < R) crho[c * kCornerR3 + (lk * kCornerR + lj) * kCornerR + li] = D.rho[p];
+    }
+    cube[t] = v;
+  }
+  __syncthreads();
   if (threadIdx.x < 8) {
     const int c = threadIdx.x;
     const int hz = c & 1, hx = (c >> 1) & 1, hy = (c >> 2) & 1;
+    double *cu = cube + c * kCornerCube;
+    const double *cr = crho + c * kCornerR3;
+    // shell_index(q, h, n, R) is local coordinate R - 1 - q from either side
     for (int qk = 0; qk < R; ++qk)
       for (int qj = 0; qj < R; ++qj)
-        for (int qi = 0; qi < R; ++qi)
-          robin_point(B, D, shell_index(qi, hx, B.n[0], R), shell_index(qj, hy, B.n[1], R),
-                      shell_index(qk, hz, B.n[2], R), 7, mode);
+        for (int qi = 0; qi < R; ++qi) {
+          const int li = R - 1 - qi, lj = R - 1 - qj, lk = R - 1 - qk;
+          robin_corner_update(B, cu, cr[(lk * kCornerR + lj) * kCornerR + li], hx, hy, hz, li, lj, lk);
+        }
+  }
+  __syncthreads();
+  const int R3 = R * R * R;
+  for (int t = threadIdx.x; t < 8 * R3; t += blockDim.x) {
+    const int c = t / R3, q = t - c * R3;
+    const int hz = c & 1, hx = (c >> 1) & 1, hy = (c >> 2) & 1;
+    const int lk = q / (R * R), lj = (q / R) % R, li = q % R;
+    D.u[pidx(B, corner_index(li, hx, B.n[0]), corner_index(lj, hy, B.n[1]), corner_index(lk, hz, B.n[2]))] =
+        cube[c * kCornerCube + (lk * kCornerW + lj) * kCornerW + li];
   }
 }
 
@@ -844,8 +945,9 @@ __global__ __launch_bounds__(256) void k_robin_faces(const RobinItem *__restrict
 // edges and corners: one block per box
 __global__ __launch_bounds__(1024) void k_robin_edges_corners(const RobinItem *__restrict__ items,
                                                                int mode) {
+  __shared__ double cube[8 * kCornerCube], crho[8 * kCornerR3];
   const DevBox &B = *items[blockIdx.x].box;
-  robin_edges_corners(B, box_data(B), mode);
+  robin_edges_corners(B, box_data(B), mode, cube, crho);
 }
 
 // Robin operator value at a shell point (no update)
@@ -1783,12 +1885,17 @@ __device__ __forceinline__ int coarse_p(const CoarseGeo &G, int i, int j, int k)
   return c * G.cs + k * G.ps + j * G.rs + int((unsigned(i) * G.ncm) >> 16);
 }
 
-// Robin update at shell point (i, j, k) of the LDS field; mask has bit a
+// Robin update at shell point (i, j, k) of the LDS field; MASK has bit a
 // for each axis whose index is 0 or n-1 (dx2 one-sided derivative,
-// diff_2nd.cc:237-335)
+// diff_2nd.cc:237-335).  The coarsest level is 2nd order, so its layout has
+// two colours: point (i, j, k) is slot i / 2 of row (j, k) in block
+// (i + j + k) & 1, and a step along an axis moves to the other block.
+template <int MASK>
 __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, const double *srho,
-                                             int i, int j, int k, int mask) {
-  const int p = coarse_p(G, i, j, k);
+                                             int i, int j, int k) {
+  const int c = (i + j + k) & 1;
+  const int p = c * G.cs + k * G.ps + j * G.rs + (i >> 1);
+  const int flip = (1 - 2 * c) * G.cs;
   const int ijk[3] = {i, j, k};
   double X[3];
 #pragma unroll
@@ -1797,12 +1904,19 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
   double sF = 0.0, sdF = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if (!(mask & (1 << a))) continue;
+    if (!(MASK & (1 << a))) continue;
     const bool lo = ijk[a] == 0;
     const int dir = lo ? 1 : -1;
-    const int e0 = a == 0 ? dir : 0, e1 = a == 1 ? dir : 0, e2 = a == 2 ? dir : 0;
-    const int o1 = coarse_p(G, i + e0, j + e1, k + e2) - p;
-    const int o2 = coarse_p(G, i + 2 * e0, j + 2 * e1, k + 2 * e2) - p;
+    // offsets of the points 1 and 2 steps inwards along axis a
+    int o1, o2;
+    if (a == 0) {
+      o1 = flip + ((i + dir) >> 1) - (i >> 1);
+      o2 = dir;
+    } else {
+      const int st = a == 1 ? G.rs : G.ps;
+      o1 = flip + dir * st;
+      o2 = 2 * dir * st;
+    }
     const double id = G.idx[a];
     double cf, dv;
     if (lo) {
@@ -1821,15 +1935,14 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
   su[p] = su[p] - Fv / dF;
 }
 
-constexpr int kCoarseThreads = 512;
-
 // OWN: interior points per colour per thread (colour c's points are
-// numbered row by row, ceil(ni / 2) slots per interior row).  With up to
-// 4 points per colour the coefficients of a thread's points stay in
-// registers for the whole loop; with more they would not fit the register
-// file (the kernel spilled), so each colour's coefficients are loaded from
-// the (cache-resident) fields at the start of the colour's pass.
-template <int EQ, int OWN, int NT = kCoarseThreads>
+// numbered row by row, ceil(ni / 2) slots per interior row).  With at most
+// 2048 points per colour over the workgroup (OWN * NT) the coefficients of
+// a thread's points stay in registers for the whole loop; beyond that they
+// would not fit the register file (the kernel spilled), so each colour's
+// coefficients are loaded from the (cache-resident) fields at the start of
+// the colour's pass.
+template <int EQ, int OWN, int NT>
 __global__ __launch_bounds__(NT) void k_coarse_solve(const CoarseItem *__restrict__ items,
                                                                  double eps, int max_iter) {
   extern __shared__ double lds[];
@@ -1868,7 +1981,7 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(const CoarseItem *__restric
   const int ni = n0 - 2, nj = n1 - 2, nk = n2 - 2;
   const int per_row = (ni + 1) / 2;
   const int slots = nj * nk * per_row;
-  constexpr bool kCached = OWN <= 4 && NT <= 512;
+  constexpr bool kCached = OWN * NT <= 2048;
   constexpr int NCACHE = kCached ? 2 * OWN : 1;
   int own_p[2 * OWN];
   double own_b[NCACHE], own_cc[NCACHE], own_d[NCACHE], own_e[NCACHE];
@@ -1910,46 +2023,44 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(const CoarseItem *__restric
   do {
     prev = cur;
     for (int t = threadIdx.x; t < nface; t += NT) {
-      int i, j, k, mask, r = t;
+      int r = t;
       if (r < 2 * fx) {
         const int hi = r >= fx;
         r -= hi * fx;
         const int q = int(fast_div(unsigned(r), unsigned(nj), inj));
-        i = hi ? n0 - 1 : 0; j = 1 + r - q * nj; k = 1 + q; mask = 1;
+        coarse_robin<1>(G, su, srho, hi ? n0 - 1 : 0, 1 + r - q * nj, 1 + q);
       } else if ((r -= 2 * fx) < 2 * fy) {
         const int hi = r >= fy;
         r -= hi * fy;
         const int q = int(fast_div(unsigned(r), unsigned(ni), ini));
-        j = hi ? n1 - 1 : 0; i = 1 + r - q * ni; k = 1 + q; mask = 2;
+        coarse_robin<2>(G, su, srho, 1 + r - q * ni, hi ? n1 - 1 : 0, 1 + q);
       } else {
         r -= 2 * fy;
         const int hi = r >= fz;
         r -= hi * fz;
         const int q = int(fast_div(unsigned(r), unsigned(ni), ini));
-        k = hi ? n2 - 1 : 0; i = 1 + r - q * ni; j = 1 + q; mask = 4;
+        coarse_robin<4>(G, su, srho, 1 + r - q * ni, 1 + q, hi ? n2 - 1 : 0);
       }
-      coarse_robin(G, su, srho, i, j, k, mask);
     }
     __syncthreads();
     if (threadIdx.x < nedge) {
-      int t = threadIdx.x, i, j, k, mask;
+      int t = threadIdx.x;
       if (t < 4 * nk) {
         const int e = int(fast_div(unsigned(t), unsigned(nk), ink));
-        i = (e >> 1) ? n0 - 1 : 0; j = (e & 1) ? n1 - 1 : 0; k = 1 + t - e * nk; mask = 3;
+        coarse_robin<3>(G, su, srho, (e >> 1) ? n0 - 1 : 0, (e & 1) ? n1 - 1 : 0, 1 + t - e * nk);
       } else if ((t -= 4 * nk) < 4 * nj) {
         const int e = int(fast_div(unsigned(t), unsigned(nj), inj));
-        i = (e >> 1) ? n0 - 1 : 0; k = (e & 1) ? n2 - 1 : 0; j = 1 + t - e * nj; mask = 5;
+        coarse_robin<5>(G, su, srho, (e >> 1) ? n0 - 1 : 0, 1 + t - e * nj, (e & 1) ? n2 - 1 : 0);
       } else {
         t -= 4 * nj;
         const int e = int(fast_div(unsigned(t), unsigned(ni), ini));
-        k = (e >> 1) ? n2 - 1 : 0; j = (e & 1) ? n1 - 1 : 0; i = 1 + t - e * ni; mask = 6;
+        coarse_robin<6>(G, su, srho, 1 + t - e * ni, (e & 1) ? n1 - 1 : 0, (e >> 1) ? n2 - 1 : 0);
       }
-      coarse_robin(G, su, srho, i, j, k, mask);
     }
     __syncthreads();
     if (threadIdx.x < 8)
-      coarse_robin(G, su, srho, (threadIdx.x >> 1) & 1 ? n0 - 1 : 0, (threadIdx.x >> 2) & 1 ? n1 - 1 : 0,
-                   threadIdx.x & 1 ? n2 - 1 : 0, 7);
+      coarse_robin<7>(G, su, srho, (threadIdx.x >> 1) & 1 ? n0 - 1 : 0,
+                      (threadIdx.x >> 2) & 1 ? n1 - 1 : 0, threadIdx.x & 1 ? n2 - 1 : 0);
     __syncthreads();
     double mx = 0.0;
 #pragma unroll
@@ -2569,20 +2680,26 @@ void launch_ghost_fill(hipStream_t s, const PtBlock *blocks, int nblocks, int or
   }
 }
 
+// coarse-solve shape by box size: up to 2048 points of a colour, 1024
+// threads with 2 points each and their coefficients in registers (512
+// threads with 4 each for the scalar-field and EM equations, whose longer
+// update would spill at 1024 threads); up to 4096 (the 21^3 coarsest box
+// of N=161), 1024 threads with 4 each
 bool launch_coarse_solve(hipStream_t s, const CoarseItem *items, int n, int max_pts, int max_int,
                          int eq, double eps, int max_iter) {
   const size_t lds = 2 * sizeof(double) * size_t(max_pts);  // max_pts: largest B.size
   // max_int: largest number of points of one colour slot list, ceil(ni/2) nj nk
-  // up to 4 points of a colour per thread: 512 threads, or 1024 for
-  // larger boxes (the 21^3 coarsest box of N=161)
-  if (lds + 2048 > size_t(kLdsLimit) || max_int > 2 * kCoarseThreads * 4) return false;
+  if (lds + 2048 > size_t(kLdsLimit) || max_int > 4096) return false;
   if (n == 0) return true;
-  const bool small = max_int <= kCoarseThreads * 4;
+  const bool small = max_int <= 2048;
+  const bool wide = eq != 1 && eq != kEqPuncturesEm;
 #define OLL_COARSE(E)                                                                              \
-  if (small)                                                                                       \
-    hipLaunchKernelGGL((k_coarse_solve<E, 4>), dim3(n), dim3(kCoarseThreads), lds, s, items, eps, max_iter); \
+  if (small && wide)                                                                               \
+    hipLaunchKernelGGL((k_coarse_solve<E, 2, 1024>), dim3(n), dim3(1024), lds, s, items, eps, max_iter); \
+  else if (small)                                                                                  \
+    hipLaunchKernelGGL((k_coarse_solve<E, 4, 512>), dim3(n), dim3(512), lds, s, items, eps, max_iter); \
   else                                                                                             \
-    hipLaunchKernelGGL((k_coarse_solve<E, 4, 2 * kCoarseThreads>), dim3(n), dim3(2 * kCoarseThreads), lds, s, items, eps, max_iter);
+    hipLaunchKernelGGL((k_coarse_solve<E, 4, 1024>), dim3(n), dim3(1024), lds, s, items, eps, max_iter);
   switch (eq) {
     case 1: OLL_COARSE(1) break;
     case 2: OLL_COARSE(2) break;
@@ -2622,16 +2739,13 @@ void launch_zero_lists(hipStream_t s, const ZeroItem *items, int n) {
 void kernels_init() {
   static bool done = false;
   if (done) return;
-  const void *fns[] = {reinterpret_cast<const void *>(&k_coarse_solve<0, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<0, 4, 2 * kCoarseThreads>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<1, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<1, 4, 2 * kCoarseThreads>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<2, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<2, 4, 2 * kCoarseThreads>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<3, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<3, 4, 2 * kCoarseThreads>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 4>),
-                       reinterpret_cast<const void *>(&k_coarse_solve<kEqPuncturesEm, 4, 2 * kCoarseThreads>)};
+#define OLL_COARSE_FNS(E)                                          \
+  reinterpret_cast<const void *>(&k_coarse_solve<E, 4, 512>),      \
+      reinterpret_cast<const void *>(&k_coarse_solve<E, 2, 1024>), \
+      reinterpret_cast<const void *>(&k_coarse_solve<E, 4, 1024>)
+  const void *fns[] = {OLL_COARSE_FNS(0), OLL_COARSE_FNS(1), OLL_COARSE_FNS(2), OLL_COARSE_FNS(3),
+                       OLL_COARSE_FNS(kEqPuncturesEm)};
+#undef OLL_COARSE_FNS
   for (const void *f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsLimit - 1024);
   done = true;
