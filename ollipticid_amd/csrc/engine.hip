@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <tuple>
 
 // host arithmetic that must equal the reference's (coordinates, the
@@ -644,6 +645,7 @@ struct Builder {
       }
       std::vector<RowBlock> cb, sb;
       double bytes[5] = {0, 0, 0, 0, 0};
+      std::set<const DevBox *> boxes;
       // colour-pass workgroup size from the launch-shape table, at least
       // one interior row
       int threads = sweep_threads(mixed ? 1 : 0, order);
@@ -653,6 +655,8 @@ struct Builder {
         const int l = lv[size_t(h)];
         row_blocks(H, l, mixed ? nullptr : slot(H, l), false, pc, cb, threads);
         if (mixed) row_blocks(H, l, slot(H, l), true, ps, sb);
+        for (int b = 0; b < H.geo.level_count[size_t(l)]; ++b)
+          boxes.insert(H.dbox + H.geo.level_first[size_t(l)] + b);
         // algorithmic bytes per updated point: FP64 reads u, b, c, rho
         // (+ d, e) and writes u; FP32 reads e, q, r and writes e
         const double per = mixed ? 16.0
@@ -717,6 +721,7 @@ struct Builder {
           L.max_row = std::min(pc, threads);
           L.threads = threads;
           L.bytes = bytes[q];
+          if (boxes.size() == 1) L.box1 = *boxes.begin();
           S.launches.push_back(L);
         }
       if (mixed) whole(Launch::UPDATE32);
@@ -1051,7 +1056,7 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         break;
       case Launch::SMOOTH:
         launch_smooth(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq,
-                      L.colour, L.threads, e0, e1);
+                      L.colour, L.threads, e0, e1, L.box1);
         break;
       case Launch::DEFECT32:
         launch_defect32(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq);

@@ -100,6 +100,7 @@ struct Launch {
   int max_pts = 0, max_int = 0;
   int pair = 0, parity = 0;  // SMOOTH_FUSED: two passes or one; 0 reads u, 1 reads r
   double bytes = 0;   // algorithmic bytes (smoother launches, FP64 or FP32)
+  const DevBox *box1 = nullptr;  // row launches over a single box: its descriptor
 };
 
 struct Schedule {

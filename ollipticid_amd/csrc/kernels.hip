@@ -207,6 +207,18 @@ typedef const __attribute__((address_space(4))) DevBox ConstBox;
 typedef __attribute__((address_space(1))) char gchar;
 
 __device__ __forceinline__ ConstBox &cbox(const DevBox *b) { return *(ConstBox *)b; }
+// brings every cache line of a box descriptor into the scalar cache with
+// loads issued together (one dword per 64 bytes, and the last), and waits
+// once for them
+__device__ __forceinline__ void fetch_box(const DevBox *b) {
+  typedef const __attribute__((address_space(4))) unsigned CWord;
+  CWord *w = (CWord *)b;
+  constexpr int kWords = int(sizeof(DevBox) / 4);
+  unsigned acc = w[kWords - 1];
+#pragma unroll
+  for (int q = 0; q < kWords; q += 16) acc ^= w[q];
+  asm volatile("" ::"s"(acc));
+}
 // the generic helpers (robin_op_at) take a generic reference
 __device__ __forceinline__ const DevBox &cbox_plain(ConstBox &b) { return *(const DevBox *)&b; }
 // x / nc and the colour of (i, j, k) from a constant-space box (kernels.hpp
@@ -414,13 +426,10 @@ __device__ __forceinline__ Nbr<NC> box_offs(ConstBox &B, int c, int s) {
 // smoother: one colour of one Newton-Gauss-Seidel sweep on the rows of
 // every listed block; thread (lr, t) owns colour point t of local row lr
 template <int ORD, int EQ>
-__global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
-                                                        unsigned per_row, float inv_per_row,
-                                                        int colour) {
+__device__ __forceinline__ void smooth_rows(const RowBlock &Bk, ConstBox &B, unsigned per_row,
+                                            float inv_per_row, int colour) {
   constexpr int R = ORD / 2;
   constexpr int NC = colours_of(ORD, EQ);
-  const RowBlock Bk = blocks[xcd_block()];
-  ConstBox &B = cbox(Bk.box);
   const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
   const int t = int(threadIdx.x - lr * per_row);
   double rmax = 0.0;
@@ -441,6 +450,26 @@ __global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__res
     if (!(un == un)) rmax = __longlong_as_double(0x7ff0000000000000LL);  // inf flags NaN
   }
   wave_norm(Bk.norm, rmax);
+}
+
+// box1: the one box of every listed block, when the launch covers a single
+// box (else null); its descriptor is then fetched without waiting for the
+// block's.  Either way the descriptor's cache lines are fetched in one batch
+// first: the kernel branches on several of its fields, and loads issued
+// branch by branch would each wait for the one before.
+template <int ORD, int EQ>
+__global__ __launch_bounds__(kMaxRowThreads) void k_smooth(const RowBlock *__restrict__ blocks,
+                                                        unsigned per_row, float inv_per_row,
+                                                        int colour, const DevBox *box1) {
+  if (box1) {
+    const RowBlock Bk = blocks[xcd_block()];
+    fetch_box(box1);
+    smooth_rows<ORD, EQ>(Bk, cbox(box1), per_row, inv_per_row, colour);
+  } else {
+    const RowBlock Bk = blocks[xcd_block()];
+    fetch_box(Bk.box);
+    smooth_rows<ORD, EQ>(Bk, cbox(Bk.box), per_row, inv_per_row, colour);
+  }
 }
 
 // ------------------------------------------------------------------
@@ -2454,14 +2483,14 @@ inline unsigned ceil_div(long long a, long long b) { return unsigned((a + b - 1)
 
 template <int ORD, int EQ>
 void smooth_impl(hipStream_t s, const RowBlock *blocks, int nb, int per_row, int colour,
-                 int threads, hipEvent_t e0, hipEvent_t e1) {
+                 int threads, hipEvent_t e0, hipEvent_t e1, const DevBox *box1) {
   if (nb == 0) return;
   if (e0)  // timed (never under graph capture): the runtime stamps e0, e1
     hipExtLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(threads), 0, s, e0, e1, 0, blocks,
-                          unsigned(per_row), 1.0f / float(per_row), colour);
+                          unsigned(per_row), 1.0f / float(per_row), colour, box1);
   else
     hipLaunchKernelGGL((k_smooth<ORD, EQ>), dim3(nb), dim3(threads), 0, s, blocks,
-                       unsigned(per_row), 1.0f / float(per_row), colour);
+                       unsigned(per_row), 1.0f / float(per_row), colour, box1);
 }
 
 template <int ORD, int EQ>
@@ -2555,8 +2584,10 @@ bool set_sweep_threads(int mixed, int order, int threads) {
 }
 
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                   int eq, int colour, int threads, hipEvent_t e0, hipEvent_t e1) {
-  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour, threads, e0, e1);
+                   int eq, int colour, int threads, hipEvent_t e0, hipEvent_t e1,
+                   const DevBox *box1) {
+  OLL_DISPATCH_ORDER(order, eq, smooth_impl, s, blocks, nblocks, per_row, colour, threads, e0, e1,
+                     box1);
 }
 
 // tile shape of the fused passes, [order / 2 - 1]

@@ -385,8 +385,10 @@ void launch_coefficients_box(hipStream_t s, const DevBox &B, const PunctureArgs 
 // below 4 GiB (checked when the hierarchy is built).
 constexpr long long kMaxFieldBytes = 1LL << 32;
 // e0, e1 (optional): events the runtime stamps at the kernel's start and end
+// box1: the device descriptor of the one box every block lists, or null
 void launch_smooth(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
-                   int eq, int colour, int threads, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                   int eq, int colour, int threads, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                   const DevBox *box1 = nullptr);
 // what k_residual writes on the interior: r = N(u), w = rho - N(u), or
 // r = dN/du_p (the Newton diagonal, elliptic.h:337-345 duOPERATOR)
 enum ResidualMode { RES_OPERATOR = 0, RES_DEFECT = 1, RES_DERIVATIVE = 2 };
