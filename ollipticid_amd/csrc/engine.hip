@@ -572,9 +572,9 @@ struct Builder {
   // (pr: the level's longest row in threads; a launch runs min(pr,
   // threads) threads per row, splitting longer rows into segments)
   static void row_blocks(const Hier &H, int level, unsigned long long *norm, bool subrows, int pr,
-                         std::vector<RowBlock> &out, int threads = kRowThreads) {
+                         std::vector<RowBlock> &out, int threads = kRowThreads, int reps = 1) {
     const int seg = std::min(pr, threads);
-    const int rpb = std::max(1, threads / seg);
+    const int rpb = std::max(1, threads / seg) * reps;
     const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
     for (int b = f; b < f + n; ++b) {
       const DevBox &B = H.hbox[size_t(b)];
@@ -584,6 +584,22 @@ struct Builder {
       for (int r0 = 0; r0 < rows; r0 += rpb)
         out.push_back(RowBlock{H.dbox + b, norm, r0, std::min(rpb, rows - r0), nseg});
     }
+  }
+
+  // blocks row_blocks() gives at reps = 1
+  static long long row_block_count(const Hier &H, int level, bool subrows, int pr,
+                                   int threads = kRowThreads) {
+    const int seg = std::min(pr, threads);
+    const int rpb = std::max(1, threads / seg);
+    const int f = H.geo.level_first[size_t(level)], n = H.geo.level_count[size_t(level)];
+    long long nb = 0;
+    for (int b = f; b < f + n; ++b) {
+      const DevBox &B = H.hbox[size_t(b)];
+      const int len = subrows ? B.rl : (B.n[0] - 2 * B.R + B.nc - 1) / B.nc;
+      const long long rows = (long long)interior_rows(B) * (subrows ? B.nc : 1) * ((len + seg - 1) / seg);
+      nb += (rows + rpb - 1) / rpb;
+    }
+    return nb;
   }
 
   // tiles of the fused passes over the interior rows of every box of a
@@ -650,11 +666,17 @@ struct Builder {
       // one interior row
       int threads = sweep_threads(mixed ? 1 : 0, order);
       while (threads < pc && threads < kMaxRowThreads) threads *= 2;
+      int sreps = 1;
+      if (mixed) {
+        long long nsb = 0;
+        for (int h : g.second) nsb += row_block_count(*M[size_t(h)], lv[size_t(h)], true, ps);
+        sreps = block_reps(g.second.size(), nsb);
+      }
       for (int h : g.second) {
         const Hier &H = *M[size_t(h)];
         const int l = lv[size_t(h)];
         row_blocks(H, l, mixed ? nullptr : slot(H, l), false, pc, cb, threads);
-        if (mixed) row_blocks(H, l, slot(H, l), true, ps, sb);
+        if (mixed) row_blocks(H, l, slot(H, l), true, ps, sb, kRowThreads, sreps);
         for (int b = 0; b < H.geo.level_count[size_t(l)]; ++b)
           boxes.insert(H.dbox + H.geo.level_first[size_t(l)] + b);
         // algorithmic bytes per updated point: FP64 reads u, b, c, rho
@@ -737,10 +759,13 @@ struct Builder {
       int ps = 0;
       for (int h : g.second) ps = std::max(ps, per_row(*M[size_t(h)], lv[size_t(h)], true));
       std::vector<RowBlock> blocks;
+      long long nb = 0;
+      for (int h : g.second) nb += row_block_count(*M[size_t(h)], lv[size_t(h)], true, ps);
+      const int reps = block_reps(g.second.size(), nb);
       for (int h : g.second) {
         const Hier &H = *M[size_t(h)];
         row_blocks(H, lv[size_t(h)], with_norm ? H.slot(H.slot_scratch()) : nullptr, true, ps,
-                   blocks);
+                   blocks, kRowThreads, reps);
       }
       Launch L{Launch::RESIDUAL};
       L.order = g.first.first;
@@ -781,17 +806,30 @@ struct Builder {
   // X[k][t] (k = the member's fine level of the pair)
   // (shell: the ghost shell's points, or with columns its columns of R
   // points across the face slabs)
-  static void points(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
-                     bool shell, int mg, std::vector<PtBlock> &out, bool columns = false) {
+  static long long point_count(const DevBox &B, bool shell, bool columns) {
+    return shell ? (columns ? shell_pts(B) / B.R : shell_pts(B)) : npts(B);
+  }
+  // blocks points() gives level tl of H at reps = 1
+  static long long point_blocks(const Hier &H, int tl, bool shell, bool columns = false) {
     const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
+    long long nb = 0;
+    for (int t = 0; t < n; ++t)
+      nb += (point_count(H.hbox[size_t(f + t)], shell, columns) + kPtThreads - 1) / kPtThreads;
+    return nb;
+  }
+  static void points(const Hier &H, int tl, int k, const std::vector<std::vector<Hier::XRef>> &X,
+                     bool shell, int mg, std::vector<PtBlock> &out, bool columns = false,
+                     int reps = 1) {
+    const int f = H.geo.level_first[size_t(tl)], n = H.geo.level_count[size_t(tl)];
+    const long long per = (long long)kPtThreads * reps;
     for (int t = 0; t < n; ++t) {
       const DevBox &B = H.hbox[size_t(f + t)];
       const Hier::XRef r = X[size_t(k)][size_t(t)];
-      const long long np = shell ? (columns ? shell_pts(B) / B.R : shell_pts(B)) : npts(B);
+      const long long np = point_count(B, shell, columns);
       const long long plane = (long long)B.n[0] * B.n[1];
-      for (long long q0 = 0; q0 < np; q0 += kPtThreads)
+      for (long long q0 = 0; q0 < np; q0 += per)
         out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, mg, unsigned(q0),
-                              unsigned(std::min<long long>(kPtThreads, np - q0)),
+                              unsigned(std::min<long long>(per, np - q0)),
                               int(q0 % B.n[0]), int((q0 % plane) / B.n[0]), int(q0 / plane)});
     }
   }
@@ -838,11 +876,38 @@ struct Builder {
     }
   }
 
+  // Blocks per kPtThreads points (or per workgroup of rows) for a launch
+  // over one request; a launch over several requests gives each block reps
+  // times that many, which the kernels' threads step through, so that its
+  // table (built and uploaded again whenever the set of requests changes)
+  // is a fraction of the size while the launch keeps >= 4096 blocks.
+  static int block_reps(size_t members, long long blocks) {
+    if (members < 2) return 1;
+    return int(std::max<long long>(1, std::min<long long>(16, blocks / 4096)));
+  }
+
   void transfer(Launch::Kind kind, const std::vector<Hier *> &M, const std::vector<int> &lv) {
     // lv[h] = the member's fine level k of the pair (k-1, k); restrictions
     // are grouped by the coarse level's (order, eq), ghost fills by the
     // fine level's order (their Lagrange stencil)
     std::map<std::pair<int, int>, std::vector<PtBlock>> groups;
+    // point blocks of this stage over all members at one block per
+    // kPtThreads points (row-segment blocks are never joined)
+    long long nb = 0;
+    size_t members = 0;
+    for (size_t h = 0; h < M.size(); ++h) {
+      const int k = lv[h];
+      if (k < 1) continue;
+      const Hier &H = *M[h];
+      const int mg = k <= H.geo.mgrid;
+      ++members;
+      if ((kind == Launch::RESTRICT_U || kind == Launch::RESTRICT_DEFECT) && mg)
+        nb += point_blocks(H, k - 1, false);
+      if (kind == Launch::PROLONG) nb += point_blocks(H, k, false);
+      if (kind == Launch::GHOST && !mg)
+        nb += point_blocks(H, k, true, ghost_columns_for(H.level_order(k)));
+    }
+    const int reps = block_reps(members, nb);
     for (size_t h = 0; h < M.size(); ++h) {
       const int k = lv[h];
       if (k < 1) continue;
@@ -850,20 +915,20 @@ struct Builder {
       const int mg = k <= H.geo.mgrid;
       if (kind == Launch::RESTRICT_U) {
         if (mg)
-          points(H, k - 1, k, H.ru, false, mg, groups[{0, 0}]);
+          points(H, k - 1, k, H.ru, false, mg, groups[{0, 0}], false, reps);
         else
           segments(H, k - 1, k, H.ru, groups[{0, 0}], false);
       }
       if (kind == Launch::RESTRICT_DEFECT) {
         if (mg)
-          points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}]);
+          points(H, k - 1, k, H.rw, false, mg, groups[{H.level_order(k - 1), H.prob.eq}], false, reps);
         else
           segments(H, k - 1, k, H.rw, groups[{H.level_order(k - 1), H.prob.eq}], true);
       }
-      if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, groups[{0, 0}]);
+      if (kind == Launch::PROLONG) points(H, k, k, H.pv, false, mg, groups[{0, 0}], false, reps);
       if (kind == Launch::GHOST && !mg)
         points(H, k, k, H.gf, true, 0, groups[{H.level_order(k), 0}],
-               ghost_columns_for(H.level_order(k)));
+               ghost_columns_for(H.level_order(k)), reps);
     }
     for (auto &g : groups) {
       if (g.second.empty()) continue;

@@ -390,11 +390,13 @@ __device__ __forceinline__ bool colour_slot(const RowBlock &Bk, ConstBox &B, uns
 // ghost and padding slots
 template <int R, int NC = R + 1>
 __device__ __forceinline__ bool row_slot(const RowBlock &Bk, ConstBox &B, unsigned per_row,
-                                         float inv_per_row, unsigned &p, int &c, int &s, int &i,
-                                         int &j, int &k) {
-  const unsigned lr = fast_div(threadIdx.x, per_row, inv_per_row);
-  int t = int(threadIdx.x - lr * per_row);
-  if (lr >= unsigned(Bk.rows)) return false;
+                                         float inv_per_row, unsigned rpb, unsigned lr0, unsigned &p,
+                                         int &c, int &s, int &i, int &j, int &k) {
+  const unsigned lrl = fast_div(threadIdx.x, per_row, inv_per_row);
+  int t = int(threadIdx.x - lrl * per_row);
+  // threads past the chunk's whole rows idle (the next chunk has them)
+  const unsigned lr = lr0 + lrl;
+  if (lrl >= rpb || lr >= unsigned(Bk.rows)) return false;
   unsigned sr = unsigned(Bk.row0) + lr;
   if (Bk.nseg > 1) {  // segment `seg` of the colour row: slots seg * per_row on
     const unsigned seg = sr % unsigned(Bk.nseg);
@@ -410,6 +412,23 @@ __device__ __forceinline__ bool row_slot(const RowBlock &Bk, ConstBox &B, unsign
   p = unsigned(c) * unsigned(B.cs) + unsigned(k) * unsigned(B.ps) + unsigned(j) * unsigned(B.rs) +
       unsigned(t);
   return true;
+}
+
+// the larger of acc and a, a NaN in either kept (one step from acc = 0
+// gives a itself, NaN included)
+__device__ __forceinline__ double nan_max(double acc, double a) {
+  return (a > acc || !(a == a)) ? a : acc;
+}
+
+// a whole-row block's rows, rpb = blockDim / per_row at a time: f(rpb,
+// lr0) for the first local row lr0 of each chunk (blocks of one launch
+// over many requests hold several chunks)
+template <class F>
+__device__ __forceinline__ void for_rows(const RowBlock &Bk, unsigned per_row, float inv_per_row,
+                                         F &&f) {
+  const unsigned rpb = fast_div(blockDim.x, per_row, inv_per_row);
+#pragma unroll 1
+  for (unsigned lr0 = 0; lr0 < unsigned(Bk.rows); lr0 += rpb) f(rpb, lr0);
 }
 
 // neighbour byte offsets (ES bytes per element) of a point of colour c
@@ -594,9 +613,10 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
-  unsigned p;
-  int c, s, i, j, k;
-  if (row_slot<R, NC>(Bk, B, per_row, inv_per_row, p, c, s, i, j, k)) {
+  for_rows(Bk, per_row, inv_per_row, [&](unsigned rpb, unsigned lr0) {
+    unsigned p;
+    int c, s, i, j, k;
+    if (!row_slot<R, NC>(Bk, B, per_row, inv_per_row, rpb, lr0, p, c, s, i, j, k)) return;
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u;
     const double *u = B.f[F_U];
@@ -606,8 +626,8 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
       st<double>(B.f[F_W], o, ld<double>(B.f[F_RHO], o) - N);
     else
       st<double>(B.f[F_R], o, mode == RES_DERIVATIVE ? dN : N);
-    rmax = fabs(N);
-  }
+    rmax = nan_max(rmax, fabs(N));
+  });
   wave_norm(Bk.norm, rmax);
 }
 
@@ -628,9 +648,10 @@ __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__rest
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
   double rmax = 0.0;
-  unsigned p;
-  int c, s, i, j, k;
-  if (row_slot<R, NC>(Bk, B, per_row, inv_per_row, p, c, s, i, j, k)) {
+  for_rows(Bk, per_row, inv_per_row, [&](unsigned rpb, unsigned lr0) {
+    unsigned p;
+    int c, s, i, j, k;
+    if (!row_slot<R, NC>(Bk, B, per_row, inv_per_row, rpb, lr0, p, c, s, i, j, k)) return;
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u, o4 = p * 4u;
     const double *u = B.f[F_U];
@@ -640,9 +661,9 @@ __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__rest
     st<float>(B.g[G_R], o4, float(res));
     st<float>(B.g[G_Q], o4, float(dN - B.dulap));
     st<float>(B.g[G_E], o4, 0.0f);
-    rmax = fabs(res);
+    rmax = nan_max(rmax, fabs(res));
     if (!(res == res)) rmax = __longlong_as_double(0x7ff0000000000000LL);
-  }
+  });
   wave_norm(Bk.norm, rmax);
 }
 
@@ -677,11 +698,13 @@ __global__ __launch_bounds__(kRowThreads) void k_update32(const RowBlock *__rest
   constexpr int R = ORD / 2;
   const RowBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
-  unsigned p;
-  int c, s, i, j, k;
-  if (!row_slot<R>(Bk, B, per_row, inv_per_row, p, c, s, i, j, k)) return;
-  double *u = B.f[F_U];
-  st<double>(u, p * 8u, ld<double>(u, p * 8u) + double(ld<float>(B.g[G_E], p * 4u)));
+  for_rows(Bk, per_row, inv_per_row, [&](unsigned rpb, unsigned lr0) {
+    unsigned p;
+    int c, s, i, j, k;
+    if (!row_slot<R>(Bk, B, per_row, inv_per_row, rpb, lr0, p, c, s, i, j, k)) return;
+    double *u = B.f[F_U];
+    st<double>(u, p * 8u, ld<double>(u, p * 8u) + double(ld<float>(B.g[G_E], p * 4u)));
+  });
 }
 
 // ------------------------------------------------------------------
@@ -1040,23 +1063,31 @@ struct Walk {
   __device__ __forceinline__ unsigned z(unsigned &c) const { return colour(c) + ps8; }
 };
 
-// the block's target point in lexicographic order (point q0 + thread,
-// stepped from the block's first point so that only small offsets are
-// divided); false past the block's end
-__device__ __forceinline__ bool pt_lex(const PtBlock &Bk, ConstBox &T, int &i, int &j, int &k) {
-  if (threadIdx.x >= Bk.nq) return false;
+// point q0 + off of a block in lexicographic order, stepped from the
+// block's first point (i0, j0, k0) so that only offsets below 2^24 are
+// divided
+__device__ __forceinline__ void pt_lex(const PtBlock &Bk, ConstBox &T, unsigned off, int &i, int &j,
+                                       int &k) {
   const unsigned n0 = unsigned(T.n[0]), n1 = unsigned(T.n[1]);
-  const unsigned x = unsigned(Bk.i0) + threadIdx.x;
+  const unsigned x = unsigned(Bk.i0) + off;
   const unsigned dj = fast_div(x, n0, T.in0);
   unsigned jj = unsigned(Bk.j0) + dj, kk = unsigned(Bk.k0);
-  while (jj >= n1) {  // a block spans at most a few rows: n1 >= 3
-    jj -= n1;
-    ++kk;
+  if (jj >= n1) {
+    const unsigned dk = fast_div(jj, n1, 1.0f / float(n1));
+    jj -= dk * n1;
+    kk += dk;
   }
   i = int(x - dj * n0);
   j = int(jj);
   k = int(kk);
-  return true;
+}
+
+// f(off) for the offsets off < nq of a block's points that fall to this
+// thread (kPtThreads threads step through blocks of any length)
+template <class F>
+__device__ __forceinline__ void for_points(const PtBlock &Bk, F &&f) {
+#pragma unroll 1
+  for (unsigned off = threadIdx.x; off < Bk.nq; off += kPtThreads) f(off);
 }
 
 __device__ __forceinline__ bool is_interior(ConstBox &B, int i, int j, int k) {
@@ -1284,37 +1315,39 @@ __global__ __launch_bounds__(kPtThreads) void k_transfer_fields(const PtBlock *_
                                                                 int ff, int tf) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  int i, j, k;
-  if (!pt_lex(Bk, T, i, j, k)) return;
-  const int region = Bk.mg & 3, all = (Bk.mg >> 2) & 1, to_zero = (Bk.mg >> 3) & 1;
-  const bool inside = is_interior(T, i, j, k);
-  if ((region == 1 && !inside) || (region == 2 && inside)) return;
-  const int t[3] = {i, j, k};
-  const unsigned o = boff(T, i, j, k);
-  ConstXfer *xs = (ConstXfer *)Bk.src;
-  for (int x = Bk.nsrc - 1; x >= 0; --x) {
-    ConstXfer &X = xs[x];
-    unsigned fl[3];
-    const unsigned f = axis_flags(X, t, fl);
-    if ((f & 1) || (all && (f & 2))) {
-      ConstBox &S = cbox(X.src);
-      double v;
-      if constexpr (KP == 2) {
-        if (f & 4)  // a source node: its value (interpolation.cc:140-147)
-          v = ld<double>(S.f[ff], boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]));
-        else
-          v = trilinear_field(S, S.f[ff], X, t);
-      } else {
-        v = tinterp<KP>(S, S.f[ff], X, t, fl);
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    pt_lex(Bk, T, off, i, j, k);
+    const int region = Bk.mg & 3, all = (Bk.mg >> 2) & 1, to_zero = (Bk.mg >> 3) & 1;
+    const bool inside = is_interior(T, i, j, k);
+    if ((region == 1 && !inside) || (region == 2 && inside)) return;
+    const int t[3] = {i, j, k};
+    const unsigned o = boff(T, i, j, k);
+    ConstXfer *xs = (ConstXfer *)Bk.src;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      ConstXfer &X = xs[x];
+      unsigned fl[3];
+      const unsigned f = axis_flags(X, t, fl);
+      if ((f & 1) || (all && (f & 2))) {
+        ConstBox &S = cbox(X.src);
+        double v;
+        if constexpr (KP == 2) {
+          if (f & 4)  // a source node: its value (interpolation.cc:140-147)
+            v = ld<double>(S.f[ff], boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]));
+          else
+            v = trilinear_field(S, S.f[ff], X, t);
+        } else {
+          v = tinterp<KP>(S, S.f[ff], X, t, fl);
+        }
+        st<double>(T.f[tf], o, v);
+        return;
       }
-      st<double>(T.f[tf], o, v);
-      return;
+      if (to_zero) {
+        st<double>(T.f[tf], o, 0.0);
+        return;
+      }
     }
-    if (to_zero) {
-      st<double>(T.f[tf], o, 0.0);
-      return;
-    }
-  }
+  });
 }
 
 // u restriction: coarse INSIDE points inside a fine box's non-ghost part
@@ -1322,18 +1355,21 @@ __global__ __launch_bounds__(kPtThreads) void k_transfer_fields(const PtBlock *_
 __global__ __launch_bounds__(kPtThreads) void k_restrict_u(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  int i, j, k;
-  if (!pt_lex(Bk, T, i, j, k) || !is_interior(T, i, j, k)) return;
-  const int t[3] = {i, j, k};
-  ConstXfer *xs = (ConstXfer *)Bk.src;
-  for (int x = Bk.nsrc - 1; x >= 0; --x) {
-    ConstXfer &X = xs[x];
-    unsigned fl[3];
-    if (!(axis_flags(X, t, fl) & 1)) continue;
-    ConstBox &S = cbox(X.src);
-    st<double>(T.f[F_U], boff(T, i, j, k), tinterp<4>(S, S.f[F_U], X, t, fl));
-    return;
-  }
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    pt_lex(Bk, T, off, i, j, k);
+    if (!is_interior(T, i, j, k)) return;
+    const int t[3] = {i, j, k};
+    ConstXfer *xs = (ConstXfer *)Bk.src;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      ConstXfer &X = xs[x];
+      unsigned fl[3];
+      if (!(axis_flags(X, t, fl) & 1)) continue;
+      ConstBox &S = cbox(X.src);
+      st<double>(T.f[F_U], boff(T, i, j, k), tinterp<4>(S, S.f[F_U], X, t, fl));
+      return;
+    }
+  });
 }
 
 // full weighting (1/4, 1/2, 1/4)^3 of field w of box S around node
@@ -1377,13 +1413,15 @@ __device__ __forceinline__ double fw27(ConstBox &S, const void *w, int ci, int c
 __global__ __launch_bounds__(kPtThreads) void k_filter_defect(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &S = cbox(Bk.box);
-  int i, j, k;
-  if (!pt_lex(Bk, S, i, j, k)) return;
-  const int R = S.R;
-  const unsigned o = boff(S, i, j, k);
-  const bool in = i - 1 >= R && i + 1 < S.n[0] - R && j - 1 >= R && j + 1 < S.n[1] - R &&
-                  k - 1 >= R && k + 1 < S.n[2] - R;
-  st<double>(S.f[F_V], o, in ? fw27(S, S.f[F_W], i, j, k) : ld<double>(S.f[F_W], o));
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    pt_lex(Bk, S, off, i, j, k);
+    const int R = S.R;
+    const unsigned o = boff(S, i, j, k);
+    const bool in = i - 1 >= R && i + 1 < S.n[0] - R && j - 1 >= R && j + 1 < S.n[1] - R &&
+                    k - 1 >= R && k + 1 < S.n[2] - R;
+    st<double>(S.f[F_V], o, in ? fw27(S, S.f[F_W], i, j, k) : ld<double>(S.f[F_W], o));
+  });
 }
 
 // The same filter, one thread per run of kFilterCx points along x of one
@@ -1476,43 +1514,45 @@ template <int ORD, int EQ>
 __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  int i, j, k;
-  if (!pt_lex(Bk, T, i, j, k)) return;
-  const unsigned o = boff(T, i, j, k);
-  const double uc = ld<double>(T.f[F_U], o);
-  st<double>(T.f[F_WW], o, uc);
-  const bool inside = is_interior(T, i, j, k);
-  if (!Bk.mg && !inside) return;
-  const int t[3] = {i, j, k};
-  bool covered = false;
-  double wc = 0.0;
-  ConstXfer *xs = (ConstXfer *)Bk.src;
-  for (int x = Bk.nsrc - 1; x >= 0; --x) {
-    ConstXfer &X = xs[x];
-    unsigned fl[3];
-    const unsigned f = axis_flags(X, t, fl);
-    if (!((f & 1) || (Bk.mg && (f & 2)))) continue;
-    ConstBox &S = cbox(X.src);
-    if (f & 8) {
-      wc = fw27(S, S.f[F_W], X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
-    } else {
-      wc = tinterp<3>(S, X.filt ? S.f[F_V] : S.f[F_W], X, t, fl);
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    pt_lex(Bk, T, off, i, j, k);
+    const unsigned o = boff(T, i, j, k);
+    const double uc = ld<double>(T.f[F_U], o);
+    st<double>(T.f[F_WW], o, uc);
+    const bool inside = is_interior(T, i, j, k);
+    if (!Bk.mg && !inside) return;
+    const int t[3] = {i, j, k};
+    bool covered = false;
+    double wc = 0.0;
+    ConstXfer *xs = (ConstXfer *)Bk.src;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      ConstXfer &X = xs[x];
+      unsigned fl[3];
+      const unsigned f = axis_flags(X, t, fl);
+      if (!((f & 1) || (Bk.mg && (f & 2)))) continue;
+      ConstBox &S = cbox(X.src);
+      if (f & 8) {
+        wc = fw27(S, S.f[F_W], X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
+      } else {
+        wc = tinterp<3>(S, X.filt ? S.f[F_V] : S.f[F_W], X, t, fl);
+      }
+      covered = true;
+      break;
     }
-    covered = true;
-    break;
-  }
-  if (!covered && !Bk.mg) return;
-  if (covered) st<double>(T.f[F_W], o, wc); else wc = ld<double>(T.f[F_W], o);
-  double Rc;
-  if (inside) {
-    constexpr int NC = colours_of(ORD, EQ);
-    const Nbr<NC> nb = box_offs<NC, 8>(T, int(colourc(T, i, j, k)), i % NC);
-    double dN;
-    point_op<ORD, EQ>(T, T.f[F_U], o, nb, uc, i, j, k, Rc, dN);
-  } else {
-    Rc = robin_op_at(cbox_plain(T), i, j, k);
-  }
-  st<double>(T.f[F_RHO], o, wc + Rc);
+    if (!covered && !Bk.mg) return;
+    if (covered) st<double>(T.f[F_W], o, wc); else wc = ld<double>(T.f[F_W], o);
+    double Rc;
+    if (inside) {
+      constexpr int NC = colours_of(ORD, EQ);
+      const Nbr<NC> nb = box_offs<NC, 8>(T, int(colourc(T, i, j, k)), i % NC);
+      double dN;
+      point_op<ORD, EQ>(T, T.f[F_U], o, nb, uc, i, j, k, Rc, dN);
+    } else {
+      Rc = robin_op_at(cbox_plain(T), i, j, k);
+    }
+    st<double>(T.f[F_RHO], o, wc + Rc);
+  });
 }
 
 // prolongation of the coarse correction and u += v (all points on
@@ -1520,32 +1560,34 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
 __global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  int i, j, k;
-  if (!pt_lex(Bk, T, i, j, k)) return;
-  if (!Bk.mg && !is_interior(T, i, j, k)) return;
-  const int t[3] = {i, j, k};
-  // the target value is loaded first; its latency overlaps the source's
-  const unsigned o = boff(T, i, j, k);
-  const double u0 = ld<double>(T.f[F_U], o);
-  ConstXfer *xs = (ConstXfer *)Bk.src;
-  for (int x = Bk.nsrc - 1; x >= 0; --x) {
-    ConstXfer &X = xs[x];
-    unsigned fl[3];
-    const unsigned f = axis_flags(X, t, fl);
-    if (!(f & 3)) continue;  // all = true
-    ConstBox &S = cbox(X.src);
-    double v;
-    if (f & 4) {
-      const unsigned c = boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
-      v = ld<double>(S.f[F_U], c) - ld<double>(S.f[F_WW], c);
-    } else if (X.affine) {
-      v = trilinear_nested(S, X, t);
-    } else {
-      v = trilinear_corr(S, X, t);
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    pt_lex(Bk, T, off, i, j, k);
+    if (!Bk.mg && !is_interior(T, i, j, k)) return;
+    const int t[3] = {i, j, k};
+    // the target value is loaded first; its latency overlaps the source's
+    const unsigned o = boff(T, i, j, k);
+    const double u0 = ld<double>(T.f[F_U], o);
+    ConstXfer *xs = (ConstXfer *)Bk.src;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      ConstXfer &X = xs[x];
+      unsigned fl[3];
+      const unsigned f = axis_flags(X, t, fl);
+      if (!(f & 3)) continue;  // all = true
+      ConstBox &S = cbox(X.src);
+      double v;
+      if (f & 4) {
+        const unsigned c = boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
+        v = ld<double>(S.f[F_U], c) - ld<double>(S.f[F_WW], c);
+      } else if (X.affine) {
+        v = trilinear_nested(S, X, t);
+      } else {
+        v = trilinear_corr(S, X, t);
+      }
+      st<double>(T.f[F_U], o, u0 + v);
+      return;
     }
-    st<double>(T.f[F_U], o, u0 + v);
-    return;
-  }
+  });
 }
 
 // shell point q of a box: the R-thick z slabs first, then y slabs, then x slabs
@@ -1589,20 +1631,21 @@ template <int KP>
 __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  if (threadIdx.x >= Bk.nq) return;
-  int i, j, k;
-  shell_point(T, Bk.q0 + threadIdx.x, i, j, k);
-  const int t[3] = {i, j, k};
-  ConstXfer *xs = (ConstXfer *)Bk.src;
-  for (int x = Bk.nsrc - 1; x >= 0; --x) {
-    ConstXfer &X = xs[x];
-    unsigned fl[3];
-    const unsigned f = axis_flags(X, t, fl);
-    if (!(f & 3)) continue;
-    ConstBox &S = cbox(X.src);
-    st<double>(T.f[F_U], boff(T, i, j, k), tinterp<KP>(S, S.f[F_U], X, t, fl));
-    return;
-  }
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    shell_point(T, Bk.q0 + off, i, j, k);
+    const int t[3] = {i, j, k};
+    ConstXfer *xs = (ConstXfer *)Bk.src;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      ConstXfer &X = xs[x];
+      unsigned fl[3];
+      const unsigned f = axis_flags(X, t, fl);
+      if (!(f & 3)) continue;
+      ConstBox &S = cbox(X.src);
+      st<double>(T.f[F_U], boff(T, i, j, k), tinterp<KP>(S, S.f[F_U], X, t, fl));
+      return;
+    }
+  });
 }
 
 // column c of the ghost shell: the R shell points that differ along the
@@ -1863,16 +1906,17 @@ template <int KP>
 __global__ __launch_bounds__(kPtThreads) void k_ghost_cols(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  if (threadIdx.x >= Bk.nq) return;
-  int i, j, k, ax;
-  shell_column(T, Bk.q0 + threadIdx.x, i, j, k, ax);
-  const int p0[3] = {i, j, k};
-  if (ax == 0)
-    ghost_column<KP, 0>(Bk, T, p0);
-  else if (ax == 1)
-    ghost_column<KP, 1>(Bk, T, p0);
-  else
-    ghost_column<KP, 2>(Bk, T, p0);
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k, ax;
+    shell_column(T, Bk.q0 + off, i, j, k, ax);
+    const int p0[3] = {i, j, k};
+    if (ax == 0)
+      ghost_column<KP, 0>(Bk, T, p0);
+    else if (ax == 1)
+      ghost_column<KP, 1>(Bk, T, p0);
+    else
+      ghost_column<KP, 2>(Bk, T, p0);
+  });
 }
 
 // r = u on the ghost shell of the listed boxes (the fused sweeps read
@@ -1880,11 +1924,12 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_cols(const PtBlock *__rest
 __global__ __launch_bounds__(kPtThreads) void k_copy_shell(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &B = cbox(Bk.box);
-  if (threadIdx.x >= Bk.nq) return;
-  int i, j, k;
-  shell_point(B, Bk.q0 + threadIdx.x, i, j, k);
-  const unsigned o = boff(B, i, j, k);
-  st<double>(B.f[F_R], o, ld<double>(B.f[F_U], o));
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    shell_point(B, Bk.q0 + off, i, j, k);
+    const unsigned o = boff(B, i, j, k);
+    st<double>(B.f[F_R], o, ld<double>(B.f[F_U], o));
+  });
 }
 
 // ------------------------------------------------------------------
