@@ -286,12 +286,20 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
       // rows longer than a launch's threads are split into segments (64
       // slots at the smallest launch shape)
       rows += interior_rows(X) * ((X.rl + 63) / 64);
-      // points updated by colour pass q (flipped boxes update colour nc-1-q)
-      for (int k = X.R; k < X.n[2] - X.R; ++k)
-        for (int j = X.R; j < X.n[1] - X.R; ++j)
-          for (int i = X.R; i < X.n[0] - X.R; ++i) {
-            const int c = (i + j + k) % nc;
-            cnt[size_t(X.flip ? nc - 1 - c : c)] += 1.0;
+      // points updated by colour pass q (flipped boxes update colour
+      // nc-1-q): interior indices counted by residue mod nc per axis, the
+      // colour (i + j + k) mod nc from the three residues
+      std::vector<long long> hist[3];
+      for (int a = 0; a < 3; ++a) {
+        hist[a].assign(size_t(nc), 0);
+        for (int x = X.R; x < X.n[a] - X.R; ++x) ++hist[a][size_t(x % nc)];
+      }
+      for (int ri = 0; ri < nc; ++ri)
+        for (int rj = 0; rj < nc; ++rj)
+          for (int rk = 0; rk < nc; ++rk) {
+            const int c = (ri + rj + rk) % nc;
+            cnt[size_t(X.flip ? nc - 1 - c : c)] +=
+                double(hist[0][size_t(ri)] * hist[1][size_t(rj)] * hist[2][size_t(rk)]);
           }
     }
     // row kernels divide row indices with 24-bit exact float division
