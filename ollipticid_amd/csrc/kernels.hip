@@ -1959,14 +1959,13 @@ __device__ __forceinline__ int coarse_p(const CoarseGeo &G, int i, int j, int k)
   return c * G.cs + k * G.ps + j * G.rs + int((unsigned(i) * G.ncm) >> 16);
 }
 
-// Robin update at shell point (i, j, k) of the LDS field; MASK has bit a
+// Robin update at shell point (i, j, k) of the LDS field; mask has bit a
 // for each axis whose index is 0 or n-1 (dx2 one-sided derivative,
 // diff_2nd.cc:237-335).  The coarsest level is 2nd order, so its layout has
 // two colours: point (i, j, k) is slot i / 2 of row (j, k) in block
 // (i + j + k) & 1, and a step along an axis moves to the other block.
-template <int MASK>
 __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, const double *srho,
-                                             int i, int j, int k) {
+                                             int i, int j, int k, int mask) {
   const int c = (i + j + k) & 1;
   const int p = c * G.cs + k * G.ps + j * G.rs + (i >> 1);
   const int flip = (1 - 2 * c) * G.cs;
@@ -1978,7 +1977,7 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
   double sF = 0.0, sdF = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if (!(MASK & (1 << a))) continue;
+    if (!(mask & (1 << a))) continue;
     const bool lo = ijk[a] == 0;
     const int dir = lo ? 1 : -1;
     // offsets of the points 1 and 2 steps inwards along axis a
@@ -2013,9 +2012,9 @@ __device__ __forceinline__ void coarse_robin(const CoarseGeo &G, double *su, con
 // numbered row by row, ceil(ni / 2) slots per interior row).  With at most
 // 2048 points per colour over the workgroup (OWN * NT) the coefficients of
 // a thread's points stay in registers for the whole loop; beyond that they
-// would not fit the register file (the kernel spilled), so each colour's
-// coefficients are loaded from the (cache-resident) fields at the start of
-// the colour's pass.
+// would not fit the register file (the kernel spilled), so each point's
+// coefficients are loaded from the (cache-resident) fields as the point is
+// updated.
 template <int EQ, int OWN, int NT>
 __global__ __launch_bounds__(NT) void k_coarse_solve(const CoarseItem *__restrict__ items,
                                                                  double eps, int max_iter) {
@@ -2097,66 +2096,64 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(const CoarseItem *__restric
   do {
     prev = cur;
     for (int t = threadIdx.x; t < nface; t += NT) {
-      int r = t;
+      int i, j, k, mask, r = t;
       if (r < 2 * fx) {
         const int hi = r >= fx;
         r -= hi * fx;
         const int q = int(fast_div(unsigned(r), unsigned(nj), inj));
-        coarse_robin<1>(G, su, srho, hi ? n0 - 1 : 0, 1 + r - q * nj, 1 + q);
+        i = hi ? n0 - 1 : 0; j = 1 + r - q * nj; k = 1 + q; mask = 1;
       } else if ((r -= 2 * fx) < 2 * fy) {
         const int hi = r >= fy;
         r -= hi * fy;
         const int q = int(fast_div(unsigned(r), unsigned(ni), ini));
-        coarse_robin<2>(G, su, srho, 1 + r - q * ni, hi ? n1 - 1 : 0, 1 + q);
+        j = hi ? n1 - 1 : 0; i = 1 + r - q * ni; k = 1 + q; mask = 2;
       } else {
         r -= 2 * fy;
         const int hi = r >= fz;
         r -= hi * fz;
         const int q = int(fast_div(unsigned(r), unsigned(ni), ini));
-        coarse_robin<4>(G, su, srho, 1 + r - q * ni, 1 + q, hi ? n2 - 1 : 0);
+        k = hi ? n2 - 1 : 0; i = 1 + r - q * ni; j = 1 + q; mask = 4;
       }
+      coarse_robin(G, su, srho, i, j, k, mask);
     }
     __syncthreads();
     if (threadIdx.x < nedge) {
-      int t = threadIdx.x;
+      int t = threadIdx.x, i, j, k, mask;
       if (t < 4 * nk) {
         const int e = int(fast_div(unsigned(t), unsigned(nk), ink));
-        coarse_robin<3>(G, su, srho, (e >> 1) ? n0 - 1 : 0, (e & 1) ? n1 - 1 : 0, 1 + t - e * nk);
+        i = (e >> 1) ? n0 - 1 : 0; j = (e & 1) ? n1 - 1 : 0; k = 1 + t - e * nk; mask = 3;
       } else if ((t -= 4 * nk) < 4 * nj) {
         const int e = int(fast_div(unsigned(t), unsigned(nj), inj));
-        coarse_robin<5>(G, su, srho, (e >> 1) ? n0 - 1 : 0, 1 + t - e * nj, (e & 1) ? n2 - 1 : 0);
+        i = (e >> 1) ? n0 - 1 : 0; k = (e & 1) ? n2 - 1 : 0; j = 1 + t - e * nj; mask = 5;
       } else {
         t -= 4 * nj;
         const int e = int(fast_div(unsigned(t), unsigned(ni), ini));
-        coarse_robin<6>(G, su, srho, 1 + t - e * ni, (e & 1) ? n1 - 1 : 0, (e >> 1) ? n2 - 1 : 0);
+        k = (e >> 1) ? n2 - 1 : 0; j = (e & 1) ? n1 - 1 : 0; i = 1 + t - e * ni; mask = 6;
       }
+      coarse_robin(G, su, srho, i, j, k, mask);
     }
     __syncthreads();
     if (threadIdx.x < 8)
-      coarse_robin<7>(G, su, srho, (threadIdx.x >> 1) & 1 ? n0 - 1 : 0,
-                      (threadIdx.x >> 2) & 1 ? n1 - 1 : 0, threadIdx.x & 1 ? n2 - 1 : 0);
+      coarse_robin(G, su, srho, (threadIdx.x >> 1) & 1 ? n0 - 1 : 0, (threadIdx.x >> 2) & 1 ? n1 - 1 : 0,
+                   threadIdx.x & 1 ? n2 - 1 : 0, 7);
     __syncthreads();
     double mx = 0.0;
 #pragma unroll
     for (int col = 0; col < 2; ++col) {
-      // uncached: this colour's coefficients, all loads issued together
-      double cb[kCached ? 1 : OWN], ccc[kCached ? 1 : OWN], cd[kCached ? 1 : OWN],
-          ce[kCached ? 1 : OWN];
-      if constexpr (!kCached) {
-#pragma unroll
-        for (int m = 0; m < OWN; ++m) {
-          cb[m] = ccc[m] = cd[m] = ce[m] = 0.0;
-          if (own_p[col * OWN + m] >= 0) coeffs(col * OWN + m, cb[m], ccc[m], cd[m], ce[m]);
-        }
-      }
 #pragma unroll
       for (int m = 0; m < OWN; ++m) {
         const int q = col * OWN + m;
         if (own_p[q] < 0) continue;
-        const double qb = kCached ? own_b[kCached ? q : 0] : cb[kCached ? 0 : m];
-        const double qc = kCached ? own_cc[kCached ? q : 0] : ccc[kCached ? 0 : m];
-        const double qd = kCached ? own_d[kCached ? q : 0] : cd[kCached ? 0 : m];
-        const double qe = kCached ? own_e[kCached ? q : 0] : ce[kCached ? 0 : m];
+        // uncached: the point's coefficients loaded here (cache-resident)
+        double qb, qc, qd, qe;
+        if constexpr (kCached) {
+          qb = own_b[q];
+          qc = own_cc[q];
+          qd = own_d[q];
+          qe = own_e[q];
+        } else {
+          coeffs(q, qb, qc, qd, qe);
+        }
         const int p = own_p[q] & 0x3fffffff;
         const Nbr<2> nb = stencil_offs<2, 1>(col, own_p[q] >> 30, unsigned(G.cs), unsigned(G.rs),
                                              unsigned(G.ps));
