@@ -81,6 +81,31 @@ def test_mix_equals_alone():
         check_same(tag, ref[tag], got[tag])
 
 
+def chunk_argv(t, mixed):
+    argv = PUNC + ["-NumBH", "1", "-bhmp1", "0.5", "-bhx1", "%.2f" % (0.3 + 0.05 * t), "-L", "64",
+                   "-N", "65", "-l", "4", "-O", "4", "-nu2", "2", "-s", "1e-10"]
+    return argv + (["-prec", "mixed"] if mixed else [])
+
+
+def test_chunked_blocks_equal_alone():
+    """Launches over many requests give each transfer and whole-row block
+    several chunks of points or rows (16 requests with 67^3 boxes: 18.8k
+    prolongation blocks of one chunk, joined 4 to a block; the 8 FP64 and 8
+    mixed members' defect and residual blocks 2 to a block); every request
+    still gives bit for bit its solve alone."""
+    argvs = [chunk_argv(t, t >= 8) for t in range(16)]
+    ref = [alone(a) for a in argvs]
+    problems = {t: oa.problem_from_argv(a) for t, a in enumerate(argvs)}
+    e = oa.Engine()
+    for tag in problems:
+        assert e.admit(problems[tag], tag)
+    got = {}
+    run_to_end(e, problems, got)
+    assert sorted(got) == list(range(len(argvs)))
+    for tag in got:
+        check_same(tag, ref[tag], got[tag])
+
+
 def test_staggered_arrivals_equal_alone():
     """Continuous batching: requests join while others are mid-solve."""
     ref = [alone(a) for a in MIX]
