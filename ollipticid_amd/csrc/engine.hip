@@ -789,7 +789,7 @@ struct Builder {
   // only_box >= 0: that box of the level alone (one member)
   void robin(const std::vector<Hier *> &M, const std::vector<int> &lv, int mode, int only_box = -1) {
     std::vector<RobinItem> items;
-    int total = 0;
+    int total = 0, edges = 0;
     for (size_t h = 0; h < M.size(); ++h) {
       if (lv[h] < 0) continue;
       const Hier &H = *M[h];
@@ -797,8 +797,9 @@ struct Builder {
       for (int b = f; b < f + n; ++b) {
         if (only_box >= 0 && b != only_box) continue;
         const int nc = face_columns(H.hbox[size_t(b)]);
-        items.push_back(RobinItem{H.dbox + b, total, nc});
+        items.push_back(RobinItem{H.dbox + b, total, nc, edges});
         total += nc;
+        edges += robin_edge_sections(H.hbox[size_t(b)]);
       }
     }
     if (items.empty()) return;
@@ -807,6 +808,7 @@ struct Builder {
     L.off = S.add(items);
     L.n = int(items.size());
     L.total = total;
+    L.max_int = edges;
     S.launches.push_back(L);
   }
 
@@ -1125,7 +1127,8 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         launch_zero_lists(s, reinterpret_cast<const ZeroItem *>(items), L.n);
         break;
       case Launch::ROBIN:
-        launch_robin(s, reinterpret_cast<const RobinItem *>(items), L.n, int(L.total), L.mode);
+        launch_robin(s, reinterpret_cast<const RobinItem *>(items), L.n, int(L.total), L.max_int,
+                     L.mode);
         break;
       case Launch::SMOOTH:
         launch_smooth(s, reinterpret_cast<const RowBlock *>(items), L.n, L.max_row, L.order, L.eq,

@@ -97,7 +97,7 @@ struct Launch {
   long long total = 0;
   int max_row = 0;    // row kernels: threads per row
   int threads = kRowThreads;  // colour passes: threads per workgroup
-  int max_pts = 0, max_int = 0;
+  int max_pts = 0, max_int = 0;  // COARSE: largest box, colour slot list; ROBIN: edge cross-sections
   int pair = 0, parity = 0;  // SMOOTH_FUSED: two passes or one; 0 reads u, 1 reads r
   double bytes = 0;   // algorithmic bytes (smoother launches, FP64 or FP32)
   const DevBox *box1 = nullptr;  // row launches over a single box: its descriptor

@@ -889,45 +889,152 @@ __device__ __forceinline__ void robin_corner_update(const DevBox &B, double *cu,
   cu[q] = u0 - Fv / dF;
 }
 
-// edges then corners of one box, by one block (faces must be done);
-// cube and crho: 8 kCornerCube and 8 kCornerR3 doubles of LDS, for the
-// corner chains (R <= kCornerR)
-__device__ void robin_edges_corners(const DevBox &B, BoxData D, int mode, double *cube,
-                                    double *crho) {
+// One edge cross-section's Robin updates in the reference's order (outer
+// axis A, inner axis B, both counted inwards from the shell's sides), from
+// registers: the L-shaped set of points its one-sided derivatives read (the
+// R x R shell points and the 4 inward points along A and along B of each)
+// is loaded first, the R^2 updates chain through the registers with
+// robin_point's arithmetic, and the updated points are written back.
+// axa, axb: axes A and B; ha, hb: their sides; e_idx: the index along the
+// edge's own axis (the third).
+template <int R>
+__device__ void robin_edge_chain(const DevBox &B, BoxData D, int axa, int axb, int ha, int hb,
+                                 int e_idx) {
+  constexpr int W = R + 4;
+  double va[R][W];  // local A < R: every local B
+  double vb[4][R];  // local A >= R: local B < R
+  double rho[R][R];
+  // box indices of local point (la, lb)
+  auto index = [&](int la, int lb, int &i, int &j, int &k) {
+    const int pa = ha ? B.n[axa] - 1 - la : la, pb = hb ? B.n[axb] - 1 - lb : lb;
+    i = axa == 0 ? pa : (axb == 0 ? pb : e_idx);
+    j = axa == 1 ? pa : (axb == 1 ? pb : e_idx);
+    k = axa == 2 ? pa : (axb == 2 ? pb : e_idx);
+  };
+#pragma unroll
+  for (int la = 0; la < W; ++la)
+#pragma unroll
+    for (int lb = 0; lb < W; ++lb) {
+      if (la >= R && lb >= R) continue;
+      int i, j, k;
+      index(la, lb, i, j, k);
+      const long long p = pidx(B, i, j, k);
+      if (la < R) va[la][lb] = D.u[p]; else vb[la - R][lb] = D.u[p];
+      if (la < R && lb < R) rho[la][lb] = D.rho[p];
+    }
+  // robin_point takes the masked axes in ascending order
+  const bool a_first = axa < axb;
+#pragma unroll
+  for (int qa = 0; qa < R; ++qa)
+#pragma unroll
+    for (int qb = 0; qb < R; ++qb) {
+      const int la = R - 1 - qa, lb = R - 1 - qb;
+      int i, j, k;
+      index(la, lb, i, j, k);
+      const double X[3] = {coord(B, 0, i), coord(B, 1, j), coord(B, 2, k)};
+      const double r2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+      double sF = 0.0, sdF = 0.0;
+      const double u0 = va[la][lb];
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const bool along_a = a_first == (o == 0);
+        // the 4 inward points along the axis (la + m < W, lb + m < W)
+        double w[5];
+        w[0] = u0;
+#pragma unroll
+        for (int m = 1; m <= 4; ++m)
+          w[m] = along_a ? (la + m < R ? va[la + m][lb] : vb[la + m - R][lb]) : va[la][lb + m];
+        const int a = along_a ? axa : axb;
+        const int hs = along_a ? ha : hb;
+        const double id = B.idx[a];
+        const double xa = a == 0 ? X[0] : (a == 1 ? X[1] : X[2]);
+        double cf, dv;
+        if (B.order == 2) {
+          if (!hs) {
+            cf = -3.0 * 0.5 * id;
+            dv = (-3.0 * u0 + 4.0 * w[1] - w[2]) * 0.5 * id;
+          } else {
+            cf = 3.0 * 0.5 * id;
+            dv = (3.0 * u0 - 4.0 * w[1] + w[2]) * 0.5 * id;
+          }
+        } else if (!hs) {
+          cf = -25.0 * id / 12.0;
+          dv = -(25.0 * u0 - 48.0 * w[1] + 36.0 * w[2] - 16.0 * w[3] + 3.0 * w[4]) * id / 12.0;
+        } else {
+          cf = 25.0 * id / 12.0;
+          dv = (25.0 * u0 - 48.0 * w[1] + 36.0 * w[2] - 16.0 * w[3] + 3.0 * w[4]) * id / 12.0;
+        }
+        sF += dv / xa;
+        sdF += cf / xa;
+      }
+      const double op = r2 * B.ipn * sF + u0;
+      const double Fv = op - rho[la][lb];
+      const double dF = r2 * B.ipn * sdF + 1.0;
+      va[la][lb] = u0 - Fv / dF;
+    }
+#pragma unroll
+  for (int la = 0; la < R; ++la)
+#pragma unroll
+    for (int lb = 0; lb < R; ++lb) {
+      int i, j, k;
+      index(la, lb, i, j, k);
+      D.u[pidx(B, i, j, k)] = va[la][lb];
+    }
+}
+
+// Edge cross-section t of a box (faces must be done): its R x R shell
+// points in the reference's order.  Edges XY(4) run along z, XZ(4) along
+// y, YZ(4) along x; within an edge the cross-section is visited outer axis
+// first (x before y for XY, x before z for XZ, z before y for YZ), as
+// Fix_Points orders them.
+__device__ void robin_edge_section(const DevBox &B, BoxData D, int mode, int t) {
   const int R = B.R;
   const int nint[3] = {B.n[0] - 2 * R, B.n[1] - 2 * R, B.n[2] - 2 * R};
-  // edges XY(4) run along z, XZ(4) along y, YZ(4) along x; within an edge
-  // the cross-section is visited outer axis first (x before y for XY, x
-  // before z for XZ, z before y for YZ), as Fix_Points orders them.
-  const int tot = 4 * (nint[2] + nint[1] + nint[0]);
-  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-    int grp, e, s;
-    if (t < 4 * nint[2]) {
-      grp = 0; e = t / nint[2]; s = t % nint[2];
-    } else if (t < 4 * (nint[2] + nint[1])) {
-      grp = 1; e = (t - 4 * nint[2]) / nint[1]; s = (t - 4 * nint[2]) % nint[1];
-    } else {
-      grp = 2;
-      e = (t - 4 * (nint[2] + nint[1])) / nint[0];
-      s = (t - 4 * (nint[2] + nint[1])) % nint[0];
-    }
-    const int hA = (e >> 1) & 1, hB = e & 1;
-    for (int qa = 0; qa < R; ++qa)
-      for (int qb = 0; qb < R; ++qb) {
-        int i, j, k, mask;
-        if (grp == 0) {  // XiYi, XiYf, XfYi, XfYf: x outer, y inner; along z
-          i = shell_index(qa, hA, B.n[0], R); j = shell_index(qb, hB, B.n[1], R); k = R + s; mask = 3;
-        } else if (grp == 1) {  // XiZi, XiZf, XfZi, XfZf: x outer, z inner; along y
-          i = shell_index(qa, hA, B.n[0], R); k = shell_index(qb, hB, B.n[2], R); j = R + s; mask = 5;
-        } else {  // YiZi, YiZf, YfZi, YfZf: z outer, y inner; along x
-          j = shell_index(qb, hA, B.n[1], R); k = shell_index(qa, hB, B.n[2], R); i = R + s; mask = 6;
-        }
-        robin_point(B, D, i, j, k, mask, mode);
-      }
+  int grp, e, s;
+  if (t < 4 * nint[2]) {
+    grp = 0; e = t / nint[2]; s = t % nint[2];
+  } else if (t < 4 * (nint[2] + nint[1])) {
+    grp = 1; e = (t - 4 * nint[2]) / nint[1]; s = (t - 4 * nint[2]) % nint[1];
+  } else {
+    grp = 2;
+    e = (t - 4 * (nint[2] + nint[1])) / nint[0];
+    s = (t - 4 * (nint[2] + nint[1])) % nint[0];
   }
-  __syncthreads();
-  // corners: k outer, j, i inner (domain.cc:592-695); corner c has
-  // hz = bit 0, hx = bit 1, hy = bit 2
+  const int hA = (e >> 1) & 1, hB = e & 1;
+  if (mode == ROBIN_UPDATE && R <= 3) {
+    // outer axis A and inner axis B of the cross-section and their sides
+    // (the orders below); 8th-order boxes (R = 4) would not fit the
+    // registers and take the point-by-point path
+    const int axa = grp == 2 ? 2 : 0, axb = grp == 1 ? 2 : 1;
+    const int ha = grp == 2 ? hB : hA, hb = grp == 2 ? hA : hB;
+    if (R == 1)
+      robin_edge_chain<1>(B, D, axa, axb, ha, hb, R + s);
+    else if (R == 2)
+      robin_edge_chain<2>(B, D, axa, axb, ha, hb, R + s);
+    else
+      robin_edge_chain<3>(B, D, axa, axb, ha, hb, R + s);
+    return;
+  }
+  for (int qa = 0; qa < R; ++qa)
+    for (int qb = 0; qb < R; ++qb) {
+      int i, j, k, mask;
+      if (grp == 0) {  // XiYi, XiYf, XfYi, XfYf: x outer, y inner; along z
+        i = shell_index(qa, hA, B.n[0], R); j = shell_index(qb, hB, B.n[1], R); k = R + s; mask = 3;
+      } else if (grp == 1) {  // XiZi, XiZf, XfZi, XfZf: x outer, z inner; along y
+        i = shell_index(qa, hA, B.n[0], R); k = shell_index(qb, hB, B.n[2], R); j = R + s; mask = 5;
+      } else {  // YiZi, YiZf, YfZi, YfZf: z outer, y inner; along x
+        j = shell_index(qb, hA, B.n[1], R); k = shell_index(qa, hB, B.n[2], R); i = R + s; mask = 6;
+      }
+      robin_point(B, D, i, j, k, mask, mode);
+    }
+}
+
+// corners of one box, by one block (edges must be done): k outer, j, i
+// inner (domain.cc:592-695); corner c has hz = bit 0, hx = bit 1, hy = bit
+// 2.  cube and crho: 8 kCornerCube and 8 kCornerR3 doubles of LDS, for the
+// corner chains (R <= kCornerR)
+__device__ void robin_corners(const DevBox &B, BoxData D, int mode, double *cube, double *crho) {
+  const int R = B.R;
   if (mode != ROBIN_UPDATE) {
     // the operator modes leave u alone: every corner point at once
     const int R3 = R * R * R;
@@ -994,12 +1101,28 @@ __global__ __launch_bounds__(256) void k_robin_faces(const RobinItem *__restrict
   robin_face_column(B, box_data(B), g - items[lo].col0, mode);
 }
 
-// edges and corners: one block per box
-__global__ __launch_bounds__(1024) void k_robin_edges_corners(const RobinItem *__restrict__ items,
-                                                               int mode) {
+// edges: one thread per (box, edge cross-section), spread over many
+// workgroups (a box's edges are a chain of R^2 dependent updates each)
+constexpr int kRobinEdgeThreads = 128;
+__global__ __launch_bounds__(kRobinEdgeThreads) void k_robin_edges(const RobinItem *__restrict__ items,
+                                                                   int n, int total, int mode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].ecol0 <= g) lo = mid; else hi = mid - 1;
+  }
+  const DevBox &B = *items[lo].box;
+  robin_edge_section(B, box_data(B), mode, g - items[lo].ecol0);
+}
+
+// corners: one block per box
+__global__ __launch_bounds__(512) void k_robin_corners(const RobinItem *__restrict__ items,
+                                                       int mode) {
   __shared__ double cube[8 * kCornerCube], crho[8 * kCornerR3];
   const DevBox &B = *items[blockIdx.x].box;
-  robin_edges_corners(B, box_data(B), mode, cube, crho);
+  robin_corners(B, box_data(B), mode, cube, crho);
 }
 
 // Robin operator value at a shell point (no update)
@@ -2704,11 +2827,14 @@ void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
   OLL_DISPATCH_ORDER(order, 0, update32_impl, s, blocks, nblocks, per_row);
 }
 
-void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode) {
+void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int total_edges,
+                  int mode) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_robin_faces, dim3(ceil_div(total_cols, 256)), dim3(256), 0, s, items, n,
                      total_cols, mode);
-  hipLaunchKernelGGL(k_robin_edges_corners, dim3(n), dim3(1024), 0, s, items, mode);
+  hipLaunchKernelGGL(k_robin_edges, dim3(ceil_div(total_edges, kRobinEdgeThreads)),
+                     dim3(kRobinEdgeThreads), 0, s, items, n, total_edges, mode);
+  hipLaunchKernelGGL(k_robin_corners, dim3(n), dim3(512), 0, s, items, mode);
 }
 
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks) {

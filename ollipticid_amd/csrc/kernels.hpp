@@ -291,8 +291,9 @@ constexpr int kPtThreads = 256;
 // Robin stages: face columns col0..col0+ncol of a box
 struct RobinItem {
   const DevBox *box;
-  int col0;
+  int col0;   // first face column of the box in the launch
   int ncol;
+  int ecol0;  // first edge cross-section of the box in the launch
 };
 
 struct CoarseItem {
@@ -408,7 +409,12 @@ void launch_defect32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_smooth32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order,
                      int colour, int threads, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per_row, int order);
-void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int mode);
+// faces (total_cols columns), then edges (total_edges cross-sections),
+// then corners of the listed boxes: three launches
+void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int total_edges,
+                  int mode);
+// edge cross-sections of a box: 4 (n0 + n1 + n2 - 6R)
+inline int robin_edge_sections(const DevBox &B) { return 4 * (B.n[0] + B.n[1] + B.n[2] - 6 * B.R); }
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_restrict_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int order, int eq);
 // generic transfer of field ff of the listed blocks' sources into field
