@@ -896,6 +896,37 @@ struct Builder {
     return int(std::max<long long>(1, std::min<long long>(16, blocks / 4096)));
   }
 
+  // v = u - ww on the nodes of the coarse boxes of level lv[h] - 1 that the
+  // prolongation into level lv[h] reads (Hier::prolong_nodes), one block
+  // per row segment
+  void corr_v(const std::vector<Hier *> &M, const std::vector<int> &lv) {
+    std::vector<PtBlock> items;
+    for (size_t h = 0; h < M.size(); ++h) {
+      const int k = lv[h];
+      if (k < 1) continue;
+      const Hier &H = *M[h];
+      const int f = H.geo.level_first[size_t(k - 1)], n = H.geo.level_count[size_t(k - 1)];
+      for (int b = f; b < f + n; ++b) {
+        const std::array<int, 6> &pn = H.prolong_nodes[size_t(b)];
+        if (!(pn[0] < pn[3] && pn[1] < pn[4] && pn[2] < pn[5])) continue;
+        const DevBox &B = H.hbox[size_t(b)];
+        const long long plane = (long long)B.n[0] * B.n[1];
+        for (int kk = pn[2]; kk < pn[5]; ++kk)
+          for (int j = pn[1]; j < pn[4]; ++j)
+            for (int i0 = pn[0]; i0 < pn[3]; i0 += kPtThreads) {
+              const long long q0 = kk * plane + (long long)j * B.n[0] + i0;
+              items.push_back(PtBlock{H.dbox + b, nullptr, 0, 0, unsigned(q0),
+                                      unsigned(std::min(kPtThreads, pn[3] - i0)), i0, j, kk});
+            }
+      }
+    }
+    if (items.empty()) return;
+    Launch L{Launch::CORR_V};
+    L.off = S.add(items);
+    L.n = int(items.size());
+    S.launches.push_back(L);
+  }
+
   void transfer(Launch::Kind kind, const std::vector<Hier *> &M, const std::vector<int> &lv) {
     // lv[h] = the member's fine level k of the pair (k-1, k); restrictions
     // are grouped by the coarse level's (order, eq), ghost fills by the
@@ -1048,6 +1079,7 @@ int build_vcycle_from(const std::vector<Hier *> &M, const std::vector<int> &star
       lv[h] = (start[h] - d >= 1) ? start[h] - d : -1;
       mglv[h] = (lv[h] >= 1 && lv[h] <= M[h]->geo.mgrid) ? lv[h] : -1;
     }
+    b.corr_v(M, lv);
     b.transfer(Launch::PROLONG, M, lv);
     b.transfer(Launch::GHOST, M, lv);
     b.robin(M, mglv, ROBIN_UPDATE);
@@ -1158,6 +1190,9 @@ int issue(const Schedule &S, hipStream_t s, Timing *t, std::string &err) {
         if (!launch_coarse_solve(s, reinterpret_cast<const CoarseItem *>(items), L.n, L.max_pts,
                                  L.max_int, L.eq, 2 * DBL_EPSILON, 100000))
           return err = "coarsest box exceeds the LDS of one workgroup", OLL_EINVAL;
+        break;
+      case Launch::CORR_V:
+        launch_corr_v(s, reinterpret_cast<const PtBlock *>(items), L.n);
         break;
       case Launch::PROLONG:
         launch_prolong(s, reinterpret_cast<const PtBlock *>(items), L.n);

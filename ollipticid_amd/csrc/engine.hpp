@@ -88,7 +88,7 @@ int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err);
 struct Launch {
   enum Kind {
     ZERO, ROBIN, SMOOTH, RESIDUAL, RESTRICT_U, RESTRICT_DEFECT, COARSE, PROLONG, GHOST,
-    DEFECT32, SMOOTH32, UPDATE32, FILTER, SMOOTH_FUSED, COPY_SHELL
+    DEFECT32, SMOOTH32, UPDATE32, FILTER, SMOOTH_FUSED, COPY_SHELL, CORR_V
   };
   Kind kind;
   int order = 2, eq = 0, colour = 0, mode = 0;

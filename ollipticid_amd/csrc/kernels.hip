@@ -1315,10 +1315,10 @@ __device__ __forceinline__ double tinterp(ConstBox &S, const void *f, ConstXfer 
   return acc;
 }
 
-// trilinear interpolation of (u - ww) of box S (interpolation.cc:125-182)
+// trilinear interpolation of the coarse correction v = u - ww of box S
+// (interpolation.cc:125-182; k_corr_v forms v)
 __device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, const int t[3]) {
-  const void *u = S.f[F_U];
-  const void *ww = S.f[F_WW];
+  const void *v = S.f[F_V];
   const int x0 = X.ax[0].start[t[0]], y0 = X.ax[1].start[t[1]], z0 = X.ax[2].start[t[2]];
   const double *wx = X.ax[0].w + 2 * t[0];
   const double *wy = X.ax[1].w + 2 * t[1];
@@ -1343,7 +1343,7 @@ __device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, cons
   const unsigned o011 = o010 + wk.z(cz);
   unsigned cw = c0;
   const unsigned o001 = o000 + wk.z(cw);
-#define V(o) (ld<double>(u, o) - ld<double>(ww, o))
+#define V(o) ld<double>(v, o)
   return ((V(o000) * dxi + V(o100) * dxf) * dyi + (V(o010) * dxi + V(o110) * dxf) * dyf) * dzi +
          ((V(o001) * dxi + V(o101) * dxf) * dyi + (V(o011) * dxi + V(o111) * dxf) * dyf) * dzf;
 #undef V
@@ -1353,8 +1353,7 @@ __device__ __forceinline__ double trilinear_corr(ConstBox &S, ConstXfer &X, cons
 // weights (1, 0 at a coincident axis; 1/2, 1/2 at a midpoint) from the index
 // map instead of the tables, the eight corners walked as above
 __device__ __forceinline__ double trilinear_nested(ConstBox &S, ConstXfer &X, const int t[3]) {
-  const void *u = S.f[F_U];
-  const void *ww = S.f[F_WW];
+  const void *v = S.f[F_V];
   int c[3];
   double wi[3], wf[3];
 #pragma unroll
@@ -1386,7 +1385,7 @@ __device__ __forceinline__ double trilinear_nested(ConstBox &S, ConstXfer &X, co
   const unsigned o011 = o010 + wk.z(cz);
   unsigned cw = c0;
   const unsigned o001 = o000 + wk.z(cw);
-#define V(o) (ld<double>(u, o) - ld<double>(ww, o))
+#define V(o) ld<double>(v, o)
   return ((V(o000) * wi[0] + V(o100) * wf[0]) * wi[1] + (V(o010) * wi[0] + V(o110) * wf[0]) * wf[1]) * wi[2] +
          ((V(o001) * wi[0] + V(o101) * wf[0]) * wi[1] + (V(o011) * wi[0] + V(o111) * wf[0]) * wf[1]) * wf[2];
 #undef V
@@ -1678,8 +1677,22 @@ __global__ __launch_bounds__(kPtThreads) void k_restrict_defect(const PtBlock *_
   });
 }
 
-// prolongation of the coarse correction and u += v (all points on
-// multigrid levels, interior points on refinement levels)
+// v = u - ww on the listed points of a coarse box (the nodes the next
+// prolongation reads): the coarse correction, formed once per node instead
+// of once per fine point that interpolates from it
+__global__ __launch_bounds__(kPtThreads) void k_corr_v(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &S = cbox(Bk.box);
+  for_points(Bk, [&](unsigned off) {
+    int i, j, k;
+    pt_lex(Bk, S, off, i, j, k);
+    const unsigned o = boff(S, i, j, k);
+    st<double>(S.f[F_V], o, ld<double>(S.f[F_U], o) - ld<double>(S.f[F_WW], o));
+  });
+}
+
+// prolongation of the coarse correction v (k_corr_v) and u += I v (all
+// points on multigrid levels, interior points on refinement levels)
 __global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restrict__ blocks) {
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
@@ -1700,8 +1713,7 @@ __global__ __launch_bounds__(kPtThreads) void k_prolong(const PtBlock *__restric
       ConstBox &S = cbox(X.src);
       double v;
       if (f & 4) {
-        const unsigned c = boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]);
-        v = ld<double>(S.f[F_U], c) - ld<double>(S.f[F_WW], c);
+        v = ld<double>(S.f[F_V], boff(S, X.ax[0].coinc[i], X.ax[1].coinc[j], X.ax[2].coinc[k]));
       } else if (X.affine) {
         v = trilinear_nested(S, X, t);
       } else {
@@ -2855,6 +2867,11 @@ void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int
 }
 
 int filter_run_length() { return kFilterCx; }
+
+void launch_corr_v(hipStream_t s, const PtBlock *blocks, int nblocks) {
+  if (nblocks == 0) return;
+  hipLaunchKernelGGL(k_corr_v, dim3(nblocks), dim3(kPtThreads), 0, s, blocks);
+}
 
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks) {
   if (nblocks == 0) return;

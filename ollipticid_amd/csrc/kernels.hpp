@@ -429,6 +429,8 @@ bool launch_transfer_fields(hipStream_t s, const PtBlock *blocks, int nblocks, i
 // (ceil(n0 / length) runs per row), 0: blocks over points
 void launch_filter_defect(hipStream_t s, const PtBlock *blocks, int nblocks, int runs);
 int filter_run_length();
+// v = u - ww on the listed coarse points (before a prolongation)
+void launch_corr_v(hipStream_t s, const PtBlock *blocks, int nblocks);
 void launch_prolong(hipStream_t s, const PtBlock *blocks, int nblocks);
 // order: stencil order of the target (refinement) boxes; Lagrange order+1
 // columns = 1: blocks over shell columns (R points across a face slab, one
