@@ -817,7 +817,7 @@ struct Builder {
   // (shell: the ghost shell's points, or with columns its columns of R
   // points across the face slabs)
   static long long point_count(const DevBox &B, bool shell, bool columns) {
-    return shell ? (columns ? shell_pts(B) / B.R : shell_pts(B)) : npts(B);
+    return shell ? (columns ? ghost_column_slots(B) : shell_pts(B)) : npts(B);
   }
   // blocks points() gives level tl of H at reps = 1
   static long long point_blocks(const Hier &H, int tl, bool shell, bool columns = false) {

@@ -1783,42 +1783,55 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
   });
 }
 
-// column c of the ghost shell: the R shell points that differ along the
-// slab normal `axis` only (z slabs: columns (i, j); y slabs: (i, k) with k
-// interior; x slabs: (j, k) with j, k interior; low side first); returns
-// the column's first point.  The columns of a box are shell_pts / R.
-__device__ __forceinline__ void shell_column(ConstBox &B, unsigned q, int &i, int &j, int &k,
+// column slot q of the ghost shell (ghost_column_slots): the R shell
+// points that differ along the slab normal `axis` only (z slabs: columns
+// (i, j); y slabs: (i, k) with k interior; x slabs: (j, k) with j, k
+// interior; low side first, each face in tiles of kColTile^2 columns);
+// returns the column's first point, or false for a padding slot
+__device__ __forceinline__ bool face_tile_pos(unsigned r, unsigned f0, unsigned f1, unsigned &a0,
+                                              unsigned &a1) {
+  const unsigned tx = col_tiles(f0);
+  const unsigned tile = r / unsigned(kColTile * kColTile), w = r % unsigned(kColTile * kColTile);
+  const unsigned ty = fast_div(tile, tx, 1.0f / float(tx));
+  a0 = (tile - ty * tx) * kColTile + w % kColTile;
+  a1 = ty * kColTile + w / kColTile;
+  return a0 < f0 && a1 < f1;
+}
+__device__ __forceinline__ bool shell_column(ConstBox &B, unsigned q, int &i, int &j, int &k,
                                              int &axis) {
   const unsigned R = unsigned(B.R);
   const unsigned n0 = unsigned(B.n[0]), n1 = unsigned(B.n[1]), n2 = unsigned(B.n[2]);
-  const unsigned zc = n0 * n1;
+  unsigned a0, a1;
+  const unsigned zc = face_slots(n0, n1);
   if (q < 2 * zc) {
-    const unsigned s = q >= zc, r = q - s * zc;
-    j = int(fast_div(r, n0, B.in0));
-    i = int(r - unsigned(j) * n0);
+    const unsigned s = q >= zc;
+    if (!face_tile_pos(q - s * zc, n0, n1, a0, a1)) return false;
+    i = int(a0);
+    j = int(a1);
     k = s ? int(n2 - R) : 0;
     axis = 2;
-    return;
+    return true;
   }
   q -= 2 * zc;
-  const unsigned yc = n0 * (n2 - 2 * R);
+  const unsigned yc = face_slots(n0, n2 - 2 * R);
   if (q < 2 * yc) {
-    const unsigned s = q >= yc, r = q - s * yc;
-    const unsigned kk = fast_div(r, n0, B.in0);
-    i = int(r - kk * n0);
-    k = int(R + kk);
+    const unsigned s = q >= yc;
+    if (!face_tile_pos(q - s * yc, n0, n2 - 2 * R, a0, a1)) return false;
+    i = int(a0);
+    k = int(R + a1);
     j = s ? int(n1 - R) : 0;
     axis = 1;
-    return;
+    return true;
   }
   q -= 2 * yc;
-  const unsigned xc = (n1 - 2 * R) * (n2 - 2 * R);
-  const unsigned s = q >= xc, r = q - s * xc;
-  const unsigned kk = fast_div(r, n1 - 2 * R, B.inyi);
-  j = int(R + (r - kk * (n1 - 2 * R)));
-  k = int(R + kk);
+  const unsigned xc = face_slots(n1 - 2 * R, n2 - 2 * R);
+  const unsigned s = q >= xc;
+  if (!face_tile_pos(q - s * xc, n1 - 2 * R, n2 - 2 * R, a0, a1)) return false;
+  j = int(R + a0);
+  k = int(R + a1);
   i = s ? int(n0 - R) : 0;
   axis = 0;
+  return true;
 }
 
 // Ghost shell of refinement boxes, one thread per shell column (above).
@@ -2043,7 +2056,7 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_cols(const PtBlock *__rest
   ConstBox &T = cbox(Bk.box);
   for_points(Bk, [&](unsigned off) {
     int i, j, k, ax;
-    shell_column(T, Bk.q0 + off, i, j, k, ax);
+    if (!shell_column(T, Bk.q0 + off, i, j, k, ax)) return;
     const int p0[3] = {i, j, k};
     if (ax == 0)
       ghost_column<KP, 0>(Bk, T, p0);

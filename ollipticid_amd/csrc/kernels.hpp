@@ -413,6 +413,23 @@ void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 // then corners of the listed boxes: three launches
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int total_edges,
                   int mode);
+// Ghost-shell columns by faces (z, then y, then x slabs, low side first),
+// each face's columns in tiles of kColTile x kColTile, a tile's columns
+// consecutive (a wave of 64 threads takes one tile, so its stencil loads
+// cover a compact patch of the coarser box); the tiles at a face's far edges
+// are padded to full size and their extra slots idle.  Slots of one face
+// side of F0 x F1 columns:
+constexpr int kColTile = 8;
+__host__ __device__ inline unsigned col_tiles(unsigned f) { return (f + kColTile - 1) / kColTile; }
+__host__ __device__ inline unsigned face_slots(unsigned f0, unsigned f1) {
+  return col_tiles(f0) * col_tiles(f1) * unsigned(kColTile * kColTile);
+}
+// every column slot of a box's ghost shell
+inline long long ghost_column_slots(const DevBox &b) {
+  const unsigned R = unsigned(b.R), n0 = unsigned(b.n[0]), n1 = unsigned(b.n[1]),
+                 n2 = unsigned(b.n[2]);
+  return 2LL * (face_slots(n0, n1) + face_slots(n0, n2 - 2 * R) + face_slots(n1 - 2 * R, n2 - 2 * R));
+}
 // edge cross-sections of a box: 4 (n0 + n1 + n2 - 6R)
 inline int robin_edge_sections(const DevBox &B) { return 4 * (B.n[0] + B.n[1] + B.n[2] - 6 * B.R); }
 void launch_restrict_u(hipStream_t s, const PtBlock *blocks, int nblocks);
