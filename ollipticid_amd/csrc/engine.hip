@@ -817,7 +817,8 @@ struct Builder {
   // (shell: the ghost shell's points, or with columns its columns of R
   // points across the face slabs)
   static long long point_count(const DevBox &B, bool shell, bool columns) {
-    return shell ? (columns ? ghost_column_slots(B) : shell_pts(B)) : npts(B);
+    // shell points by column slot (k_ghost_fill: R points a slot)
+    return shell ? (columns ? ghost_column_slots(B) : ghost_column_slots(B) * B.R) : npts(B);
   }
   // blocks points() gives level tl of H at reps = 1
   static long long point_blocks(const Hier &H, int tl, bool shell, bool columns = false) {

@@ -1760,29 +1760,6 @@ __device__ __forceinline__ void shell_point(ConstBox &B, unsigned q, int &i, int
   i = s ? int(n0 - R + l) : int(l);
 }
 
-// ghost shell of refinement boxes from the coarser level (Lagrange
-// order+1, KP = order+2 nodes per axis); one thread per shell point
-template <int KP>
-__global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__restrict__ blocks) {
-  const PtBlock Bk = blocks[xcd_block()];
-  ConstBox &T = cbox(Bk.box);
-  for_points(Bk, [&](unsigned off) {
-    int i, j, k;
-    shell_point(T, Bk.q0 + off, i, j, k);
-    const int t[3] = {i, j, k};
-    ConstXfer *xs = (ConstXfer *)Bk.src;
-    for (int x = Bk.nsrc - 1; x >= 0; --x) {
-      ConstXfer &X = xs[x];
-      unsigned fl[3];
-      const unsigned f = axis_flags(X, t, fl);
-      if (!(f & 3)) continue;
-      ConstBox &S = cbox(X.src);
-      st<double>(T.f[F_U], boff(T, i, j, k), tinterp<KP>(S, S.f[F_U], X, t, fl));
-      return;
-    }
-  });
-}
-
 // column slot q of the ghost shell (ghost_column_slots): the R shell
 // points that differ along the slab normal `axis` only (z slabs: columns
 // (i, j); y slabs: (i, k) with k interior; x slabs: (j, k) with j, k
@@ -1832,6 +1809,35 @@ __device__ __forceinline__ bool shell_column(ConstBox &B, unsigned q, int &i, in
   i = s ? int(n0 - R) : 0;
   axis = 0;
   return true;
+}
+
+// ghost shell of refinement boxes from the coarser level (Lagrange
+// order+1, KP = order+2 nodes per axis); one thread per shell point,
+// point q being layer q mod R of column slot q / R (shell_column), so that
+// a wave's points cover a compact patch of a face
+template <int KP>
+__global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__restrict__ blocks) {
+  const PtBlock Bk = blocks[xcd_block()];
+  ConstBox &T = cbox(Bk.box);
+  const unsigned R = unsigned(T.R);
+  for_points(Bk, [&](unsigned off) {
+    const unsigned q = Bk.q0 + off, c = q / R;
+    int i0, j0, k0, ax;
+    if (!shell_column(T, c, i0, j0, k0, ax)) return;
+    const int l = int(q - c * R);
+    const int i = i0 + (ax == 0 ? l : 0), j = j0 + (ax == 1 ? l : 0), k = k0 + (ax == 2 ? l : 0);
+    const int t[3] = {i, j, k};
+    ConstXfer *xs = (ConstXfer *)Bk.src;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      ConstXfer &X = xs[x];
+      unsigned fl[3];
+      const unsigned f = axis_flags(X, t, fl);
+      if (!(f & 3)) continue;
+      ConstBox &S = cbox(X.src);
+      st<double>(T.f[F_U], boff(T, i, j, k), tinterp<KP>(S, S.f[F_U], X, t, fl));
+      return;
+    }
+  });
 }
 
 // Ghost shell of refinement boxes, one thread per shell column (above).
