@@ -1768,10 +1768,11 @@ __device__ __forceinline__ void shell_point(ConstBox &B, unsigned q, int &i, int
 __device__ __forceinline__ bool face_tile_pos(unsigned r, unsigned f0, unsigned f1, unsigned &a0,
                                               unsigned &a1) {
   const unsigned tx = col_tiles(f0);
-  const unsigned tile = r / unsigned(kColTile * kColTile), w = r % unsigned(kColTile * kColTile);
-  const unsigned ty = fast_div(tile, tx, 1.0f / float(tx));
-  a0 = (tile - ty * tx) * kColTile + w % kColTile;
-  a1 = ty * kColTile + w / kColTile;
+  const unsigned st = r / unsigned(kColSuper * kColSuper), w = r % unsigned(kColSuper * kColSuper);
+  const unsigned sy = fast_div(st, tx, 1.0f / float(tx));
+  const unsigned wave = w / unsigned(kColTile * kColTile), lane = w % unsigned(kColTile * kColTile);
+  a0 = (st - sy * tx) * kColSuper + (wave & 1u) * kColTile + lane % kColTile;
+  a1 = sy * kColSuper + (wave >> 1) * kColTile + lane / kColTile;
   return a0 < f0 && a1 < f1;
 }
 __device__ __forceinline__ bool shell_column(ConstBox &B, unsigned q, int &i, int &j, int &k,
@@ -1848,8 +1849,65 @@ __global__ __launch_bounds__(kPtThreads) void k_ghost_fill(const PtBlock *__rest
 // then y, then x, ascending), so every value is tinterp's to the bit.
 // Columns whose points take different sources fall back to tinterp point
 // by point.
+// The nodes a column's fast path reads (the per-axis ranges below: the
+// two axes across the slab and the union along the normal) and its source;
+// false when the column takes the point-by-point fallback
 template <int KP, int ax>
-__device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, const int p0[3]) {
+__device__ __forceinline__ bool column_region(const PtBlock &Bk, const int p0[3], int &src,
+                                              int lo3[3], int hi3[3]) {
+  constexpr int KU = KP + 3;
+  constexpr int RG = (KP - 2) / 2;
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  unsigned fn[RG], fl0[3] = {0, 0, 0};
+  src = -1;
+#pragma unroll
+  for (int l = 0; l < RG; ++l) {
+    int sl = -1;
+    fn[l] = 0;
+    int t[3] = {p0[0], p0[1], p0[2]};
+    t[ax] += l;
+    for (int x = Bk.nsrc - 1; x >= 0; --x) {
+      unsigned fl[3];
+      const unsigned f = axis_flags(xs[x], t, fl);
+      if (!(f & 3)) continue;
+      sl = x;
+      fn[l] = fl[ax];
+      if (l == 0) {
+        fl0[0] = fl[0];
+        fl0[1] = fl[1];
+        fl0[2] = fl[2];
+      }
+      break;
+    }
+    if (l == 0) src = sl;
+    if (sl < 0 || sl != src) return false;
+  }
+  ConstXfer &X = xs[src];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a == ax) continue;
+    lo3[a] = (fl0[a] & 4) ? X.ax[a].coinc[p0[a]] : X.ax[a].start[p0[a]];
+    hi3[a] = lo3[a] + ((fl0[a] & 4) ? 1 : KP);
+  }
+  int lo = 1 << 30, hi = -1;
+#pragma unroll
+  for (int l = 0; l < RG; ++l) {
+    const int t = p0[ax] + l;
+    const int first = (fn[l] & 4) ? X.ax[ax].coinc[t] : X.ax[ax].start[t];
+    lo = min(lo, first);
+    hi = max(hi, first + ((fn[l] & 4) ? 1 : KP) - 1);
+  }
+  if (hi - lo + 1 > KU) return false;
+  lo3[ax] = lo;
+  hi3[ax] = hi + 1;
+  return true;
+}
+
+// STAGED: the column's source nodes are read from lds, the workgroup's copy
+// of the source box's nodes [lb, lb + (nx, ny, ...)) (k_ghost_cols)
+template <int KP, int ax, bool STAGED>
+__device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, const int p0[3],
+                                             const double *lds, const int lb[3], int nx, int ny) {
   constexpr int KU = KP + 3;       // union of the column's node ranges, at most
   constexpr int RG = (KP - 2) / 2;  // ghost width of the boxes of this stencil
   ConstXfer *xs = (ConstXfer *)Bk.src;
@@ -1978,7 +2036,12 @@ __device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, con
         double v[KP];
 #pragma unroll
         for (int K = 0; K < KP; ++K) {
-          v[K] = (J < U && K < kk[2]) ? ld<double>(f, off) : 0.0;
+          if constexpr (STAGED)
+            v[K] = (J < U && K < kk[2]) ? lds[((s0[2] + K - lb[2]) * ny + (s0[1] + J - lb[1])) * nx +
+                                              (s0[0] + I - lb[0])]
+                                        : 0.0;
+          else
+            v[K] = (J < U && K < kk[2]) ? ld<double>(f, off) : 0.0;
           off += wk.z(c);
         }
         double s = 0.0;
@@ -2002,7 +2065,12 @@ __device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, con
           double v[KU];
 #pragma unroll
           for (int K = 0; K < KU; ++K) {
-            v[K] = K < U ? ld<double>(f, off) : 0.0;
+            if constexpr (STAGED)
+              v[K] = K < U ? lds[((s0[2] + K - lb[2]) * ny + (s0[1] + J - lb[1])) * nx +
+                                 (s0[0] + I - lb[0])]
+                           : 0.0;
+            else
+              v[K] = K < U ? ld<double>(f, off) : 0.0;
             off += wk.z(c);
           }
 #pragma unroll
@@ -2018,7 +2086,12 @@ __device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, con
           double v[KP];
 #pragma unroll
           for (int K = 0; K < KP; ++K) {
-            v[K] = K < kk[2] ? ld<double>(f, off) : 0.0;
+            if constexpr (STAGED)
+              v[K] = K < kk[2] ? lds[((s0[2] + K - lb[2]) * ny + (s0[1] + J - lb[1])) * nx +
+                                     (s0[0] + I - lb[0])]
+                               : 0.0;
+            else
+              v[K] = K < kk[2] ? ld<double>(f, off) : 0.0;
             off += wk.z(c);
           }
           double s = 0.0;
@@ -2056,21 +2129,78 @@ __device__ __forceinline__ void ghost_column(const PtBlock &Bk, ConstBox &T, con
   }
 }
 
+// A workgroup takes one kColSuper^2 tile of a face's columns; when the
+// columns on the fast path share their source, the nodes they read (their
+// bounding box, at most kGhostLds) are staged in LDS once and every column
+// reads its stencil from there.  The same values in the same order either
+// way.
+constexpr int kGhostLds = 4608;
 template <int KP>
 __global__ __launch_bounds__(kPtThreads) void k_ghost_cols(const PtBlock *__restrict__ blocks) {
+  __shared__ double lds[kGhostLds];
+  __shared__ int red[9];  // lo[3], hi[3], min and max source, a column off the fast path
   const PtBlock Bk = blocks[xcd_block()];
   ConstBox &T = cbox(Bk.box);
-  for_points(Bk, [&](unsigned off) {
-    int i, j, k, ax;
-    if (!shell_column(T, Bk.q0 + off, i, j, k, ax)) return;
-    const int p0[3] = {i, j, k};
-    if (ax == 0)
-      ghost_column<KP, 0>(Bk, T, p0);
-    else if (ax == 1)
-      ghost_column<KP, 1>(Bk, T, p0);
-    else
-      ghost_column<KP, 2>(Bk, T, p0);
-  });
+  ConstXfer *xs = (ConstXfer *)Bk.src;
+  // every thread walks the same chunks: the barriers below are uniform
+  for (unsigned base = 0; base < Bk.nq; base += kPtThreads) {
+    int p0[3] = {0, 0, 0}, ax = 0;
+    const bool valid =
+        base + threadIdx.x < Bk.nq && shell_column(T, Bk.q0 + base + threadIdx.x, p0[0], p0[1], p0[2], ax);
+    int src = -1, lo3[3] = {0, 0, 0}, hi3[3] = {0, 0, 0};
+    bool fast = false;
+    if (valid) {
+      if (ax == 0)
+        fast = column_region<KP, 0>(Bk, p0, src, lo3, hi3);
+      else if (ax == 1)
+        fast = column_region<KP, 1>(Bk, p0, src, lo3, hi3);
+      else
+        fast = column_region<KP, 2>(Bk, p0, src, lo3, hi3);
+    }
+    if (threadIdx.x < 9) red[threadIdx.x] = threadIdx.x < 3 ? (1 << 30) : (threadIdx.x == 6 ? (1 << 30) : -1);
+    __syncthreads();
+    if (fast) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&red[a], lo3[a]);
+        atomicMax(&red[3 + a], hi3[a]);
+      }
+      atomicMin(&red[6], src);
+      atomicMax(&red[7], src);
+    }
+    __syncthreads();
+    const int lb[3] = {red[0], red[1], red[2]};
+    const int nx = red[3] - red[0], ny = red[4] - red[1], nz = red[5] - red[2];
+    const bool staged = red[6] == red[7] && nx > 0 && ny > 0 && nz > 0 && nx * ny * nz <= kGhostLds;
+    if (staged) {
+      ConstBox &S = cbox(xs[red[6]].src);
+      const void *f = S.f[F_U];
+      const int nxy = nx * ny;
+      for (int q = threadIdx.x; q < nxy * nz; q += kPtThreads) {
+        const int z = q / nxy, r = q - z * nxy, y = r / nx, x = r - y * nx;
+        lds[q] = ld<double>(f, boff(S, lb[0] + x, lb[1] + y, lb[2] + z));
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      if (staged && fast) {
+        if (ax == 0)
+          ghost_column<KP, 0, true>(Bk, T, p0, lds, lb, nx, ny);
+        else if (ax == 1)
+          ghost_column<KP, 1, true>(Bk, T, p0, lds, lb, nx, ny);
+        else
+          ghost_column<KP, 2, true>(Bk, T, p0, lds, lb, nx, ny);
+      } else {
+        if (ax == 0)
+          ghost_column<KP, 0, false>(Bk, T, p0, nullptr, lb, 0, 0);
+        else if (ax == 1)
+          ghost_column<KP, 1, false>(Bk, T, p0, nullptr, lb, 0, 0);
+        else
+          ghost_column<KP, 2, false>(Bk, T, p0, nullptr, lb, 0, 0);
+      }
+    }
+    __syncthreads();  // lds and red are reused by the next chunk
+  }
 }
 
 // r = u on the ghost shell of the listed boxes (the fused sweeps read

@@ -414,15 +414,16 @@ void launch_update32(hipStream_t s, const RowBlock *blocks, int nblocks, int per
 void launch_robin(hipStream_t s, const RobinItem *items, int n, int total_cols, int total_edges,
                   int mode);
 // Ghost-shell columns by faces (z, then y, then x slabs, low side first),
-// each face's columns in tiles of kColTile x kColTile, a tile's columns
-// consecutive (a wave of 64 threads takes one tile, so its stencil loads
-// cover a compact patch of the coarser box); the tiles at a face's far edges
-// are padded to full size and their extra slots idle.  Slots of one face
-// side of F0 x F1 columns:
+// each face's columns in tiles of kColSuper x kColSuper, one per workgroup
+// of 256 threads, each made of four kColTile x kColTile tiles, one per wave
+// (so a wave's and a workgroup's stencils cover compact patches of the
+// coarser box); the tiles at a face's far edges are padded to full size and
+// their extra slots idle.  Slots of one face side of F0 x F1 columns:
 constexpr int kColTile = 8;
-__host__ __device__ inline unsigned col_tiles(unsigned f) { return (f + kColTile - 1) / kColTile; }
+constexpr int kColSuper = 16;
+__host__ __device__ inline unsigned col_tiles(unsigned f) { return (f + kColSuper - 1) / kColSuper; }
 __host__ __device__ inline unsigned face_slots(unsigned f0, unsigned f1) {
-  return col_tiles(f0) * col_tiles(f1) * unsigned(kColTile * kColTile);
+  return col_tiles(f0) * col_tiles(f1) * unsigned(kColSuper * kColSuper);
 }
 // every column slot of a box's ghost shell
 inline long long ghost_column_slots(const DevBox &b) {
