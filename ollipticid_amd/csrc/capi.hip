@@ -268,6 +268,11 @@ int oll_set_field(oll_hier *h, int ibox, int field, const double *host) {
   CAPI_CHECK(hipMemcpyAsync(b.f[field], st.data(), st.size() * sizeof(double),
                             hipMemcpyHostToDevice, h->stream));
   CAPI_CHECK(hipStreamSynchronize(h->stream));
+  if (field == F_RHO) {  // the caller's rho may be nonzero anywhere in the box
+    std::string err;
+    const int rc = hier_rho_whole(h->H, ibox, 1, h->stream, err);
+    if (rc) return fail(rc, err);
+  }
   return OLL_OK;
 }
 

@@ -184,7 +184,57 @@ int upload(T **dst, const std::vector<T> &v, size_t &bytes, std::string &err) {
   return OLL_OK;
 }
 
+// adds to cnt[q] the interior points of X inside [lo, hi) that colour
+// pass q updates (flipped boxes update colour nc-1-q): indices counted by
+// residue mod nc per axis, the colour (i + j + k) mod nc from the three
+// residues
+void colour_counts(const DevBox &X, const int lo[3], const int hi[3], std::vector<double> &cnt) {
+  const int nc = X.nc;
+  std::vector<long long> hist[3];
+  for (int a = 0; a < 3; ++a) {
+    hist[a].assign(size_t(nc), 0);
+    for (int x = std::max(X.R, lo[a]); x < std::min(X.n[a] - X.R, hi[a]); ++x)
+      ++hist[a][size_t(x % nc)];
+  }
+  for (int ri = 0; ri < nc; ++ri)
+    for (int rj = 0; rj < nc; ++rj)
+      for (int rk = 0; rk < nc; ++rk) {
+        const int c = (ri + rj + rk) % nc;
+        cnt[size_t(X.flip ? nc - 1 - c : c)] +=
+            double(hist[0][size_t(ri)] * hist[1][size_t(rj)] * hist[2][size_t(rk)]);
+      }
+}
+
 }  // namespace
+
+// per colour pass, the points of level l's boxes inside their rho boxes
+std::vector<double> level_rho_points(const Hier &H, int l) {
+  const int f = H.geo.level_first[size_t(l)];
+  std::vector<double> cnt(size_t(H.hbox[size_t(f)].nc), 0.0);
+  for (int b = 0; b < H.geo.level_count[size_t(l)]; ++b) {
+    const DevBox &X = H.hbox[size_t(f + b)];
+    colour_counts(X, X.rlo, X.rhi, cnt);
+  }
+  return cnt;
+}
+
+int hier_rho_whole(Hier &H, int first, int count, hipStream_t s, std::string &err) {
+  bool changed = false;
+  for (int b = first; b < first + count; ++b) {
+    DevBox &d = H.hbox[size_t(b)];
+    for (int a = 0; a < 3; ++a) {
+      changed |= d.rlo[a] != 0 || d.rhi[a] != d.n[a];
+      d.rlo[a] = 0;
+      d.rhi[a] = d.n[a];
+    }
+  }
+  if (!changed) return OLL_OK;
+  ENG_CHECK(hipMemcpyAsync(H.dbox + first, H.hbox.data() + first, size_t(count) * sizeof(DevBox),
+                           hipMemcpyHostToDevice, s));
+  ENG_CHECK(hipStreamSynchronize(s));
+  for (int l = 0; l < H.geo.levels; ++l) H.colour_rho[size_t(l)] = level_rho_points(H, l);
+  return OLL_OK;
+}
 
 int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err) {
   int rc = validate(p, err);
@@ -270,6 +320,10 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
     d.ipn = ipn;
     set_layout(d, colours_of(b.order, p.eq));
     d.flip = colour_flip(g, b, d.nc);
+    for (int a = 0; a < 3; ++a) {  // the whole box until the transfer tables narrow it
+      d.rlo[a] = 0;
+      d.rhi[a] = d.n[a];
+    }
     for (int f = 0; f < NFIELDS; ++f)
       d.f[f] = f < nf ? H.slab + size_t(f) * size_t(total) + size_t(off[ib]) : nullptr;
     for (int f = 0; f < NFIELDS32; ++f)
@@ -286,21 +340,8 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
       // rows longer than a launch's threads are split into segments (64
       // slots at the smallest launch shape)
       rows += interior_rows(X) * ((X.rl + 63) / 64);
-      // points updated by colour pass q (flipped boxes update colour
-      // nc-1-q): interior indices counted by residue mod nc per axis, the
-      // colour (i + j + k) mod nc from the three residues
-      std::vector<long long> hist[3];
-      for (int a = 0; a < 3; ++a) {
-        hist[a].assign(size_t(nc), 0);
-        for (int x = X.R; x < X.n[a] - X.R; ++x) ++hist[a][size_t(x % nc)];
-      }
-      for (int ri = 0; ri < nc; ++ri)
-        for (int rj = 0; rj < nc; ++rj)
-          for (int rk = 0; rk < nc; ++rk) {
-            const int c = (ri + rj + rk) % nc;
-            cnt[size_t(X.flip ? nc - 1 - c : c)] +=
-                double(hist[0][size_t(ri)] * hist[1][size_t(rj)] * hist[2][size_t(rk)]);
-          }
+      const int lo[3] = {0, 0, 0};
+      colour_counts(X, lo, X.n, cnt);
     }
     // row kernels divide row indices with 24-bit exact float division
     if ((long long)rows * nc >= (1LL << 24))
@@ -328,6 +369,31 @@ int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err) {
       H.gf[size_t(k)].push_back(P.add(g, tb, k - 1, 1, p.order + 1));
     }
   }
+  // rho boxes: rho stays +0 at the interior points no restriction writes.
+  // A box of the top level is no restriction's target; a multigrid level's
+  // box is covered whole by the finer one; otherwise the target points
+  // inside some finer box's non-ghost part (XRef lo, hi) bound the FAS
+  // source.  Poisson's rho is its source term, nonzero everywhere
+  if (p.eq != 2)
+    for (int l = 0; l < L; ++l)
+      for (int t = 0; t < g.level_count[size_t(l)]; ++t) {
+        DevBox &d = H.hbox[size_t(g.level_first[size_t(l)] + t)];
+        if (l + 1 <= g.mgrid) continue;
+        for (int a = 0; a < 3; ++a) {
+          d.rlo[a] = 0;
+          d.rhi[a] = 0;
+        }
+        if (l + 1 >= L) continue;
+        const Hier::XRef &r = H.rw[size_t(l + 1)][size_t(t)];
+        if (r.lo[0] < r.hi[0] && r.lo[1] < r.hi[1] && r.lo[2] < r.hi[2])
+          for (int a = 0; a < 3; ++a) {
+            d.rlo[a] = r.lo[a];
+            d.rhi[a] = r.hi[a];
+          }
+      }
+  ENG_CHECK(hipMemcpy(H.dbox, H.hbox.data(), H.hbox.size() * sizeof(DevBox), hipMemcpyHostToDevice));
+  H.colour_rho.clear();
+  for (int l = 0; l < L; ++l) H.colour_rho.push_back(level_rho_points(H, l));
   H.prolong_nodes.assign(g.boxes.size(), std::array<int, 6>{1 << 30, 1 << 30, 1 << 30, 0, 0, 0});
   for (const auto &kv : P.src_nodes) H.prolong_nodes[size_t(kv.first)] = kv.second;
   if ((rc = upload(&H.ipool, P.ip, H.bytes, err))) return rc;
@@ -443,6 +509,10 @@ int transfer_fields(Hier &H, hipStream_t s, int from, int ffield, int to, int tf
   Pools P;
   std::vector<Hier::XRef> refs;
   const int ft = g.level_first[size_t(to)], nt = g.level_count[size_t(to)];
+  if (tfield == F_RHO) {  // rho may now be nonzero anywhere on the target level
+    const int rc = hier_rho_whole(H, ft, nt, s, err);
+    if (rc) return rc;
+  }
   for (int t = 0; t < nt; ++t) refs.push_back(P.add(g, ft + t, from, method, method == 0 ? 1 : ord));
   int *ip = nullptr;
   unsigned char *bp = nullptr;
@@ -688,10 +758,14 @@ struct Builder {
         for (int b = 0; b < H.geo.level_count[size_t(l)]; ++b)
           boxes.insert(H.dbox + H.geo.level_first[size_t(l)] + b);
         // algorithmic bytes per updated point: FP64 reads u, b, c, rho
-        // (+ d, e) and writes u; FP32 reads e, q, r and writes e
+        // (+ d, e) and writes u, rho only inside the box's rho box; FP32
+        // reads e, q, r and writes e
         const double per = mixed ? 16.0
                            : (eq == OLL_EQ_PUNCTURES_SCALAR ? 56.0 : (eq == OLL_EQ_PUNCTURES_EM ? 88.0 : 40.0));
-        for (int q = 0; q < nc; ++q) bytes[q] += per * H.colour_pts[size_t(l)][size_t(q)];
+        for (int q = 0; q < nc; ++q) {
+          const double pts = H.colour_pts[size_t(l)][size_t(q)];
+          bytes[q] += per * pts - (mixed ? 0.0 : 8.0 * (pts - H.colour_rho[size_t(l)][size_t(q)]));
+        }
       }
       // FP64 sweeps in pairs as fused double-buffered launches (an even
       // count, so u ends in its own field); the rest one colour at a time

@@ -55,6 +55,7 @@ struct Hier {
   PunctureArgs pa{};
   double eps_solv = 0, eps_coarse = 0, coarse_omega = 1;
   std::vector<std::vector<double>> colour_pts;  // [level][colour]
+  std::vector<std::vector<double>> colour_rho;  // of them inside the boxes' rho boxes
   // V-cycle state
   int cycle = 0;
   double res = 1e12, res_prev = 1e12;
@@ -81,6 +82,11 @@ struct Hier {
 // coefficients; returns an OLL_* code and sets err
 int hier_build(Hier &H, const oll_problem &p, hipStream_t s, std::string &err);
 void hier_release(Hier &H);
+// per colour pass, the points of level l inside their boxes' rho boxes
+std::vector<double> level_rho_points(const Hier &H, int l);
+// boxes first .. first + count - 1 read rho everywhere from now on (after a
+// caller writes the field); waits for the descriptor copy on s
+int hier_rho_whole(Hier &H, int first, int count, hipStream_t s, std::string &err);
 int hier_reset(Hier &H, hipStream_t s, std::string &err);
 // host-only footprint of a problem's hierarchy (bytes of HBM)
 int problem_footprint(const oll_problem &p, size_t &bytes, std::string &err);

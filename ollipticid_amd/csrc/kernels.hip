@@ -237,6 +237,17 @@ template <typename T>
 __device__ __forceinline__ void st(void *base, unsigned off, T v) {
   *(__attribute__((address_space(1))) T *)((gchar *)base + off) = v;
 }
+// rho at byte offset o of point (i, j, k): +0 outside B's rho box without
+// a load.  Callers take it before the stencil so that the load, when there
+// is one, is in flight with the stencil's
+__device__ __forceinline__ double rho_at(ConstBox &B, unsigned o, int i, int j, int k) {
+  const bool in = (unsigned(i - B.rlo[0]) < unsigned(B.rhi[0] - B.rlo[0])) &
+                  (unsigned(j - B.rlo[1]) < unsigned(B.rhi[1] - B.rlo[1])) &
+                  (unsigned(k - B.rlo[2]) < unsigned(B.rhi[2] - B.rlo[2]));
+  double r = 0.0;
+  if (in) r = ld<double>(B.f[F_RHO], o);
+  return r;
+}
 
 // axis-star Laplacian at byte offset o of a field; neighbour byte offsets
 // from stencil_offs<NC, sizeof(T)>
@@ -459,10 +470,11 @@ __device__ __forceinline__ void smooth_rows(const RowBlock &Bk, ConstBox &B, uns
     const Nbr<NC> nb = box_offs<NC, 8>(B, col, s);
     const unsigned o = p * 8u;
     double *u = B.f[F_U];
+    const double rho = rho_at(B, o, i, j, k);
     const double u0 = ld<double>(u, o);
     double N, dN;
     point_op<ORD, EQ>(B, u, o, nb, u0, i, j, k, N, dN);
-    const double res = N - ld<double>(B.f[F_RHO], o);
+    const double res = N - rho;
     const double un = u0 - res / dN;
     st<double>(u, o, un);
     rmax = fabs(res);
@@ -620,10 +632,11 @@ __global__ __launch_bounds__(kRowThreads) void k_residual(const RowBlock *__rest
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u;
     const double *u = B.f[F_U];
+    const double rho = mode == RES_DEFECT ? rho_at(B, o, i, j, k) : 0.0;
     double N, dN;
     point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), i, j, k, N, dN);
     if (mode == RES_DEFECT)
-      st<double>(B.f[F_W], o, ld<double>(B.f[F_RHO], o) - N);
+      st<double>(B.f[F_W], o, rho - N);
     else
       st<double>(B.f[F_R], o, mode == RES_DERIVATIVE ? dN : N);
     rmax = nan_max(rmax, fabs(N));
@@ -655,9 +668,10 @@ __global__ __launch_bounds__(kRowThreads) void k_defect32(const RowBlock *__rest
     const Nbr<NC> nb = box_offs<NC, 8>(B, c, s);
     const unsigned o = p * 8u, o4 = p * 4u;
     const double *u = B.f[F_U];
+    const double rho = rho_at(B, o, i, j, k);
     double N, dN;
     point_op<ORD, EQ>(B, u, o, nb, ld<double>(u, o), i, j, k, N, dN);
-    const double res = ld<double>(B.f[F_RHO], o) - N;
+    const double res = rho - N;
     st<float>(B.g[G_R], o4, float(res));
     st<float>(B.g[G_Q], o4, float(dN - B.dulap));
     st<float>(B.g[G_E], o4, 0.0f);

@@ -58,6 +58,12 @@ struct DevBox {
   // of colour 1 sweep their colours in descending order (colour pass q
   // updates colour 1 - q): the coincident colour is swept first
   int flip;
+  // index box [rlo, rhi) outside which rho is +0 at every interior point:
+  // the coefficients write 0 there and afterwards only the FAS source of a
+  // restriction writes rho, at the points some finer box covers.  The
+  // smoother and the defect read rho inside it only (x - (+0) and (+0) - x
+  // are the same bits as x - rho and rho - x there)
+  int rlo[3], rhi[3];
 };
 
 constexpr int kMaxAxisPoints = 4096;

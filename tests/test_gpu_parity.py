@@ -103,6 +103,46 @@ def test_smooth_sweeps_every_level(case):
             assert rel_err(h.get_field(ib, oa.F_U), s.field(ib, O.F_U)) < 1e-12, (case, lv, ib)
 
 
+@pytest.mark.parametrize("case", ["o4_merge", "sf_o6"])
+def test_caller_rho_reaches_smoother_and_defect(case):
+    """The smoother and the defect skip the rho load where only zeros can
+    be (points no restriction writes, the whole top level): a rho the caller
+    writes, with oll_set_field or oll_transfer, must still be read at every
+    interior point of every level."""
+    h, s = pair(CASES[case])
+    try:
+        seed_state(h, s, 5)
+        rng = np.random.default_rng(5)
+        top = s.levels - 1
+        for lv in range(s.levels):
+            for ib in s.level_boxes(lv):
+                if lv == top - 1:  # this level's rho comes by a transfer below
+                    continue
+                shape = s.field(ib, O.F_RHO).shape
+                rho = 1e-3 * rng.standard_normal(shape)
+                s.field(ib, O.F_RHO)[:] = rho
+                h.set_field(ib, oa.F_RHO, rho)
+        # rho of the level below the top: u of the top level, Lagrange order 3
+        h.transfer(top, oa.F_U, top - 1, oa.F_RHO, 0, 1, 3, True, True)
+        s.transfer(top, O.F_U, top - 1, O.F_RHO, 0, 1, 3, 1, 1)
+        for lv in range(s.levels):
+            ng = h.smooth(lv, 2)
+            no = max(s.smooth(ib, 2) for ib in s.level_boxes(lv))
+            assert abs(ng - no) <= 1e-11 * no, (case, lv, ng, no)
+            for ib in s.level_boxes(lv):
+                assert rel_err(h.get_field(ib, oa.F_U), s.field(ib, O.F_U)) < 1e-12, (case, lv, ib)
+        # one FAS step from the top: its defect rho - N(u) of the top level
+        # (rho the caller's) is restricted into every coarser level's source
+        ng, no = h.fas(top), s.fasn(top)
+        assert abs(ng - no) <= 1e-9 * no, (case, ng, no)
+        for ib, b in enumerate(s.boxes()):
+            g, o = h.get_field(ib, oa.F_U), s.field(ib, O.F_U)
+            assert np.abs(g - o).max() <= 1e-11 * max(1.0, np.abs(o).max()), (case, ib)
+    finally:
+        h.close()
+        s.close()
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_robin_and_boundary_operator(case):
     """Robin operator on a seeded state (non-zero everywhere on the shell),
