@@ -118,13 +118,18 @@ def test_caller_rho_reaches_smoother_and_defect(case):
             for ib in s.level_boxes(lv):
                 if lv == top - 1:  # this level's rho comes by a transfer below
                     continue
-                shape = s.field(ib, O.F_RHO).shape
-                rho = 1e-3 * rng.standard_normal(shape)
+                # interior points only: the ghost shell's rho is the Robin
+                # constant the boundary stages take
+                R = s.box_info(ib)["R"]
+                rho = s.field(ib, O.F_RHO).copy()
+                inner = rho[R:-R, R:-R, R:-R]
+                inner[:] = 1e-3 * rng.standard_normal(inner.shape)
                 s.field(ib, O.F_RHO)[:] = rho
                 h.set_field(ib, oa.F_RHO, rho)
-        # rho of the level below the top: u of the top level, Lagrange order 3
-        h.transfer(top, oa.F_U, top - 1, oa.F_RHO, 0, 1, 3, True, True)
-        s.transfer(top, O.F_U, top - 1, O.F_RHO, 0, 1, 3, 1, 1)
+        # interior rho of the level below the top: u of the top level,
+        # Lagrange order 3
+        h.transfer(top, oa.F_U, top - 1, oa.F_RHO, 1, 1, 3, True, False)
+        s.transfer(top, O.F_U, top - 1, O.F_RHO, 1, 1, 3, 1, 0)
         for lv in range(s.levels):
             ng = h.smooth(lv, 2)
             no = max(s.smooth(ib, 2) for ib in s.level_boxes(lv))
