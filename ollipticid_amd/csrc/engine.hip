@@ -919,7 +919,7 @@ struct Builder {
     }
   }
 
-  // point blocks, one per row segment, over the region of a refinement
+  // point blocks, patches of whole rows, over the region of a refinement
   // pair's coarse box where a restriction acts: the interior points inside
   // some fine box's non-ghost part (XRef lo, hi), and with `ww` also the
   // nodes the pair's prolongation reads back (u - ww: k_restrict_defect
@@ -948,17 +948,24 @@ struct Builder {
             }
           }
       }
-      const int ilo = lo[0], ihi = hi[0];
-      if (ilo >= ihi || lo[1] >= hi[1] || lo[2] >= hi[2]) continue;
-      const long long plane = (long long)B.n[0] * B.n[1];
-      for (int kk = lo[2]; kk < hi[2]; ++kk)
-        for (int j = lo[1]; j < hi[1]; ++j)
-          for (int i0 = ilo; i0 < ihi; i0 += kPtThreads) {
-            const long long q0 = kk * plane + (long long)j * B.n[0] + i0;
-            out.push_back(PtBlock{H.dbox + f + t, H.dxfer + r.off, r.cnt, 0, unsigned(q0),
-                                  unsigned(std::min(kPtThreads, ihi - i0)), i0, j, kk});
-          }
+      row_patches(H.dbox + f + t, H.dxfer + r.off, r.cnt, B, lo, hi, out);
     }
+  }
+
+  // blocks over the sub-box [lo, hi) of box B: whole rows, as many of one
+  // plane as fill kPtThreads threads (one row a block when a row is longer)
+  static void row_patches(const DevBox *dbox, const DevXfer *src, int nsrc, const DevBox &B,
+                          const int lo[3], const int hi[3], std::vector<PtBlock> &out) {
+    if (lo[0] >= hi[0] || lo[1] >= hi[1] || lo[2] >= hi[2]) return;
+    const int w = hi[0] - lo[0], m = std::max(1, kPtThreads / w);
+    const long long plane = (long long)B.n[0] * B.n[1];
+    for (int kk = lo[2]; kk < hi[2]; ++kk)
+      for (int j = lo[1]; j < hi[1]; j += m) {
+        const int rows = std::min(m, hi[1] - j);
+        const long long q0 = kk * plane + (long long)j * B.n[0] + lo[0];
+        out.push_back(PtBlock{dbox, src, nsrc, 0, unsigned(q0), unsigned(rows * w), lo[0], j, kk, w,
+                              1.0f / float(w)});
+      }
   }
 
   // Blocks per kPtThreads points (or per workgroup of rows) for a launch
@@ -972,8 +979,8 @@ struct Builder {
   }
 
   // v = u - ww on the nodes of the coarse boxes of level lv[h] - 1 that the
-  // prolongation into level lv[h] reads (Hier::prolong_nodes), one block
-  // per row segment
+  // prolongation into level lv[h] reads (Hier::prolong_nodes), in patches
+  // of whole rows
   void corr_v(const std::vector<Hier *> &M, const std::vector<int> &lv) {
     std::vector<PtBlock> items;
     for (size_t h = 0; h < M.size(); ++h) {
@@ -984,15 +991,8 @@ struct Builder {
       for (int b = f; b < f + n; ++b) {
         const std::array<int, 6> &pn = H.prolong_nodes[size_t(b)];
         if (!(pn[0] < pn[3] && pn[1] < pn[4] && pn[2] < pn[5])) continue;
-        const DevBox &B = H.hbox[size_t(b)];
-        const long long plane = (long long)B.n[0] * B.n[1];
-        for (int kk = pn[2]; kk < pn[5]; ++kk)
-          for (int j = pn[1]; j < pn[4]; ++j)
-            for (int i0 = pn[0]; i0 < pn[3]; i0 += kPtThreads) {
-              const long long q0 = kk * plane + (long long)j * B.n[0] + i0;
-              items.push_back(PtBlock{H.dbox + b, nullptr, 0, 0, unsigned(q0),
-                                      unsigned(std::min(kPtThreads, pn[3] - i0)), i0, j, kk});
-            }
+        const int lo[3] = {pn[0], pn[1], pn[2]}, hi[3] = {pn[3], pn[4], pn[5]};
+        row_patches(H.dbox + b, nullptr, 0, H.hbox[size_t(b)], lo, hi, items);
       }
     }
     if (items.empty()) return;
@@ -1008,7 +1008,7 @@ struct Builder {
     // fine level's order (their Lagrange stencil)
     std::map<std::pair<int, int>, std::vector<PtBlock>> groups;
     // point blocks of this stage over all members at one block per
-    // kPtThreads points (row-segment blocks are never joined)
+    // kPtThreads points (row-patch blocks are never joined)
     long long nb = 0;
     size_t members = 0;
     for (size_t h = 0; h < M.size(); ++h) {

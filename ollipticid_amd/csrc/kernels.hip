@@ -1202,9 +1202,16 @@ struct Walk {
 
 // point q0 + off of a block in lexicographic order, stepped from the
 // block's first point (i0, j0, k0) so that only offsets below 2^24 are
-// divided
+// divided; or point off of a block's patch of rows
 __device__ __forceinline__ void pt_lex(const PtBlock &Bk, ConstBox &T, unsigned off, int &i, int &j,
                                        int &k) {
+  if (Bk.w > 0) {  // a patch of rows
+    const unsigned r = fast_div(off, unsigned(Bk.w), Bk.inw);
+    i = Bk.i0 + int(off - r * unsigned(Bk.w));
+    j = Bk.j0 + int(r);
+    k = Bk.k0;
+    return;
+  }
   const unsigned n0 = unsigned(T.n[0]), n1 = unsigned(T.n[1]);
   const unsigned x = unsigned(Bk.i0) + off;
   const unsigned dj = fast_div(x, n0, T.in0);

@@ -291,6 +291,10 @@ struct PtBlock {
   unsigned q0;
   unsigned nq;
   int i0, j0, k0;      // lexicographic blocks: coordinates of point q0
+  // w > 0: a patch of whole rows of w points in plane k0, from (i0, j0):
+  // offset off is point (i0 + off % w, j0 + off / w, k0); inw = 1 / w
+  int w;
+  float inw;
 };
 constexpr int kPtThreads = 256;
 
